@@ -1,0 +1,231 @@
+// Block-level primitives for 256-thread (4 x wave64) workgroups on gfx950:
+// wave scans through DPP-backed shuffles, block scans through one LDS round,
+// and a stable LDS radix sort pass with 4-bit digits.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dte {
+
+constexpr int kWave = 64;
+constexpr int kWaves = 256 / kWave;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint32_t y = __shfl_up(x, off, kWave);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t x) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    uint64_t y = __shfl_up(x, off, kWave);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t x) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    int64_t y = __shfl_up(x, off, kWave);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+// Exclusive scan over the block; s_w holds kWaves words.  Two barriers.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  const uint32_t inc = wave_incl_scan(v);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if (lane == kWave - 1) s_w[w] = inc;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) {
+    const uint32_t s = s_w[i];
+    pre += (i < w) ? s : 0u;
+    tot += s;
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return pre + inc - v;
+}
+
+__device__ __forceinline__ int64_t block_excl_scan_i64(int64_t v, int64_t* s_w, int64_t* total) {
+  const int64_t inc = wave_incl_scan_i64(v);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if (lane == kWave - 1) s_w[w] = inc;
+  __syncthreads();
+  int64_t pre = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) {
+    const int64_t s = s_w[i];
+    pre += (i < w) ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return pre + inc - v;
+}
+
+// Exclusive scan of N lanes of counters at once; s_w holds N * kWaves words.
+template <int N>
+__device__ __forceinline__ void block_excl_scan_vec(uint32_t (&v)[N], uint32_t* s_w,
+                                                     uint32_t (&tot)[N]) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  uint32_t inc[N];
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    inc[c] = wave_incl_scan(v[c]);
+    if (lane == kWave - 1) s_w[c * kWaves + w] = inc[c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    uint32_t pre = 0, t = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) {
+      const uint32_t s = s_w[c * kWaves + i];
+      pre += (i < w) ? s : 0u;
+      t += s;
+    }
+    tot[c] = t;
+    v[c] = pre + inc[c] - v[c];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t block_reduce_max(uint32_t v, uint32_t* s_w) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, kWave));
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if (lane == 0) s_w[w] = v;
+  __syncthreads();
+  uint32_t r = s_w[0];
+#pragma unroll
+  for (int i = 1; i < kWaves; ++i) r = max(r, s_w[i]);
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ uint32_t block_reduce_min(uint32_t v, uint32_t* s_w) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, kWave));
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if (lane == 0) s_w[w] = v;
+  __syncthreads();
+  uint32_t r = s_w[0];
+#pragma unroll
+  for (int i = 1; i < kWaves; ++i) r = min(r, s_w[i]);
+  __syncthreads();
+  return r;
+}
+
+// Best split candidate: larger gain wins, equal gain -> smaller key (lower
+// threshold).  key = UINT32_MAX means "none".
+struct BestCand {
+  double g;
+  uint32_t key;
+  uint32_t prev;
+};
+
+__device__ __forceinline__ bool cand_better(double ga, uint32_t ka, double gb, uint32_t kb) {
+  return ga > gb || (ga == gb && ka < kb);
+}
+
+__device__ __forceinline__ BestCand wave_best(BestCand b) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const double g = __shfl_xor(b.g, off, kWave);
+    const uint32_t k = __shfl_xor(b.key, off, kWave);
+    const uint32_t p = __shfl_xor(b.prev, off, kWave);
+    if (cand_better(g, k, b.g, b.key)) {
+      b.g = g;
+      b.key = k;
+      b.prev = p;
+    }
+  }
+  return b;
+}
+
+// Block reduction of the best candidate; every thread gets the result.
+__device__ __forceinline__ BestCand block_best(BestCand b, double* s_g, uint32_t* s_k, uint32_t* s_p) {
+  b = wave_best(b);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if (lane == 0) {
+    s_g[w] = b.g;
+    s_k[w] = b.key;
+    s_p[w] = b.prev;
+  }
+  __syncthreads();
+  BestCand r{s_g[0], s_k[0], s_p[0]};
+#pragma unroll
+  for (int i = 1; i < kWaves; ++i)
+    if (cand_better(s_g[i], s_k[i], r.g, r.key)) r = BestCand{s_g[i], s_k[i], s_p[i]};
+  __syncthreads();
+  return r;
+}
+
+// One stable LSD pass over n <= 256*IPT (key, val) pairs held in LDS, on the
+// 4-bit digit at `shift`.  Thread t owns positions [t*IPT, t*IPT+IPT).
+// s_cnt: 16 * 256 uint16 counters.  Reads (kin, vin), writes (kout, vout).
+template <int IPT>
+__device__ __forceinline__ void lds_sort_pass(const uint32_t* kin, const uint16_t* vin, uint32_t* kout,
+                                              uint16_t* vout, int n, int shift, uint16_t* s_cnt,
+                                              uint32_t* s_w) {
+  const int t = threadIdx.x;
+  const int lo = t * IPT;
+  uint32_t k[IPT];
+  uint16_t v[IPT];
+  uint32_t d[IPT];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) s_cnt[i * 256 + t] = 0;
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const int p = lo + i;
+    if (p < n) {
+      k[i] = kin[p];
+      v[i] = vin[p];
+      d[i] = (k[i] >> shift) & 15u;
+      s_cnt[d[i] * 256 + t] += 1;
+    }
+  }
+  __syncthreads();
+  // Exclusive scan of the 16 x 256 counter matrix in digit-major order.
+  // Thread t covers flat entries [16t, 16t+16).
+  uint32_t loc[16];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    loc[i] = s_cnt[t * 16 + i];
+    sum += loc[i];
+  }
+  uint32_t base = block_excl_scan(sum, s_w, nullptr);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    s_cnt[t * 16 + i] = (uint16_t)base;
+    base += loc[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    const int p = lo + i;
+    if (p < n) {
+      const uint32_t pos = s_cnt[d[i] * 256 + t];
+      s_cnt[d[i] * 256 + t] = (uint16_t)(pos + 1);
+      kout[pos] = k[i];
+      vout[pos] = v[i];
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace dte

@@ -1,0 +1,2041 @@
+// Level-wise tree-ensemble build for MI355X (gfx950).
+//
+// Replaces the device side of the reference's build (lib_ensembles/source/
+// gpudte_algorithm.cc:14-267 driving gpurf.cu / gpuert.cu / gpudte.cu):
+//
+//   reference                               here
+//   kSetupKernel (cuRAND states)            counter-based Philox, no state
+//   kInitTreeBatch (bootstrap, 1 blk/tree)  k_boot_count + k_boot_compact
+//   kFindSplit (1 64-lane block per node)   k_search_rf / k_search_ert over
+//                                           (node, feature draw) items of all
+//                                           trees of a level
+//   kPerformSplit (atomic slots)            k_partition: stable compaction
+//   host node cache + frame swapping        level arrays resident in HBM
+//
+// Layout in HBM during a fit:
+//   X     fp32 [F][N]    feature-major samples (canonical: -0 -> +0)
+//   K     u32  [F][N]    dense rank of X[f][row] among the distinct values of
+//                        feature f (random forest only)
+//   SV    fp32 [sum S_f] distinct sorted values of each feature
+//   rows  u32  [E]       entries: in-bag rows of every tree, ascending row id
+//                        inside every node (E = sum over trees of distinct
+//                        bootstrap rows), ping-pong A/B between levels
+//   aux   u32  [E]       label << 16 | bootstrap multiplicity
+//   ey    i64  [E]       fixed-point target of the entry (regression)
+//   nodes: level-major SoA (tree, id, depth, count, m, stats, hash, split)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+#include "dte_block.h"
+#include "dte_internal.h"
+
+namespace dte {
+
+__device__ LnTable g_ln_table;
+
+namespace {
+
+constexpr int kIpt = 8;                 // elements per thread in the LDS sort
+constexpr int kCap = kBlock * kIpt;     // LDS chunk capacity (2048)
+constexpr int kHistWords = 8192;        // LDS histogram words (32 KB)
+constexpr int kMaxBuckets = 2048;       // bucket table of the big-item path
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void load_ln_table(double* s_inv, double* s_logc) {
+  const int t = threadIdx.x;
+  if (t < 128) s_inv[t] = g_ln_table.inv[t];
+  else s_logc[t - 128] = g_ln_table.logc[t - 128];
+}
+
+// ------------------------------------------------------------- args -----
+struct DataArgs {
+  const float* X;
+  const uint32_t* K;
+  const float* SV;
+  const int64_t* SVoff;
+  const uint32_t* S;
+  int64_t N;
+  int F;
+  int C;
+  int shift;
+  uint64_t seed;
+  int min_node;
+  int max_depth;
+};
+
+struct LevelArgs {
+  // level node slice (level-local index i)
+  const int32_t* tree;   // batch tree index
+  const uint32_t* id;    // tree-local level-order id
+  const int32_t* count;  // sum of weights
+  const int32_t* m;      // entries
+  const uint32_t* cls;   // class counts [i * C]
+  const int64_t* sum;    // regression fixed-point sum
+  const int64_t* start;  // entry start (current buffer)
+  const uint32_t* open_idx;  // open position -> level index
+  const int32_t* tree_global;
+  const uint32_t* row;   // current entry buffer
+  const uint32_t* aux;
+  const int64_t* ey;
+};
+
+struct ItemArgs {
+  const uint32_t* list;  // optional: item group -> open position
+  int64_t n_groups;
+  int per_group;         // draws per group in this phase
+  int j0;                // first draw index of this phase
+  int D;                 // result stride per open node
+  float* res_score;
+  float* res_thr;
+  int32_t* res_feat;
+  unsigned long long* bytes;  // algorithmic bytes counter (may be NULL)
+  uint32_t* scr_key;     // big-item scratch, one slab per block
+  uint32_t* scr_aux;
+  int64_t* scr_y;
+  int64_t scr_stride;
+};
+
+// ------------------------------------------------- per-node context -----
+struct NodeCtx {
+  int64_t n;        // sum of weights
+  int64_t ssum;     // regression: fixed-point target sum
+  int64_t P;        // classification: T(n) - sum_c T(n_c)
+  double inv;       // 1 / (n ln 2)
+  int F;            // fixed-point shift of the x ln x terms
+  int C;
+};
+
+template <int MAXC, int TASK>
+struct Carry {
+  uint32_t run[MAXC];  // classification: class counts before the chunk
+  int64_t run_n;       // weights before the chunk
+  int64_t run_s;       // regression: target sum before the chunk
+  uint32_t last_key;   // last key before the chunk (valid if run_n > 0)
+  double best_g;
+  uint32_t best_key;
+  uint32_t best_prev;
+};
+
+// Shared scratch of the classification evaluators.
+template <int MAXC>
+struct ClsScratch {
+  uint32_t* pre;     // [MAXC][kBlock] per-thread class counters (columns)
+  uint32_t* ncls;    // [MAXC] node class counts
+  uint32_t* ctot;    // [MAXC] chunk totals
+};
+
+// ------------------------------------------------- chunk evaluation -----
+// Evaluate every boundary of a key-sorted chunk held in LDS.  keys[] are
+// relative to kbase; vals[] index the chunk's payload (s_aux, s_y).
+// Exclusive scan of the per-thread class columns pre[c][t] across the block,
+// plus the carry; chunk totals go to ctot[c].
+template <int MAXC, int TASK>
+__device__ void scan_class_columns(const ClsScratch<MAXC>& cs, int C, Carry<MAXC, TASK>* cr, uint32_t* s_w) {
+  const int t = threadIdx.x;
+  for (int c0 = 0; c0 < C; c0 += 4) {
+    uint32_t v[4], tot[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = (c0 + q < C) ? cs.pre[(c0 + q) * kBlock + t] : 0u;
+    block_excl_scan_vec<4>(v, s_w, tot);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c0 + q < C) {
+        cs.pre[(c0 + q) * kBlock + t] = v[q] + cr->run[c0 + q];
+        if (t == 0) cs.ctot[c0 + q] = tot[q];
+      }
+  }
+}
+
+// Incremental state of one thread's walk: exact fixed-point sums of the left
+// and right x ln x terms.
+struct WalkState {
+  int64_t sl, sr;
+  int64_t nl;
+};
+
+template <int MAXC>
+__device__ __forceinline__ WalkState walk_init(const ClsScratch<MAXC>& cs, const NodeCtx& nc,
+                                               const double* s_inv, const double* s_logc) {
+  const int t = threadIdx.x;
+  WalkState w{0, 0, 0};
+  for (int c = 0; c < nc.C; ++c) {
+    const uint32_t l = cs.pre[c * kBlock + t];
+    w.nl += l;
+    w.sl += xlnx_fixed(l, nc.F, s_inv, s_logc);
+    w.sr += xlnx_fixed(cs.ncls[c] - l, nc.F, s_inv, s_logc);
+  }
+  return w;
+}
+
+__device__ __forceinline__ double walk_gain(const WalkState& w, const NodeCtx& nc, const double* s_inv,
+                                            const double* s_logc) {
+  const int64_t q = (w.sl - xlnx_fixed((uint64_t)w.nl, nc.F, s_inv, s_logc)) +
+                    (w.sr - xlnx_fixed((uint64_t)(nc.n - w.nl), nc.F, s_inv, s_logc));
+  return ig_from_fixed(nc.P + q, nc.F, nc.inv);
+}
+
+template <int MAXC>
+__device__ __forceinline__ void walk_add(WalkState& w, const ClsScratch<MAXC>& cs, int lab, uint32_t add,
+                                         const NodeCtx& nc, const double* s_inv, const double* s_logc) {
+  const int t = threadIdx.x;
+  const uint32_t lo = cs.pre[lab * kBlock + t];
+  const uint32_t ln = lo + add;
+  cs.pre[lab * kBlock + t] = ln;
+  const uint32_t tot = cs.ncls[lab];
+  w.sl += xlnx_fixed(ln, nc.F, s_inv, s_logc) - xlnx_fixed(lo, nc.F, s_inv, s_logc);
+  w.sr += xlnx_fixed(tot - ln, nc.F, s_inv, s_logc) - xlnx_fixed(tot - lo, nc.F, s_inv, s_logc);
+  w.nl += add;
+}
+
+template <int MAXC, int TASK>
+__device__ __forceinline__ void carry_best(Carry<MAXC, TASK>* cr, const BestCand& b) {
+  if (b.key != kNone && cand_better(b.g, b.key, cr->best_g, cr->best_key)) {
+    cr->best_g = b.g;
+    cr->best_key = b.key;
+    cr->best_prev = b.prev;
+  }
+}
+
+// Evaluate every boundary of a key-sorted chunk held in LDS.  keys[] are
+// relative to kbase; vals[] index the chunk's payload (s_aux, s_y).
+template <int MAXC, int TASK>
+__device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, const uint32_t* s_aux,
+                                  const int64_t* s_y, int n, uint32_t kbase, const NodeCtx& nc,
+                                  Carry<MAXC, TASK>* cr, const ClsScratch<MAXC>& cs, const double* s_inv,
+                                  const double* s_logc, uint32_t* s_w, int64_t* s_w64, double* s_bg,
+                                  uint32_t* s_bk, uint32_t* s_bp) {
+  const int t = threadIdx.x;
+  const int lo = t * kIpt;
+  const int hi = min(lo + kIpt, n);
+  BestCand tb{0.0, kNone, kNone};
+  if (TASK == 0) {
+    for (int c = 0; c < nc.C; ++c) cs.pre[c * kBlock + t] = 0;
+    for (int p = lo; p < hi; ++p) {
+      const uint32_t a = s_aux[vals[p]];
+      cs.pre[(a >> 16) * kBlock + t] += a & 0xFFFFu;
+    }
+    scan_class_columns<MAXC, TASK>(cs, nc.C, cr, s_w);
+    if (lo < hi) {
+      WalkState w = walk_init<MAXC>(cs, nc, s_inv, s_logc);
+      uint32_t prevk = (lo > 0) ? keys[lo - 1] + kbase : cr->last_key;
+      for (int p = lo; p < hi; ++p) {
+        const uint32_t k = keys[p] + kbase;
+        if (w.nl > 0 && prevk < k) {
+          const double g = walk_gain(w, nc, s_inv, s_logc);
+          if (g > tb.g) tb = BestCand{g, k, prevk};
+        }
+        const uint32_t a = s_aux[vals[p]];
+        walk_add<MAXC>(w, cs, (int)(a >> 16), a & 0xFFFFu, nc, s_inv, s_logc);
+        prevk = k;
+      }
+    }
+    const BestCand b = block_best(tb, s_bg, s_bk, s_bp);
+    if (t == 0) {
+      carry_best(cr, b);
+      for (int c = 0; c < nc.C; ++c) {
+        cr->run[c] += cs.ctot[c];
+        cr->run_n += cs.ctot[c];
+      }
+      if (n > 0) cr->last_key = keys[n - 1] + kbase;
+    }
+    __syncthreads();
+  } else {
+    int64_t ln = 0, ls = 0;
+    for (int p = lo; p < hi; ++p) {
+      const uint16_t v = vals[p];
+      const int64_t w = (int64_t)(s_aux[v] & 0xFFFFu);
+      ln += w;
+      ls += w * s_y[v];
+    }
+    int64_t tot_n, tot_s;
+    int64_t pn = block_excl_scan_i64(ln, s_w64, &tot_n);
+    int64_t ps = block_excl_scan_i64(ls, s_w64, &tot_s);
+    pn += cr->run_n;
+    ps += cr->run_s;
+    if (lo < hi) {
+      uint32_t prevk = (lo > 0) ? keys[lo - 1] + kbase : cr->last_key;
+      for (int p = lo; p < hi; ++p) {
+        const uint32_t k = keys[p] + kbase;
+        if (pn > 0 && prevk < k) {
+          const double g = variance_gain(pn, ps, nc.n - pn, nc.ssum - ps, 0);
+          if (g > tb.g) tb = BestCand{g, k, prevk};
+        }
+        const uint16_t v = vals[p];
+        const int64_t w = (int64_t)(s_aux[v] & 0xFFFFu);
+        pn += w;
+        ps += w * s_y[v];
+        prevk = k;
+      }
+    }
+    const BestCand b = block_best(tb, s_bg, s_bk, s_bp);
+    if (t == 0) {
+      carry_best(cr, b);
+      cr->run_n += tot_n;
+      cr->run_s += tot_s;
+      if (n > 0) cr->last_key = keys[n - 1] + kbase;
+    }
+    __syncthreads();
+  }
+}
+
+// Evaluate one bucket of W key slots by an LDS histogram (used when the
+// bucket holds more elements than the sort capacity: heavy ties).
+// Classification: hist[slot * C + class]; regression: counts in words
+// [0, W), 64-bit sums from word W on.
+template <int MAXC, int TASK>
+__device__ void eval_hist_bucket(const uint32_t* skey, const uint32_t* saux, const int64_t* sy, int64_t n_el,
+                                 uint32_t kbase, int W, uint32_t* s_hist, const NodeCtx& nc,
+                                 Carry<MAXC, TASK>* cr, const ClsScratch<MAXC>& cs, const double* s_inv,
+                                 const double* s_logc, uint32_t* s_w, int64_t* s_w64, double* s_bg,
+                                 uint32_t* s_bk, uint32_t* s_bp, uint32_t* s_last) {
+  const int t = threadIdx.x;
+  const int C = nc.C;
+  const int zero_words = (TASK == 0) ? W * C : 3 * W;
+  for (int i = t; i < zero_words; i += kBlock) s_hist[i] = 0;
+  __syncthreads();
+  unsigned long long* hsum = (unsigned long long*)(s_hist + W);
+  for (int64_t e = t; e < n_el; e += kBlock) {
+    const uint32_t slot = skey[e] - kbase;
+    const uint32_t a = saux[e];
+    const uint32_t w = a & 0xFFFFu;
+    if (TASK == 0) {
+      atomicAdd(&s_hist[slot * C + (a >> 16)], w);
+    } else {
+      atomicAdd(&s_hist[slot], w);
+      atomicAdd(&hsum[slot], (unsigned long long)((int64_t)w * sy[e]));
+    }
+  }
+  __syncthreads();
+  const int spt = (W + kBlock - 1) / kBlock;
+  const int s0 = t * spt, s1 = min(s0 + spt, W);
+  uint32_t my_last = 0;  // last non-empty slot key + 1 in my range, 0 = none
+  BestCand tb{0.0, kNone, kNone};
+  if (TASK == 0) {
+    for (int c = 0; c < C; ++c) cs.pre[c * kBlock + t] = 0;
+    for (int sl = s0; sl < s1; ++sl) {
+      uint32_t any = 0;
+      for (int c = 0; c < C; ++c) {
+        const uint32_t h = s_hist[sl * C + c];
+        cs.pre[c * kBlock + t] += h;
+        any |= h;
+      }
+      if (any) my_last = kbase + sl + 1;
+    }
+    s_last[t] = my_last;
+    scan_class_columns<MAXC, TASK>(cs, C, cr, s_w);
+    uint32_t prev_last = 0;
+    for (int i = 0; i < t; ++i) prev_last = max(prev_last, s_last[i]);
+    if (s0 < s1) {
+      WalkState w = walk_init<MAXC>(cs, nc, s_inv, s_logc);
+      uint32_t prevk = prev_last ? prev_last - 1 : cr->last_key;
+      for (int sl = s0; sl < s1; ++sl) {
+        uint32_t any = 0;
+        for (int c = 0; c < C; ++c) any |= s_hist[sl * C + c];
+        if (!any) continue;
+        const uint32_t k = kbase + sl;
+        if (w.nl > 0 && prevk < k) {
+          const double g = walk_gain(w, nc, s_inv, s_logc);
+          if (g > tb.g) tb = BestCand{g, k, prevk};
+        }
+        for (int c = 0; c < C; ++c) {
+          const uint32_t h = s_hist[sl * C + c];
+          if (h) walk_add<MAXC>(w, cs, c, h, nc, s_inv, s_logc);
+        }
+        prevk = k;
+      }
+    }
+    const BestCand b = block_best(tb, s_bg, s_bk, s_bp);
+    if (t == 0) {
+      carry_best(cr, b);
+      uint32_t last = 0;
+      for (int i = 0; i < kBlock; ++i) last = max(last, s_last[i]);
+      for (int c = 0; c < C; ++c) {
+        cr->run[c] += cs.ctot[c];
+        cr->run_n += cs.ctot[c];
+      }
+      if (last) cr->last_key = last - 1;
+    }
+    __syncthreads();
+  } else {
+    int64_t ln = 0, ls = 0;
+    for (int sl = s0; sl < s1; ++sl) {
+      const uint32_t hc = s_hist[sl];
+      if (hc) {
+        ln += hc;
+        ls += (int64_t)hsum[sl];
+        my_last = kbase + sl + 1;
+      }
+    }
+    s_last[t] = my_last;
+    int64_t tot_n, tot_s;
+    int64_t pn = block_excl_scan_i64(ln, s_w64, &tot_n);
+    int64_t ps = block_excl_scan_i64(ls, s_w64, &tot_s);
+    uint32_t prev_last = 0;
+    for (int i = 0; i < t; ++i) prev_last = max(prev_last, s_last[i]);
+    pn += cr->run_n;
+    ps += cr->run_s;
+    uint32_t prevk = prev_last ? prev_last - 1 : cr->last_key;
+    for (int sl = s0; sl < s1; ++sl) {
+      const uint32_t hc = s_hist[sl];
+      if (!hc) continue;
+      const uint32_t k = kbase + sl;
+      if (pn > 0 && prevk < k) {
+        const double g = variance_gain(pn, ps, nc.n - pn, nc.ssum - ps, 0);
+        if (g > tb.g) tb = BestCand{g, k, prevk};
+      }
+      pn += hc;
+      ps += (int64_t)hsum[sl];
+      prevk = k;
+    }
+    const BestCand b = block_best(tb, s_bg, s_bk, s_bp);
+    if (t == 0) {
+      carry_best(cr, b);
+      uint32_t last = 0;
+      for (int i = 0; i < kBlock; ++i) last = max(last, s_last[i]);
+      cr->run_n += tot_n;
+      cr->run_s += tot_s;
+      if (last) cr->last_key = last - 1;
+    }
+    __syncthreads();
+  }
+}
+
+// Sort a chunk (keys relative to kbase, span < 2^bits) held in LDS and
+// evaluate it.  On return the sorted order is in buffer `which`.
+template <int MAXC, int TASK>
+__device__ void sort_and_eval(uint32_t* s_k0, uint32_t* s_k1, uint16_t* s_v0, uint16_t* s_v1,
+                              uint16_t* s_cnt, int n, int bits, uint32_t kbase, const uint32_t* s_aux,
+                              const int64_t* s_y, const NodeCtx& nc, Carry<MAXC, TASK>* cr,
+                              const ClsScratch<MAXC>& cs, const double* s_inv, const double* s_logc,
+                              uint32_t* s_w, int64_t* s_w64, double* s_bg, uint32_t* s_bk, uint32_t* s_bp) {
+  uint32_t* ka = s_k0;
+  uint32_t* kb = s_k1;
+  uint16_t* va = s_v0;
+  uint16_t* vb = s_v1;
+  for (int sh = 0; sh < bits; sh += 4) {
+    lds_sort_pass<kIpt>(ka, va, kb, vb, n, sh, s_cnt, s_w);
+    uint32_t* tk = ka;
+    ka = kb;
+    kb = tk;
+    uint16_t* tv = va;
+    va = vb;
+    vb = tv;
+  }
+  eval_sorted_chunk<MAXC, TASK>(ka, va, s_aux, s_y, n, kbase, nc, cr, cs, s_inv, s_logc, s_w, s_w64, s_bg,
+                                s_bk, s_bp);
+}
+
+__device__ __forceinline__ int bit_length(uint32_t v) { return v ? 32 - __clz((int)v) : 0; }
+
+// ------------------------------------------------- RF split search ------
+// One block per (node, feature draw) item, grid-stride over items.  Nodes of
+// up to kCap entries are sorted entirely in LDS; larger nodes are bucketed
+// by dense key into a per-block scratch slab first, then consecutive buckets
+// are packed into LDS-sized chunks (a bucket overfull with ties goes through
+// an LDS histogram).  Every boundary between distinct values is scored.
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, ItemArgs it) {
+  __shared__ double s_inv[128], s_logc[128];
+  __shared__ union alignas(16) {
+    struct {
+      uint32_t k[2][kCap];
+      uint16_t v[2][kCap];
+      uint16_t cnt[16 * kBlock];
+    } srt;
+    uint32_t hist[kHistWords];
+    uint32_t bpos[kMaxBuckets];
+  } u;
+  __shared__ uint32_t s_aux[kCap];
+  __shared__ int64_t s_y[TASK == 1 ? kCap : 1];
+  __shared__ uint32_t s_boff[kMaxBuckets + 1];
+  __shared__ uint32_t s_pre[TASK == 0 ? MAXC * kBlock : 1];
+  __shared__ uint32_t s_ncls[MAXC], s_ctot[MAXC];
+  __shared__ uint32_t s_last[kBlock];
+  __shared__ uint32_t s_w[4 * kWaves + kWaves];
+  __shared__ int64_t s_w64[kWaves];
+  __shared__ double s_bg[kWaves];
+  __shared__ uint32_t s_bk[kWaves], s_bp[kWaves];
+  __shared__ Carry<MAXC, TASK> s_cr;
+
+  load_ln_table(s_inv, s_logc);
+  __syncthreads();
+  const ClsScratch<MAXC> cs{s_pre, s_ncls, s_ctot};
+  const int t = threadIdx.x;
+  const int64_t n_items = it.n_groups * it.per_group;
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int64_t grp = item / it.per_group;
+    const int j = it.j0 + (int)(item % it.per_group);
+    const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
+    const uint32_t i = lv.open_idx[o];
+    const int tr = lv.tree[i];
+    const uint32_t nid = lv.id[i];
+    const int f = (int)(rand_u32(d.seed, kStreamFeat, (uint32_t)lv.tree_global[tr], nid, (uint64_t)j) %
+                        (uint32_t)d.F);
+    const int m = lv.m[i];
+    const int64_t st = lv.start[i];
+    NodeCtx nc;
+    nc.C = d.C;
+    nc.n = lv.count[i];
+    nc.ssum = 0;
+    nc.P = 0;
+    nc.F = 0;
+    nc.inv = 0.0;
+    bool pure = false;
+    if (TASK == 0) {
+      uint32_t mx = 0;
+      for (int c = 0; c < d.C; ++c) mx = max(mx, lv.cls[(int64_t)i * d.C + c]);
+      pure = (int64_t)mx == nc.n;
+      if (t < d.C) s_ncls[t] = lv.cls[(int64_t)i * d.C + t];
+      nc.F = ig_shift((uint64_t)nc.n, d.C, s_inv, s_logc);
+      nc.inv = 1.0 / ((double)nc.n * kLn2);
+      int64_t P = xlnx_fixed((uint64_t)nc.n, nc.F, s_inv, s_logc);
+      for (int c = 0; c < d.C; ++c) P -= xlnx_fixed(lv.cls[(int64_t)i * d.C + c], nc.F, s_inv, s_logc);
+      nc.P = P;
+    } else {
+      nc.ssum = lv.sum[i];
+    }
+    const int64_t ridx = (int64_t)o * it.D + j;
+    if (pure) {
+      // Every boundary of a pure node has gain exactly 0: no positive split.
+      if (t == 0) {
+        it.res_score[ridx] = 0.0f;
+        it.res_thr[ridx] = 0.0f;
+        it.res_feat[ridx] = f;
+      }
+      __syncthreads();
+      continue;
+    }
+    if (t == 0) {
+      for (int c = 0; c < MAXC; ++c) s_cr.run[c] = 0;
+      s_cr.run_n = 0;
+      s_cr.run_s = 0;
+      s_cr.last_key = 0;
+      s_cr.best_g = 0.0;
+      s_cr.best_key = kNone;
+      s_cr.best_prev = kNone;
+      if (it.bytes) atomicAdd(it.bytes, (unsigned long long)m * (TASK == 0 ? 12ull : 20ull));
+    }
+    const uint32_t* Kf = d.K + (int64_t)f * d.N;
+    if (m <= kCap) {
+      // -------- LDS path: gather keys, sort, evaluate.
+      uint32_t kmin = kNone, kmax = 0;
+      for (int e = t; e < m; e += kBlock) {
+        const uint32_t row = lv.row[st + e];
+        const uint32_t key = Kf[row];
+        u.srt.k[0][e] = key;
+        u.srt.v[0][e] = (uint16_t)e;
+        s_aux[e] = lv.aux[st + e];
+        if (TASK == 1) s_y[e] = lv.ey[st + e];
+        kmin = min(kmin, key);
+        kmax = max(kmax, key);
+      }
+      kmin = block_reduce_min(kmin, s_w);
+      kmax = block_reduce_max(kmax, s_w);
+      for (int e = t; e < m; e += kBlock) u.srt.k[0][e] -= kmin;
+      __syncthreads();
+      const int bits = bit_length(kmax - kmin);
+      sort_and_eval<MAXC, TASK>(u.srt.k[0], u.srt.k[1], u.srt.v[0], u.srt.v[1], u.srt.cnt, m, bits, kmin,
+                                s_aux, s_y, nc, &s_cr, cs, s_inv, s_logc, s_w, s_w64, s_bg, s_bk, s_bp);
+    } else {
+      // -------- Big path: bucket by dense key into the scratch slab.
+      const uint32_t S = d.S[f];
+      const int words = (TASK == 0) ? d.C : 3;
+      int wbits = 0;
+      while ((1 << (wbits + 1)) * words <= kHistWords) ++wbits;  // W = 2^wbits slots fit the histogram
+      while (((S + (1u << wbits) - 1) >> wbits) > (uint32_t)kMaxBuckets) ++wbits;
+      const int W = 1 << wbits;
+      const int NB = (int)((S + (uint32_t)W - 1) >> wbits);
+      uint32_t* skey = it.scr_key + (int64_t)blockIdx.x * it.scr_stride;
+      uint32_t* saux = it.scr_aux + (int64_t)blockIdx.x * it.scr_stride;
+      int64_t* sy = TASK == 1 ? it.scr_y + (int64_t)blockIdx.x * it.scr_stride : nullptr;
+      for (int b = t; b <= NB; b += kBlock) s_boff[b] = 0;
+      __syncthreads();
+      for (int e = t; e < m; e += kBlock) atomicAdd(&s_boff[Kf[lv.row[st + e]] >> wbits], 1u);
+      __syncthreads();
+      {
+        // exclusive scan of the NB bucket counts in place
+        const int per = (NB + kBlock - 1) / kBlock;
+        const int b0 = t * per, b1 = min(b0 + per, NB);
+        uint32_t loc = 0;
+        for (int b = b0; b < b1; ++b) loc += s_boff[b];
+        uint32_t base = block_excl_scan(loc, s_w, nullptr);
+        for (int b = b0; b < b1; ++b) {
+          const uint32_t c = s_boff[b];
+          s_boff[b] = base;
+          base += c;
+        }
+        if (t == 0) s_boff[NB] = (uint32_t)m;
+        __syncthreads();
+        for (int b = t; b < NB; b += kBlock) u.bpos[b] = s_boff[b];
+        __syncthreads();
+      }
+      for (int e = t; e < m; e += kBlock) {
+        const uint32_t row = lv.row[st + e];
+        const uint32_t key = Kf[row];
+        const uint32_t pos = atomicAdd(&u.bpos[key >> wbits], 1u);
+        skey[pos] = key;
+        saux[pos] = lv.aux[st + e];
+        if (TASK == 1) sy[pos] = lv.ey[st + e];
+      }
+      __syncthreads();
+      int b = 0;
+      while (b < NB) {
+        const uint32_t cb = s_boff[b + 1] - s_boff[b];
+        if (cb == 0) {
+          ++b;
+          continue;
+        }
+        if (cb > (uint32_t)kCap) {
+          eval_hist_bucket<MAXC, TASK>(skey + s_boff[b], saux + s_boff[b], TASK == 1 ? sy + s_boff[b] : nullptr,
+                                       cb, (uint32_t)b << wbits, W, u.hist, nc, &s_cr, cs, s_inv, s_logc, s_w,
+                                       s_w64, s_bg, s_bk, s_bp, s_last);
+          ++b;
+          continue;
+        }
+        int be = b;
+        uint32_t tot = 0;
+        while (be < NB) {
+          const uint32_t c = s_boff[be + 1] - s_boff[be];
+          if (c > (uint32_t)kCap || tot + c > (uint32_t)kCap) break;
+          tot += c;
+          ++be;
+        }
+        const uint32_t e0 = s_boff[b];
+        const uint32_t kbase = (uint32_t)b << wbits;
+        for (int e = t; e < (int)tot; e += kBlock) {
+          u.srt.k[0][e] = skey[e0 + e] - kbase;
+          u.srt.v[0][e] = (uint16_t)e;
+          s_aux[e] = saux[e0 + e];
+          if (TASK == 1) s_y[e] = sy[e0 + e];
+        }
+        __syncthreads();
+        const int bits = bit_length((uint32_t)(be - b) * (uint32_t)W - 1u);
+        sort_and_eval<MAXC, TASK>(u.srt.k[0], u.srt.k[1], u.srt.v[0], u.srt.v[1], u.srt.cnt, (int)tot, bits,
+                                  kbase, s_aux, s_y, nc, &s_cr, cs, s_inv, s_logc, s_w, s_w64, s_bg, s_bk,
+                                  s_bp);
+        b = be;
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      float score = 0.0f, thr = 0.0f;
+      if (s_cr.best_key != kNone) {
+        const float* sv = d.SV + d.SVoff[f];
+        score = (float)(TASK == 0 ? s_cr.best_g : ldexp_d(s_cr.best_g, -2 * d.shift));
+        thr = midpoint(sv[s_cr.best_prev], sv[s_cr.best_key]);
+      }
+      it.res_score[ridx] = score;
+      it.res_thr[ridx] = thr;
+      it.res_feat[ridx] = f;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------ ERT split search ------
+// One block per item: threshold = fp32 mean of 10 node values drawn with
+// replacement (gpuert.cu:167-183, summed in draw order), then a 2 x C
+// histogram (gpuert.cu:287-301) or per-side target sums (gpuert.cu:348-361).
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_search_ert(DataArgs d, LevelArgs lv, ItemArgs it) {
+  __shared__ double s_inv[128], s_logc[128];
+  __shared__ uint32_t s_left[MAXC];
+  __shared__ unsigned long long s_nl;
+  __shared__ unsigned long long s_sl;
+  __shared__ float s_thr;
+  load_ln_table(s_inv, s_logc);
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int64_t n_items = it.n_groups * it.per_group;
+  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    const int64_t grp = item / it.per_group;
+    const int j = it.j0 + (int)(item % it.per_group);
+    const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
+    const uint32_t i = lv.open_idx[o];
+    const int tr = lv.tree[i];
+    const uint32_t tg = (uint32_t)lv.tree_global[tr];
+    const uint32_t nid = lv.id[i];
+    const int f = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)j) % (uint32_t)d.F);
+    const int m = lv.m[i];
+    const int64_t st = lv.start[i];
+    const float* Xf = d.X + (int64_t)f * d.N;
+    if (t < MAXC) s_left[t] = 0;
+    if (t == 0) {
+      s_nl = 0;
+      s_sl = 0;
+      float acc = 0.0f;
+      for (int s = 0; s < 10; ++s) {
+        const uint32_t pos = rand_u32(d.seed, kStreamThr, tg, nid, (uint64_t)j * 10 + s) % (uint32_t)m;
+        acc = acc + Xf[lv.row[st + pos]];
+      }
+      s_thr = acc / 10.0f;
+      if (it.bytes) atomicAdd(it.bytes, (unsigned long long)m * (TASK == 0 ? 12ull : 20ull));
+    }
+    __syncthreads();
+    const float thr = s_thr;
+    uint32_t loc[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) loc[c] = 0;
+    uint64_t nl = 0;
+    int64_t sl = 0;
+    for (int e = t; e < m; e += kBlock) {
+      const uint32_t row = lv.row[st + e];
+      if (side_of(Xf[row], thr) == 0) {
+        const uint32_t a = lv.aux[st + e];
+        const uint32_t w = a & 0xFFFFu;
+        nl += w;
+        if (TASK == 0) {
+          const int lab = (int)(a >> 16);
+#pragma unroll
+          for (int c = 0; c < MAXC; ++c) loc[c] += (c == lab) ? w : 0u;
+        } else {
+          sl += (int64_t)w * lv.ey[st + e];
+        }
+      }
+    }
+    if (TASK == 0) {
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c) {
+        uint32_t v = loc[c];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+        if ((t & (kWave - 1)) == 0 && v) atomicAdd(&s_left[c], v);
+      }
+    }
+    {
+      uint64_t v = nl;
+      int64_t sm = sl;
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off, kWave);
+        sm += __shfl_xor(sm, off, kWave);
+      }
+      if ((t & (kWave - 1)) == 0) {
+        atomicAdd(&s_nl, (unsigned long long)v);
+        if (TASK == 1) atomicAdd(&s_sl, (unsigned long long)sm);
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      const int64_t n = lv.count[i];
+      const int64_t nlt = (int64_t)s_nl;
+      float score;
+      if (nlt == 0 || nlt == n) {
+        score = -FLT_MAX;
+      } else if (TASK == 0) {
+        const uint32_t* nclsp = lv.cls + (int64_t)i * d.C;
+        const int F = ig_shift((uint64_t)n, d.C, s_inv, s_logc);
+        const double inv = 1.0 / ((double)n * kLn2);
+        int64_t P = xlnx_fixed((uint64_t)n, F, s_inv, s_logc);
+        int64_t sls = 0, srs = 0;
+        for (int c = 0; c < d.C; ++c) {
+          P -= xlnx_fixed(nclsp[c], F, s_inv, s_logc);
+          sls += xlnx_fixed(s_left[c], F, s_inv, s_logc);
+          srs += xlnx_fixed(nclsp[c] - s_left[c], F, s_inv, s_logc);
+        }
+        const int64_t Q = (sls - xlnx_fixed((uint64_t)nlt, F, s_inv, s_logc)) +
+                          (srs - xlnx_fixed((uint64_t)(n - nlt), F, s_inv, s_logc));
+        score = (float)ig_from_fixed(P + Q, F, inv);
+      } else {
+        const int64_t slt = (int64_t)s_sl;
+        score = (float)variance_gain(nlt, slt, n - nlt, lv.sum[i] - slt, d.shift);
+      }
+      const int64_t ridx = (int64_t)o * it.D + j;
+      it.res_score[ridx] = score;
+      it.res_thr[ridx] = thr;
+      it.res_feat[ridx] = f;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------ decide ----------
+__device__ __forceinline__ bool split_better(float sa, int fa, float ta, float sb, int fb, float tb) {
+  if (sa != sb) return sa > sb;
+  if (fa != fb) return fa < fb;
+  return ta < tb;
+}
+
+// Phase 1: best of the mandatory draws (T1 ties); decide whether the
+// retry draws (gpurf.cu:177-182, 242) are needed.  Pure classification
+// nodes score 0 on every draw, so their result is the lowest feature index
+// among all draws, without evaluating the retries.
+struct DecideArgs {
+  int64_t n_open;
+  int D0, R, D;
+  const float* res_score;
+  const float* res_thr;
+  const int32_t* res_feat;
+  float* ch_score;
+  float* ch_thr;
+  int32_t* ch_feat;
+  uint32_t* retry_flag;
+  const uint32_t* retry_list;
+  int64_t n_retry;
+};
+
+__global__ void k_decide1(DataArgs d, LevelArgs lv, DecideArgs a, int algorithm) {
+  for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < a.n_open;
+       o += (int64_t)gridDim.x * blockDim.x) {
+    const float* rs = a.res_score + o * a.D;
+    const float* rt = a.res_thr + o * a.D;
+    const int32_t* rf = a.res_feat + o * a.D;
+    float bs = rs[0], bt = rt[0];
+    int bf = rf[0];
+    for (int j = 1; j < a.D0; ++j)
+      if (split_better(rs[j], rf[j], rt[j], bs, bf, bt)) {
+        bs = rs[j];
+        bf = rf[j];
+        bt = rt[j];
+      }
+    uint32_t need = 0;
+    if (a.R > 0 && (double)bs <= 1e-3) {
+      const uint32_t i = lv.open_idx[o];
+      bool pure = false;
+      if (algorithm == 0 && d.C > 0 && lv.cls) {
+        uint32_t mx = 0;
+        for (int c = 0; c < d.C; ++c) mx = max(mx, lv.cls[(int64_t)i * d.C + c]);
+        pure = (int64_t)mx == (int64_t)lv.count[i];
+      }
+      if (pure) {
+        const uint32_t tg = (uint32_t)lv.tree_global[lv.tree[i]];
+        for (int r = 0; r < a.R; ++r) {
+          const int f = (int)(rand_u32(d.seed, kStreamFeat, tg, lv.id[i], (uint64_t)(a.D0 + r)) % (uint32_t)d.F);
+          if (split_better(0.0f, f, 0.0f, bs, bf, bt)) {
+            bs = 0.0f;
+            bf = f;
+            bt = 0.0f;
+          }
+        }
+      } else {
+        need = 1;
+      }
+    }
+    a.ch_score[o] = bs;
+    a.ch_thr[o] = bt;
+    a.ch_feat[o] = bf;
+    a.retry_flag[o] = need;
+  }
+}
+
+// Phase 2: continue the sequential retry rule on the nodes that needed it.
+__global__ void k_decide2(DecideArgs a) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.n_retry;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t o = a.retry_list[q];
+    float bs = a.ch_score[o], bt = a.ch_thr[o];
+    int bf = a.ch_feat[o];
+    const float* rs = a.res_score + (int64_t)o * a.D;
+    const float* rt = a.res_thr + (int64_t)o * a.D;
+    const int32_t* rf = a.res_feat + (int64_t)o * a.D;
+    for (int r = 0; r < a.R; ++r) {
+      if ((double)bs > 1e-3) break;
+      const int j = a.D0 + r;
+      if (split_better(rs[j], rf[j], rt[j], bs, bf, bt)) {
+        bs = rs[j];
+        bf = rf[j];
+        bt = rt[j];
+      }
+    }
+    a.ch_score[o] = bs;
+    a.ch_thr[o] = bt;
+    a.ch_feat[o] = bf;
+  }
+}
+
+// ------------------------------------------------------ partition -------
+// One block per open node: count the left side of the chosen split, then a
+// stable two-way compaction into the next entry buffer (FIX of the
+// reference's atomic slot order, gpudte.cu:205-232).  Children occupy the
+// parent's range: left [start, start+mL), right [start+mL, start+m).
+struct PartArgs {
+  int64_t n_open;
+  const int32_t* ch_feat;
+  const float* ch_thr;
+  uint32_t* nrow;
+  uint32_t* naux;
+  int64_t* ney;
+  // left child statistics per open node
+  int32_t* l_m;
+  int32_t* l_count;
+  uint32_t* l_cls;
+  int64_t* l_sum;
+  uint64_t* l_hash;
+  uint8_t* split_ok;
+  unsigned long long* bytes;
+};
+
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, PartArgs pa) {
+  __shared__ uint32_t s_w[kWaves];
+  __shared__ uint32_t s_cls[MAXC];
+  __shared__ unsigned long long s_cnt, s_sum, s_hash;
+  __shared__ uint32_t s_ml;
+  const int t = threadIdx.x;
+  for (int64_t o = blockIdx.x; o < pa.n_open; o += gridDim.x) {
+    const uint32_t i = lv.open_idx[o];
+    const int m = lv.m[i];
+    const int64_t st = lv.start[i];
+    const int f = pa.ch_feat[o];
+    const float thr = pa.ch_thr[o];
+    const float* Xf = d.X + (int64_t)f * d.N;
+    if (t < MAXC) s_cls[t] = 0;
+    if (t == 0) {
+      s_cnt = 0;
+      s_sum = 0;
+      s_hash = 0;
+      s_ml = 0;
+    }
+    __syncthreads();
+    uint32_t loc[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) loc[c] = 0;
+    uint64_t cnt = 0, hsh = 0;
+    int64_t sm = 0;
+    uint32_t ml = 0;
+    for (int e = t; e < m; e += kBlock) {
+      const uint32_t row = lv.row[st + e];
+      if (side_of(Xf[row], thr) == 0) {
+        const uint32_t a = lv.aux[st + e];
+        const uint32_t w = a & 0xFFFFu;
+        ++ml;
+        cnt += w;
+        hsh += (uint64_t)w * mix64(row);
+        if (TASK == 0) {
+          const int lab = (int)(a >> 16);
+#pragma unroll
+          for (int c = 0; c < MAXC; ++c) loc[c] += (c == lab) ? w : 0u;
+        } else {
+          sm += (int64_t)w * lv.ey[st + e];
+        }
+      }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      cnt += __shfl_xor(cnt, off, kWave);
+      hsh += __shfl_xor(hsh, off, kWave);
+      sm += __shfl_xor(sm, off, kWave);
+      ml += __shfl_xor(ml, off, kWave);
+    }
+    if (TASK == 0) {
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c) {
+        uint32_t v = loc[c];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+        if ((t & (kWave - 1)) == 0 && v) atomicAdd(&s_cls[c], v);
+      }
+    }
+    if ((t & (kWave - 1)) == 0) {
+      atomicAdd(&s_cnt, (unsigned long long)cnt);
+      atomicAdd(&s_hash, (unsigned long long)hsh);
+      atomicAdd(&s_sum, (unsigned long long)sm);
+      atomicAdd(&s_ml, ml);
+    }
+    __syncthreads();
+    const int mL = (int)s_ml;
+    const bool ok = mL > 0 && mL < m;
+    if (t == 0) {
+      if (pa.bytes) atomicAdd(pa.bytes, (unsigned long long)m * (TASK == 0 ? 12ull : 20ull));
+      pa.split_ok[o] = ok ? 1 : 0;
+      pa.l_m[o] = mL;
+      pa.l_count[o] = (int32_t)s_cnt;
+      pa.l_hash[o] = (uint64_t)s_hash;
+      if (TASK == 1) pa.l_sum[o] = (int64_t)s_sum;
+    }
+    if (TASK == 0 && t < d.C) pa.l_cls[o * d.C + t] = s_cls[t];
+    if (ok) {
+      // Stable scatter, tile by tile.
+      uint32_t lbase = 0, rbase = (uint32_t)mL;
+      for (int e0 = 0; e0 < m; e0 += kBlock) {
+        const int e = e0 + t;
+        uint32_t row = 0, isl = 0;
+        if (e < m) {
+          row = lv.row[st + e];
+          isl = side_of(Xf[row], thr) == 0 ? 1u : 0u;
+        }
+        uint32_t tot;
+        const uint32_t pl = block_excl_scan(isl, s_w, &tot);
+        if (e < m) {
+          const uint32_t pos = isl ? lbase + pl : rbase + (uint32_t)(e - e0) - pl;
+          pa.nrow[st + pos] = row;
+          pa.naux[st + pos] = lv.aux[st + e];
+          if (TASK == 1) pa.ney[st + pos] = lv.ey[st + e];
+        }
+        lbase += tot;
+        rbase += (uint32_t)min(kBlock, m - e0) - tot;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------ children --------
+struct ChildArgs {
+  int64_t n_level;         // nodes at this level
+  int64_t off_level;       // level-major offset of this level
+  int64_t off_next;        // level-major offset of the next level
+  int depth;
+  const uint8_t* open;     // per level node
+  const uint32_t* open_pos;  // level index -> open position (valid if open)
+  const uint8_t* split_ok;   // per open position
+  const int32_t* ch_feat;
+  const float* ch_thr;
+  const float* ch_score;
+  const int32_t* l_m;
+  const int32_t* l_count;
+  const uint32_t* l_cls;
+  const int64_t* l_sum;
+  const uint64_t* l_hash;
+  uint32_t* nchild;        // per level node (scan input)
+  const uint32_t* cpos;    // exclusive scan of nchild (n_level + 1)
+  const uint32_t* lf;      // first level index of each tree (n_trees + 1)
+  const uint32_t* base;    // tree-local id of each tree's first node at this level
+  // output arrays (level-major)
+  int32_t* o_tree;
+  uint32_t* o_id;
+  int32_t* o_depth;
+  int32_t* o_count;
+  int32_t* o_m;
+  uint32_t* o_cls;
+  int64_t* o_sum;
+  uint64_t* o_hash;
+  float* o_value;
+  int32_t* o_feature;
+  float* o_thr;
+  float* o_score;
+  int64_t* o_left;
+  // next level working arrays
+  int64_t* n_start;
+  uint8_t* n_open;
+  const int64_t* start;    // current level starts
+};
+
+__global__ void k_children_count(ChildArgs a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n_level;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t nc = 0;
+    if (a.open[i]) {
+      const uint32_t o = a.open_pos[i];
+      nc = a.split_ok[o] ? 2u : 0u;
+    }
+    a.nchild[i] = nc;
+  }
+}
+
+__global__ void k_children_write(DataArgs d, ChildArgs a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n_level;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t gi = a.off_level + i;
+    if (a.nchild[i] == 0) {
+      a.o_feature[gi] = -1;
+      a.o_thr[gi] = -1.0f;
+      a.o_score[gi] = 0.0f;
+      a.o_left[gi] = -1;
+      continue;
+    }
+    const uint32_t o = a.open_pos[i];
+    const int tr = a.o_tree[gi];
+    a.o_feature[gi] = a.ch_feat[o];
+    a.o_thr[gi] = a.ch_thr[o];
+    a.o_score[gi] = a.ch_score[o];
+    const uint32_t p = a.cpos[i];
+    const int64_t gl = a.off_next + p;
+    a.o_left[gi] = gl;
+    const uint32_t first = a.cpos[a.lf[tr]];
+    const uint32_t idl = a.base[tr] + (a.lf[tr + 1] - a.lf[tr]) + (p - first);
+    const int m = a.o_m[gi];
+    const int mL = a.l_m[o];
+    const int cnt = a.o_count[gi];
+    const int cL = a.l_count[o];
+    const int64_t st = a.start[i];
+    for (int s = 0; s < 2; ++s) {
+      const int64_t g = gl + s;
+      a.o_tree[g] = tr;
+      a.o_id[g] = idl + s;
+      a.o_depth[g] = a.depth + 1;
+      const int cm = s == 0 ? mL : m - mL;
+      const int cc = s == 0 ? cL : cnt - cL;
+      a.o_m[g] = cm;
+      a.o_count[g] = cc;
+      a.o_hash[g] = s == 0 ? a.l_hash[o] : a.o_hash[gi] - a.l_hash[o];
+      if (d.C > 0 && a.o_cls) {
+        for (int c = 0; c < d.C; ++c) {
+          const uint32_t lc = a.l_cls[(int64_t)o * d.C + c];
+          a.o_cls[g * d.C + c] = s == 0 ? lc : a.o_cls[gi * d.C + c] - lc;
+        }
+        a.o_value[g] = 0.0f;
+      } else {
+        const int64_t sm = s == 0 ? a.l_sum[o] : a.o_sum[gi] - a.l_sum[o];
+        a.o_sum[g] = sm;
+        a.o_value[g] = (float)ldexp_d((double)sm / (double)cc, -d.shift);
+      }
+      a.n_start[p + s] = s == 0 ? st : st + mL;
+      a.n_open[p + s] = (cc > d.min_node && a.depth + 1 < d.max_depth) ? 1 : 0;
+    }
+  }
+}
+
+// Per-tree bookkeeping for the next level: first index and base id.
+__global__ void k_tree_update(int n_trees, const uint32_t* cpos, const uint32_t* lf, const uint32_t* base,
+                              uint32_t* lf_next, uint32_t* base_next) {
+  const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tr > n_trees) return;
+  lf_next[tr] = cpos[lf[tr]];
+  if (tr < n_trees) base_next[tr] = base[tr] + (lf[tr + 1] - lf[tr]);
+}
+
+// ------------------------------------------------------ scans -----------
+constexpr int kScanTile = kBlock * 16;
+
+__global__ void k_scan_tiles(const uint32_t* in, int64_t n, uint32_t* tile_sums) {
+  __shared__ uint32_t s_w[kWaves];
+  const int64_t t0 = (int64_t)blockIdx.x * kScanTile;
+  uint32_t s = 0;
+  for (int k = 0; k < 16; ++k) {
+    const int64_t idx = t0 + (int64_t)threadIdx.x * 16 + k;
+    if (idx < n) s += in[idx];
+  }
+  uint32_t tot;
+  block_excl_scan(s, s_w, &tot);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_apply(const uint32_t* in, int64_t n, const uint32_t* tile_off, uint32_t* out) {
+  __shared__ uint32_t s_w[kWaves];
+  const int64_t t0 = (int64_t)blockIdx.x * kScanTile;
+  uint32_t v[16];
+  uint32_t s = 0;
+  for (int k = 0; k < 16; ++k) {
+    const int64_t idx = t0 + (int64_t)threadIdx.x * 16 + k;
+    v[k] = idx < n ? in[idx] : 0u;
+    s += v[k];
+  }
+  uint32_t base = block_excl_scan(s, s_w, nullptr) + tile_off[blockIdx.x];
+  for (int k = 0; k < 16; ++k) {
+    const int64_t idx = t0 + (int64_t)threadIdx.x * 16 + k;
+    if (idx < n) out[idx] = base;
+    base += v[k];
+  }
+  const int64_t last = (int64_t)gridDim.x * kScanTile;
+  (void)last;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kBlock - 1) out[n] = base;
+}
+
+// ---------------------------------------------------- bootstrap ---------
+// Multiplicity of every row in the bootstrap of every tree: draw i of tree t
+// is rand(seed, BOOT, t, 0, i) % N (gpurf.cu:101-113 keyed by (tree, i)).
+__global__ void k_boot_count(uint32_t* mult, int64_t N, int n_trees, const int32_t* tree_global,
+                             uint64_t seed) {
+  const int tr = blockIdx.y;
+  const uint32_t tg = (uint32_t)tree_global[tr];
+  uint32_t* mt = mult + (int64_t)tr * N;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = rand_u32(seed, kStreamBoot, tg, 0, (uint64_t)i) % (uint32_t)N;
+    atomicAdd(&mt[r], 1u);
+  }
+  (void)n_trees;
+}
+
+__global__ void k_nonzero(const uint32_t* mult, int64_t n, uint32_t* flag) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    flag[i] = mult[i] ? 1u : 0u;
+}
+
+// Compact in-bag rows into entries (ascending within each tree) and
+// accumulate each root's statistics.
+__global__ void k_boot_compact(const uint32_t* mult, const uint32_t* pos, int64_t N, const uint32_t* label,
+                               const int64_t* yfix, int C, uint32_t* row, uint32_t* aux, int64_t* ey,
+                               uint32_t* root_cls, int64_t* root_sum, uint64_t* root_hash,
+                               uint32_t* overflow) {
+  const int tr = blockIdx.y;
+  const uint32_t* mt = mult + (int64_t)tr * N;
+  const uint32_t* pt = pos + (int64_t)tr * N;
+  uint64_t hsh = 0;
+  int64_t sm = 0;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t w = mt[r];
+    if (!w) continue;
+    if (w > 0xFFFFu) atomicOr(overflow, 1u);
+    const uint32_t p = pt[r];
+    const uint32_t lab = label ? label[r] : 0u;
+    row[p] = (uint32_t)r;
+    aux[p] = (lab << 16) | (w & 0xFFFFu);
+    if (yfix) {
+      ey[p] = yfix[r];
+      sm += (int64_t)w * yfix[r];
+    } else {
+      atomicAdd(&root_cls[(int64_t)tr * C + lab], w);
+    }
+    hsh += (uint64_t)w * mix64((uint64_t)r);
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    hsh += __shfl_xor(hsh, off, kWave);
+    sm += __shfl_xor(sm, off, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicAdd((unsigned long long*)&root_hash[tr], (unsigned long long)hsh);
+    if (yfix) atomicAdd((unsigned long long*)&root_sum[tr], (unsigned long long)sm);
+  }
+}
+
+// Extra trees: every tree starts from all rows with weight 1 (gpuert.cu:97-102).
+__global__ void k_identity_entries(int64_t N, int n_trees, const uint32_t* label, const int64_t* yfix,
+                                   uint32_t* row, uint32_t* aux, int64_t* ey) {
+  const int tr = blockIdx.y;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = (int64_t)tr * N + r;
+    row[p] = (uint32_t)r;
+    aux[p] = ((label ? label[r] : 0u) << 16) | 1u;
+    if (yfix) ey[p] = yfix[r];
+  }
+  (void)n_trees;
+}
+
+__global__ void k_root_stats_identity(int64_t N, const uint32_t* label, const int64_t* yfix, int C,
+                                      uint32_t* cls1, int64_t* sum1, uint64_t* hash1) {
+  uint64_t hsh = 0;
+  int64_t sm = 0;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
+    hsh += mix64((uint64_t)r);
+    if (yfix) sm += yfix[r];
+    else atomicAdd(&cls1[label[r]], 1u);
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    hsh += __shfl_xor(hsh, off, kWave);
+    sm += __shfl_xor(sm, off, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicAdd((unsigned long long*)hash1, (unsigned long long)hsh);
+    if (yfix) atomicAdd((unsigned long long*)sum1, (unsigned long long)sm);
+  }
+  (void)C;
+}
+
+__global__ void k_canon(float* X, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (X[i] == 0.0f) X[i] = 0.0f;
+}
+
+// ------------------------------------------------------ finalize --------
+struct FinalArgs {
+  int64_t n_nodes;
+  int C;
+  const int32_t* tree;
+  const uint32_t* id;
+  const int64_t* tree_start;
+  const int32_t* feature;
+  const float* thr;
+  const float* score;
+  const int64_t* left;
+  const int32_t* depth;
+  const int32_t* count;
+  const int32_t* m;
+  const uint64_t* hash;
+  const uint32_t* cls;
+  const int64_t* sum;
+  const float* value;
+  int32_t* f_feature;
+  float* f_thr;
+  float* f_score;
+  int32_t* f_left;
+  int32_t* f_depth;
+  int32_t* f_count;
+  int32_t* f_m;
+  uint64_t* f_hash;
+  uint32_t* f_cls;
+  int64_t* f_sum;
+  float* f_value;
+};
+
+__global__ void k_finalize(FinalArgs a) {
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < a.n_nodes;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t pos = a.tree_start[a.tree[g]] + a.id[g];
+    a.f_feature[pos] = a.feature[g];
+    a.f_thr[pos] = a.thr[g];
+    a.f_score[pos] = a.score[g];
+    const int64_t l = a.left[g];
+    a.f_left[pos] = l < 0 ? -1 : (int32_t)(a.tree_start[a.tree[l]] + a.id[l]);
+    a.f_depth[pos] = a.depth[g];
+    a.f_count[pos] = a.count[g];
+    a.f_m[pos] = a.m[g];
+    a.f_hash[pos] = a.hash[g];
+    if (a.cls)
+      for (int c = 0; c < a.C; ++c) a.f_cls[pos * a.C + c] = a.cls[g * a.C + c];
+    a.f_sum[pos] = a.sum ? a.sum[g] : 0;
+    a.f_value[pos] = a.value[g];
+  }
+}
+
+__global__ void k_predict_nodes(int64_t n, int C, int task, const int32_t* feature, const float* thr,
+                                const int32_t* left, const uint32_t* cls, const float* value, PNode* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    PNode p;
+    p.feature = feature[i];
+    p.pad = 0;
+    if (p.feature >= 0) {
+      p.thr = thr[i];
+      p.left = left[i];
+    } else if (task == 0) {
+      // vote = argmax, strict > from 0: lowest class on ties (gpudte.cu:341-351)
+      int best = 0;
+      uint32_t bv = 0;
+      for (int c = 0; c < C; ++c)
+        if (cls[i * C + c] > bv) {
+          bv = cls[i * C + c];
+          best = c;
+        }
+      p.thr = 0.0f;
+      p.left = best;
+    } else {
+      p.thr = value[i];
+      p.left = 0;
+    }
+    out[i] = p;
+  }
+}
+
+// ------------------------------------------------------ small helpers ---
+__global__ void k_open_flags(const uint8_t* open, int64_t n, uint32_t* flag) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    flag[i] = open[i] ? 1u : 0u;
+}
+
+__global__ void k_open_list(const uint8_t* open, const uint32_t* pos, int64_t n, uint32_t* list) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (open[i]) list[pos[i]] = (uint32_t)i;
+}
+
+__global__ void k_compact_list(const uint32_t* flag, const uint32_t* pos, int64_t n, uint32_t* list) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (flag[i]) list[pos[i]] = (uint32_t)i;
+}
+
+__global__ void k_max_m(const int32_t* m, int64_t n, uint32_t* out) {
+  uint32_t v = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v = max(v, (uint32_t)m[i]);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(out, v);
+}
+
+// ------------------------------------------------------ host side -------
+int grid_for(int64_t n, int per_block = kBlock, int cap = 65535 * 4) {
+  int64_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+// Exclusive scan of n u32 into out[0..n] (out[n] = total).
+void scan_u32(const uint32_t* in, int64_t n, uint32_t* out, hipStream_t s, std::vector<DevBuf<uint32_t>>& tmp,
+              int depth = 0) {
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  // Sized once: a resize during the recursion would move the buffers that
+  // outer levels still reference.
+  if (tmp.size() < 16) tmp.resize(16);
+  if (depth >= 7) fail(DTE_ERR_INTERNAL, "scan too deep");
+  DevBuf<uint32_t>& sums = tmp[2 * depth];
+  DevBuf<uint32_t>& offs = tmp[2 * depth + 1];
+  sums.reserve((size_t)tiles + 1);
+  offs.reserve((size_t)tiles + 1);
+  if (n == 0) {
+    DTE_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), s));
+    return;
+  }
+  hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kBlock), 0, s, in, n, sums.p);
+  if (tiles == 1) {
+    DTE_HIP(hipMemsetAsync(offs.p, 0, sizeof(uint32_t), s));
+  } else {
+    scan_u32(sums.p, tiles, offs.p, s, tmp, depth + 1);
+  }
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)tiles), dim3(kBlock), 0, s, in, n, offs.p, out);
+}
+
+template <typename T>
+T read_one(const T* p, hipStream_t s) {
+  T v;
+  DTE_HIP(hipMemcpyAsync(&v, p, sizeof(T), hipMemcpyDeviceToHost, s));
+  DTE_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+  const char* name;
+  double bytes;
+  hipStream_t s;
+  Timer(const char* nm, double by, hipStream_t st) : name(nm), bytes(by), s(st) {
+    if (profiling_enabled()) {
+      DTE_HIP(hipEventCreate(&a));
+      DTE_HIP(hipEventCreate(&b));
+      DTE_HIP(hipEventRecord(a, s));
+    }
+  }
+  ~Timer() {
+    if (a) {
+      (void)hipEventRecord(b, s);
+      profile_record(name, a, b, bytes);
+    }
+  }
+};
+
+template <int MAXC, int TASK>
+void launch_search(int algorithm, int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv,
+                   const ItemArgs& it) {
+  if (algorithm == DTE_RANDOM_FOREST)
+    hipLaunchKernelGGL((k_search_rf<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
+  else
+    hipLaunchKernelGGL((k_search_ert<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
+}
+
+template <int MAXC, int TASK>
+void launch_partition(int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv, const PartArgs& pa) {
+  hipLaunchKernelGGL((k_partition<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, pa);
+}
+
+using SearchFn = void (*)(int, int, hipStream_t, const DataArgs&, const LevelArgs&, const ItemArgs&);
+using PartFn = void (*)(int, hipStream_t, const DataArgs&, const LevelArgs&, const PartArgs&);
+
+SearchFn pick_search(int task, int C) {
+  if (task == 1) return launch_search<1, 1>;
+  if (C <= 2) return launch_search<2, 0>;
+  if (C <= 4) return launch_search<4, 0>;
+  if (C <= 8) return launch_search<8, 0>;
+  if (C <= 16) return launch_search<16, 0>;
+  return launch_search<32, 0>;
+}
+PartFn pick_part(int task, int C) {
+  if (task == 1) return launch_partition<1, 1>;
+  if (C <= 2) return launch_partition<2, 0>;
+  if (C <= 4) return launch_partition<4, 0>;
+  if (C <= 8) return launch_partition<8, 0>;
+  if (C <= 16) return launch_partition<16, 0>;
+  return launch_partition<32, 0>;
+}
+
+// Dense ranks of every feature: K[f][row] = index of X[f][row] among the
+// sorted distinct values SV_f.  Computed on the host from the canonical
+// input (std::sort per feature, one thread per feature slice).
+void dense_ranks_host(const std::vector<float>& Xh, int64_t N, int F, std::vector<uint32_t>& K,
+                      std::vector<float>& SV, std::vector<int64_t>& SVoff, std::vector<uint32_t>& S) {
+  K.resize((size_t)N * F);
+  S.resize(F);
+  std::vector<std::vector<float>> sv(F);
+  int nt = (int)std::max(1u, std::thread::hardware_concurrency());
+  nt = std::min(nt, 32);
+  std::vector<std::thread> pool;
+  for (int w = 0; w < nt; ++w) {
+    pool.emplace_back([&, w]() {
+      std::vector<std::pair<float, uint32_t>> tmp((size_t)N);
+      for (int f = w; f < F; f += nt) {
+        const float* col = Xh.data() + (size_t)f * N;
+        for (int64_t r = 0; r < N; ++r) tmp[r] = {col[r], (uint32_t)r};
+        std::sort(tmp.begin(), tmp.end(),
+                  [](const std::pair<float, uint32_t>& a, const std::pair<float, uint32_t>& b) {
+                    return a.first < b.first;
+                  });
+        uint32_t rank = 0;
+        std::vector<float>& out = sv[f];
+        out.clear();
+        for (int64_t i = 0; i < N; ++i) {
+          if (i > 0 && tmp[i - 1].first < tmp[i].first) ++rank;
+          if (i == 0 || tmp[i - 1].first < tmp[i].first) out.push_back(tmp[i].first);
+          K[(size_t)f * N + tmp[i].second] = rank;
+        }
+        S[f] = (uint32_t)out.size();
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  SVoff.resize(F);
+  int64_t off = 0;
+  for (int f = 0; f < F; ++f) {
+    SVoff[f] = off;
+    off += S[f];
+  }
+  SV.resize((size_t)off);
+  for (int f = 0; f < F; ++f) std::copy(sv[f].begin(), sv[f].end(), SV.begin() + SVoff[f]);
+}
+
+}  // namespace
+
+const LnTable& host_ln_table() {
+  static LnTable t = [] {
+    LnTable x;
+    build_ln_table(&x);
+    return x;
+  }();
+  return t;
+}
+
+void upload_ln_table(int device) {
+  (void)device;
+  const LnTable& t = host_ln_table();
+  DTE_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_ln_table), &t, sizeof(LnTable), 0, hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------------ Builder ---
+Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_t F, const dte_params& p,
+                   hipStream_t s) {
+  const int task = p.task;
+  const int algorithm = p.algorithm;
+  const int C = task == 0 ? p.n_classes : 1;
+  const int T = p.n_trees;
+  const int k = p.n_features_per_node > 0 ? p.n_features_per_node
+                                          : (int)std::round(std::log((double)F)) + 1;
+  const int D0 = std::min(k, F);
+  const int R = k < F ? std::min(10, F - k) : 0;
+  const int D = D0 + R;
+  if (C > 32) fail(DTE_ERR_INVALID_ARGUMENT, "at most 32 classes are supported");
+  if (N > (int64_t)0x7FFFFFFF) fail(DTE_ERR_INVALID_ARGUMENT, "too many rows");
+
+  // ---- dataset: canonical copy of X, labels / fixed-point targets.
+  DevBuf<float> X((size_t)N * F);
+  DTE_HIP(hipMemcpyAsync(X.p, X_dev_in, sizeof(float) * N * F, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_canon, dim3(grid_for((int64_t)N * F)), dim3(kBlock), 0, s, X.p, (int64_t)N * F);
+  DevBuf<uint32_t> label;
+  DevBuf<int64_t> yfix;
+  int shift = 0;
+  if (task == 0) {
+    std::vector<uint32_t> lab(N);
+    for (int64_t i = 0; i < N; ++i) {
+      const int c = (int)y_host[i];
+      if (c < 0 || c >= C || (float)c != y_host[i]) fail(DTE_ERR_INVALID_ARGUMENT, "class label out of range");
+      lab[i] = (uint32_t)c;
+    }
+    label.alloc(N);
+    DTE_HIP(hipMemcpyAsync(label.p, lab.data(), 4 * N, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipStreamSynchronize(s));
+  } else {
+    double mx = 0;
+    for (int64_t i = 0; i < N; ++i) mx = std::max(mx, std::fabs((double)y_host[i]));
+    shift = fixed_shift(mx, N);
+    std::vector<int64_t> yf(N);
+    for (int64_t i = 0; i < N; ++i) yf[i] = std::llrint(std::ldexp((double)y_host[i], shift));
+    yfix.alloc(N);
+    DTE_HIP(hipMemcpyAsync(yfix.p, yf.data(), 8 * N, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipStreamSynchronize(s));
+  }
+  DevBuf<uint32_t> K;
+  DevBuf<float> SV;
+  DevBuf<int64_t> SVoff;
+  DevBuf<uint32_t> Sdev;
+  if (algorithm == DTE_RANDOM_FOREST) {
+    std::vector<float> Xh((size_t)N * F);
+    DTE_HIP(hipMemcpyAsync(Xh.data(), X.p, sizeof(float) * N * F, hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> Kh, Sh;
+    std::vector<float> SVh;
+    std::vector<int64_t> offh;
+    dense_ranks_host(Xh, N, F, Kh, SVh, offh, Sh);
+    K.alloc(Kh.size());
+    SV.alloc(SVh.size());
+    SVoff.alloc(F);
+    Sdev.alloc(F);
+    DTE_HIP(hipMemcpyAsync(K.p, Kh.data(), 4 * Kh.size(), hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(SV.p, SVh.data(), 4 * SVh.size(), hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(SVoff.p, offh.data(), 8 * F, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(Sdev.p, Sh.data(), 4 * F, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipStreamSynchronize(s));
+    for (int f = 0; f < F; ++f) {
+      int words = task == 0 ? C : 3;
+      int wbits = 0;
+      while ((1 << (wbits + 1)) * words <= kHistWords) ++wbits;
+      while (((Sh[f] + (1u << wbits) - 1) >> wbits) > (uint32_t)kMaxBuckets) ++wbits;
+      if ((1 << wbits) * words > kHistWords)
+        fail(DTE_ERR_INVALID_ARGUMENT, "too many distinct values in a feature for the split-search buckets");
+    }
+  }
+  DataArgs d{};
+  d.X = X.p;
+  d.K = K.p;
+  d.SV = SV.p;
+  d.SVoff = SVoff.p;
+  d.S = Sdev.p;
+  d.N = N;
+  d.F = F;
+  d.C = task == 0 ? C : 0;
+  d.shift = shift;
+  d.seed = p.seed;
+  d.min_node = p.min_node_size;
+  d.max_depth = p.max_depth;
+
+  std::vector<int32_t> tg_h(T);
+  std::iota(tg_h.begin(), tg_h.end(), 0);
+  DevBuf<int32_t> tree_global(T);
+  DTE_HIP(hipMemcpyAsync(tree_global.p, tg_h.data(), 4 * T, hipMemcpyHostToDevice, s));
+
+  std::vector<DevBuf<uint32_t>> scan_tmp;
+  DevBuf<unsigned long long> search_bytes(1), part_bytes(1);
+  DTE_HIP(hipMemsetAsync(search_bytes.p, 0, 8, s));
+  DTE_HIP(hipMemsetAsync(part_bytes.p, 0, 8, s));
+
+  // ---- root entries.
+  int64_t E = 0;
+  std::vector<int64_t> root_start(T), root_m(T);
+  DevBuf<uint32_t> rowA, auxA, rowB, auxB;
+  DevBuf<int64_t> eyA, eyB;
+  DevBuf<uint32_t> root_cls((size_t)T * std::max(C, 1));
+  DevBuf<int64_t> root_sum(T);
+  DevBuf<uint64_t> root_hash(T);
+  DTE_HIP(hipMemsetAsync(root_cls.p, 0, root_cls.bytes(), s));
+  DTE_HIP(hipMemsetAsync(root_sum.p, 0, root_sum.bytes(), s));
+  DTE_HIP(hipMemsetAsync(root_hash.p, 0, root_hash.bytes(), s));
+  if (algorithm == DTE_RANDOM_FOREST) {
+    const int64_t TN = (int64_t)T * N;
+    DevBuf<uint32_t> mult(TN), flag(TN), pos(TN + 1);
+    DTE_HIP(hipMemsetAsync(mult.p, 0, mult.bytes(), s));
+    {
+      Timer tm("bootstrap", 4.0 * TN, s);
+      hipLaunchKernelGGL(k_boot_count, dim3(grid_for(N, kBlock, 4096), T), dim3(kBlock), 0, s, mult.p, N, T,
+                         tree_global.p, p.seed);
+    }
+    hipLaunchKernelGGL(k_nonzero, dim3(grid_for(TN)), dim3(kBlock), 0, s, mult.p, TN, flag.p);
+    scan_u32(flag.p, TN, pos.p, s, scan_tmp);
+    std::vector<uint32_t> starts(T + 1);
+    for (int t = 0; t <= T; ++t) {
+      DTE_HIP(hipMemcpyAsync(&starts[t], pos.p + (int64_t)t * N, 4, hipMemcpyDeviceToHost, s));
+    }
+    DTE_HIP(hipStreamSynchronize(s));
+    E = starts[T];
+    for (int t = 0; t < T; ++t) {
+      root_start[t] = starts[t];
+      root_m[t] = starts[t + 1] - starts[t];
+    }
+    rowA.alloc(E);
+    auxA.alloc(E);
+    rowB.alloc(E);
+    auxB.alloc(E);
+    if (task == 1) {
+      eyA.alloc(E);
+      eyB.alloc(E);
+    }
+    DevBuf<uint32_t> overflow(1);
+    DTE_HIP(hipMemsetAsync(overflow.p, 0, 4, s));
+    hipLaunchKernelGGL(k_boot_compact, dim3(grid_for(N, kBlock, 4096), T), dim3(kBlock), 0, s, mult.p, pos.p, N,
+                       label.p, yfix.p, C, rowA.p, auxA.p, eyA.p, root_cls.p, root_sum.p, root_hash.p,
+                       overflow.p);
+    if (read_one(overflow.p, s)) fail(DTE_ERR_INTERNAL, "bootstrap multiplicity above 65535");
+  } else {
+    E = (int64_t)T * N;
+    for (int t = 0; t < T; ++t) {
+      root_start[t] = (int64_t)t * N;
+      root_m[t] = N;
+    }
+    rowA.alloc(E);
+    auxA.alloc(E);
+    rowB.alloc(E);
+    auxB.alloc(E);
+    if (task == 1) {
+      eyA.alloc(E);
+      eyB.alloc(E);
+    }
+    hipLaunchKernelGGL(k_identity_entries, dim3(grid_for(N, kBlock, 4096), T), dim3(kBlock), 0, s, N, T, label.p,
+                       yfix.p, rowA.p, auxA.p, eyA.p);
+    DevBuf<uint32_t> c1(std::max(C, 1));
+    DevBuf<int64_t> s1(1);
+    DevBuf<uint64_t> h1(1);
+    DTE_HIP(hipMemsetAsync(c1.p, 0, c1.bytes(), s));
+    DTE_HIP(hipMemsetAsync(s1.p, 0, 8, s));
+    DTE_HIP(hipMemsetAsync(h1.p, 0, 8, s));
+    hipLaunchKernelGGL(k_root_stats_identity, dim3(grid_for(N, kBlock, 4096)), dim3(kBlock), 0, s, N, label.p,
+                       yfix.p, C, c1.p, s1.p, h1.p);
+    std::vector<uint32_t> c1h(std::max(C, 1));
+    int64_t s1h;
+    uint64_t h1h;
+    DTE_HIP(hipMemcpyAsync(c1h.data(), c1.p, c1.bytes(), hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipMemcpyAsync(&s1h, s1.p, 8, hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipMemcpyAsync(&h1h, h1.p, 8, hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> rc((size_t)T * std::max(C, 1));
+    std::vector<int64_t> rs(T, s1h);
+    std::vector<uint64_t> rh(T, h1h);
+    for (int t = 0; t < T; ++t)
+      for (int c = 0; c < std::max(C, 1); ++c) rc[(size_t)t * std::max(C, 1) + c] = c1h[c];
+    DTE_HIP(hipMemcpyAsync(root_cls.p, rc.data(), 4 * rc.size(), hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(root_sum.p, rs.data(), 8 * T, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(root_hash.p, rh.data(), 8 * T, hipMemcpyHostToDevice, s));
+  }
+
+  // ---- level-major node arrays (capacity: a tree has at most 2m-1 nodes).
+  const int64_t cap = 2 * E + T;
+  DevBuf<int32_t> o_tree(cap), o_depth(cap), o_count(cap), o_m(cap), o_feature(cap);
+  DevBuf<uint32_t> o_id(cap);
+  DevBuf<uint32_t> o_cls(task == 0 ? (size_t)cap * C : 1);
+  DevBuf<int64_t> o_sum(task == 1 ? cap : 1);
+  DevBuf<uint64_t> o_hash(cap);
+  DevBuf<float> o_value(cap), o_thr(cap), o_score(cap);
+  DevBuf<int64_t> o_left(cap);
+
+  // roots
+  {
+    std::vector<int32_t> tr(T), dep(T, 0), cnt(T), mm(T);
+    std::vector<uint32_t> idv(T, 0);
+    std::vector<int64_t> st(T);
+    std::vector<uint8_t> op(T);
+    for (int t = 0; t < T; ++t) {
+      tr[t] = t;
+      cnt[t] = (int32_t)N;
+      mm[t] = (int32_t)root_m[t];
+      st[t] = root_start[t];
+      op[t] = (N > p.min_node_size && 0 < p.max_depth) ? 1 : 0;
+    }
+    DTE_HIP(hipMemcpyAsync(o_tree.p, tr.data(), 4 * T, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(o_depth.p, dep.data(), 4 * T, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(o_count.p, cnt.data(), 4 * T, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(o_m.p, mm.data(), 4 * T, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(o_id.p, idv.data(), 4 * T, hipMemcpyHostToDevice, s));
+    if (task == 0)
+      DTE_HIP(hipMemcpyAsync(o_cls.p, root_cls.p, 4 * (size_t)T * C, hipMemcpyDeviceToDevice, s));
+    else
+      DTE_HIP(hipMemcpyAsync(o_sum.p, root_sum.p, 8 * (size_t)T, hipMemcpyDeviceToDevice, s));
+    DTE_HIP(hipMemcpyAsync(o_hash.p, root_hash.p, 8 * (size_t)T, hipMemcpyDeviceToDevice, s));
+    // root value (regression mean)
+    if (task == 1) {
+      std::vector<int64_t> rs(T);
+      DTE_HIP(hipMemcpyAsync(rs.data(), root_sum.p, 8 * T, hipMemcpyDeviceToHost, s));
+      DTE_HIP(hipStreamSynchronize(s));
+      std::vector<float> rv(T);
+      for (int t = 0; t < T; ++t) rv[t] = (float)std::ldexp((double)rs[t] / (double)N, -shift);
+      DTE_HIP(hipMemcpyAsync(o_value.p, rv.data(), 4 * T, hipMemcpyHostToDevice, s));
+    } else {
+      DTE_HIP(hipMemsetAsync(o_value.p, 0, 4 * T, s));
+    }
+    // working arrays of level 0
+  }
+  DevBuf<int64_t> w_start(T), w_start_next;
+  DevBuf<uint8_t> w_open(T), w_open_next;
+  {
+    std::vector<int64_t> st(root_start);
+    std::vector<uint8_t> op(T);
+    for (int t = 0; t < T; ++t) op[t] = (N > p.min_node_size && 0 < p.max_depth) ? 1 : 0;
+    DTE_HIP(hipMemcpyAsync(w_start.p, st.data(), 8 * T, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(w_open.p, op.data(), T, hipMemcpyHostToDevice, s));
+  }
+  DevBuf<uint32_t> lf(T + 1), base(T), lf_next(T + 1), base_next(T);
+  {
+    std::vector<uint32_t> l(T + 1), b(T, 0);
+    for (int t = 0; t <= T; ++t) l[t] = t;
+    DTE_HIP(hipMemcpyAsync(lf.p, l.data(), 4 * (T + 1), hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(base.p, b.data(), 4 * T, hipMemcpyHostToDevice, s));
+  }
+
+  // per-level scratch
+  DevBuf<uint32_t> flag32, open_pos, open_idx, nchild, cpos, retry_flag, retry_pos, retry_list;
+  DevBuf<float> res_score, res_thr, ch_score, ch_thr;
+  DevBuf<int32_t> res_feat, ch_feat, l_m, l_count;
+  DevBuf<uint32_t> l_cls;
+  DevBuf<int64_t> l_sum;
+  DevBuf<uint64_t> l_hash;
+  DevBuf<uint8_t> split_ok;
+  DevBuf<uint32_t> scr_key, scr_aux;
+  DevBuf<int64_t> scr_y;
+  int64_t scr_stride = 0;
+
+  int num_cu = 256;
+  {
+    int dev;
+    DTE_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    DTE_HIP(hipGetDeviceProperties(&prop, dev));
+    num_cu = prop.multiProcessorCount;
+  }
+  const int search_grid_cap = num_cu * 4;
+
+  SearchFn search = pick_search(task, C);
+  PartFn partition = pick_part(task, C);
+
+  int64_t off_level = 0;
+  int64_t n_level = T;
+  int64_t max_m = 0;
+  for (int t = 0; t < T; ++t) max_m = std::max(max_m, root_m[t]);
+
+  uint32_t* cur_row = rowA.p;
+  uint32_t* cur_aux = auxA.p;
+  int64_t* cur_ey = eyA.p;
+  uint32_t* nxt_row = rowB.p;
+  uint32_t* nxt_aux = auxB.p;
+  int64_t* nxt_ey = eyB.p;
+
+  for (int depth = 0; n_level > 0; ++depth) {
+    // ---- open list
+    flag32.reserve(n_level);
+    open_pos.reserve(n_level + 1);
+    hipLaunchKernelGGL(k_open_flags, dim3(grid_for(n_level)), dim3(kBlock), 0, s, w_open.p,
+                       n_level, flag32.p);
+    scan_u32(flag32.p, n_level, open_pos.p, s, scan_tmp);
+    const int64_t n_open = read_one(open_pos.p + n_level, s);
+    open_idx.reserve(n_open + 1);
+    if (n_open > 0)
+      hipLaunchKernelGGL(k_open_list, dim3(grid_for(n_level)), dim3(kBlock), 0, s, w_open.p,
+                         open_pos.p, n_level, open_idx.p);
+
+    LevelArgs lv{};
+    lv.tree = o_tree.p + off_level;
+    lv.id = o_id.p + off_level;
+    lv.count = o_count.p + off_level;
+    lv.m = o_m.p + off_level;
+    lv.cls = task == 0 ? o_cls.p + off_level * C : nullptr;
+    lv.sum = task == 1 ? o_sum.p + off_level : nullptr;
+    lv.start = w_start.p;
+    lv.open_idx = open_idx.p;
+    lv.tree_global = tree_global.p;
+    lv.row = cur_row;
+    lv.aux = cur_aux;
+    lv.ey = cur_ey;
+
+    if (n_open > 0) {
+      res_score.reserve(n_open * D);
+      res_thr.reserve(n_open * D);
+      res_feat.reserve(n_open * D);
+      ch_score.reserve(n_open);
+      ch_thr.reserve(n_open);
+      ch_feat.reserve(n_open);
+      retry_flag.reserve(n_open);
+      retry_pos.reserve(n_open + 1);
+      retry_list.reserve(n_open + 1);
+      if (algorithm == DTE_RANDOM_FOREST && max_m > kCap) {
+        const int64_t need = (int64_t)search_grid_cap * max_m;
+        if (scr_stride < max_m) {
+          scr_key.alloc(need);
+          scr_aux.alloc(need);
+          if (task == 1) scr_y.alloc(need);
+          scr_stride = max_m;
+        }
+      }
+      ItemArgs it{};
+      it.list = nullptr;
+      it.n_groups = n_open;
+      it.per_group = D0;
+      it.j0 = 0;
+      it.D = D;
+      it.res_score = res_score.p;
+      it.res_thr = res_thr.p;
+      it.res_feat = res_feat.p;
+      it.bytes = search_bytes.p;
+      it.scr_key = scr_key.p;
+      it.scr_aux = scr_aux.p;
+      it.scr_y = scr_y.p;
+      it.scr_stride = scr_stride;
+      {
+        Timer tm("split_search", 0.0, s);
+        search(algorithm, (int)std::min<int64_t>(n_open * D0, search_grid_cap), s, d, lv, it);
+      }
+      DecideArgs da{};
+      da.n_open = n_open;
+      da.D0 = D0;
+      da.R = R;
+      da.D = D;
+      da.res_score = res_score.p;
+      da.res_thr = res_thr.p;
+      da.res_feat = res_feat.p;
+      da.ch_score = ch_score.p;
+      da.ch_thr = ch_thr.p;
+      da.ch_feat = ch_feat.p;
+      da.retry_flag = retry_flag.p;
+      hipLaunchKernelGGL(k_decide1, dim3(grid_for(n_open)), dim3(kBlock), 0, s, d, lv, da, algorithm);
+      if (R > 0) {
+        scan_u32(retry_flag.p, n_open, retry_pos.p, s, scan_tmp);
+        const int64_t n_retry = read_one(retry_pos.p + n_open, s);
+        if (n_retry > 0) {
+          hipLaunchKernelGGL(k_compact_list, dim3(grid_for(n_open)), dim3(kBlock), 0, s,
+                             retry_flag.p, retry_pos.p, n_open, retry_list.p);
+          it.list = retry_list.p;
+          it.n_groups = n_retry;
+          it.per_group = R;
+          it.j0 = D0;
+          {
+            Timer tm("split_search", 0.0, s);
+            search(algorithm, (int)std::min<int64_t>(n_retry * R, search_grid_cap), s, d, lv, it);
+          }
+          da.retry_list = retry_list.p;
+          da.n_retry = n_retry;
+          hipLaunchKernelGGL(k_decide2, dim3(grid_for(n_retry)), dim3(kBlock), 0, s, da);
+        }
+      }
+      // ---- partition
+      l_m.reserve(n_open);
+      l_count.reserve(n_open);
+      l_cls.reserve(task == 0 ? n_open * C : 1);
+      l_sum.reserve(n_open);
+      l_hash.reserve(n_open);
+      split_ok.reserve(n_open);
+      PartArgs pa{};
+      pa.n_open = n_open;
+      pa.ch_feat = ch_feat.p;
+      pa.ch_thr = ch_thr.p;
+      pa.nrow = nxt_row;
+      pa.naux = nxt_aux;
+      pa.ney = nxt_ey;
+      pa.l_m = l_m.p;
+      pa.l_count = l_count.p;
+      pa.l_cls = l_cls.p;
+      pa.l_sum = l_sum.p;
+      pa.l_hash = l_hash.p;
+      pa.split_ok = split_ok.p;
+      pa.bytes = part_bytes.p;
+      {
+        Timer tm("partition", 0.0, s);
+        partition((int)std::min<int64_t>(n_open, search_grid_cap), s, d, lv, pa);
+      }
+    }
+
+    // ---- children
+    nchild.reserve(n_level);
+    cpos.reserve(n_level + 1);
+    ChildArgs ca{};
+    ca.n_level = n_level;
+    ca.off_level = off_level;
+    ca.off_next = off_level + n_level;
+    ca.depth = depth;
+    ca.open = w_open.p;
+    ca.open_pos = open_pos.p;
+    ca.split_ok = split_ok.p;
+    ca.ch_feat = ch_feat.p;
+    ca.ch_thr = ch_thr.p;
+    ca.ch_score = ch_score.p;
+    ca.l_m = l_m.p;
+    ca.l_count = l_count.p;
+    ca.l_cls = l_cls.p;
+    ca.l_sum = l_sum.p;
+    ca.l_hash = l_hash.p;
+    ca.nchild = nchild.p;
+    ca.cpos = cpos.p;
+    ca.lf = lf.p;
+    ca.base = base.p;
+    ca.o_tree = o_tree.p;
+    ca.o_id = o_id.p;
+    ca.o_depth = o_depth.p;
+    ca.o_count = o_count.p;
+    ca.o_m = o_m.p;
+    ca.o_cls = task == 0 ? o_cls.p : nullptr;
+    ca.o_sum = o_sum.p;
+    ca.o_hash = o_hash.p;
+    ca.o_value = o_value.p;
+    ca.o_feature = o_feature.p;
+    ca.o_thr = o_thr.p;
+    ca.o_score = o_score.p;
+    ca.o_left = o_left.p;
+    ca.start = w_start.p;
+    if (n_open == 0) {
+      DTE_HIP(hipMemsetAsync(nchild.p, 0, 4 * n_level, s));
+    } else {
+      hipLaunchKernelGGL(k_children_count, dim3(grid_for(n_level)), dim3(kBlock), 0, s, ca);
+    }
+    scan_u32(nchild.p, n_level, cpos.p, s, scan_tmp);
+    const int64_t n_next = read_one(cpos.p + n_level, s);
+    w_start_next.reserve(n_next + 1);
+    w_open_next.reserve(n_next + 1);
+    ca.n_start = w_start_next.p;
+    ca.n_open = w_open_next.p;
+    hipLaunchKernelGGL(k_children_write, dim3(grid_for(n_level)), dim3(kBlock), 0, s, d, ca);
+    hipLaunchKernelGGL(k_tree_update, dim3(grid_for(T + 1)), dim3(kBlock), 0, s, T, cpos.p, lf.p, base.p,
+                       lf_next.p, base_next.p);
+    std::swap(lf, lf_next);
+    std::swap(base, base_next);
+    std::swap(w_start, w_start_next);
+    std::swap(w_open, w_open_next);
+    std::swap(cur_row, nxt_row);
+    std::swap(cur_aux, nxt_aux);
+    std::swap(cur_ey, nxt_ey);
+    off_level += n_level;
+    n_level = n_next;
+    // largest node of the next level (for scratch sizing)
+    if (n_level > 0 && algorithm == DTE_RANDOM_FOREST) {
+      DevBuf<uint32_t> mx(1);
+      DTE_HIP(hipMemsetAsync(mx.p, 0, 4, s));
+      hipLaunchKernelGGL(k_max_m, dim3(grid_for(n_level)), dim3(kBlock), 0, s,
+                         o_m.p + off_level, n_level, mx.p);
+      max_m = read_one(mx.p, s);
+    }
+  }
+  const int64_t n_nodes = off_level;
+  if (profiling_enabled()) {
+    profile_add_bytes("split_search", (double)read_one(search_bytes.p, s));
+    profile_add_bytes("partition", (double)read_one(part_bytes.p, s));
+  }
+
+  // ---- canonical tree-major arrays
+  std::vector<uint32_t> per_tree(T);
+  DTE_HIP(hipMemcpyAsync(per_tree.data(), base.p, 4 * T, hipMemcpyDeviceToHost, s));
+  DTE_HIP(hipStreamSynchronize(s));
+  std::vector<int64_t> ts(T + 1, 0);
+  for (int t = 0; t < T; ++t) ts[t + 1] = ts[t] + per_tree[t];
+  if (ts[T] != n_nodes) fail(DTE_ERR_INTERNAL, "node count mismatch after build");
+
+  auto* fo = new Forest();
+  int dev;
+  DTE_HIP(hipGetDevice(&dev));
+  fo->device = dev;
+  fo->task = task;
+  fo->algorithm = algorithm;
+  fo->n_classes = C;
+  fo->n_features = F;
+  fo->n_trees = T;
+  fo->fixed_shift = shift;
+  fo->n_nodes = n_nodes;
+  fo->has_stats = true;
+  fo->tree_start.alloc(T + 1);
+  DTE_HIP(hipMemcpyAsync(fo->tree_start.p, ts.data(), 8 * (T + 1), hipMemcpyHostToDevice, s));
+  fo->feature.alloc(n_nodes);
+  fo->left.alloc(n_nodes);
+  fo->depth.alloc(n_nodes);
+  fo->count.alloc(n_nodes);
+  fo->n_entries.alloc(n_nodes);
+  fo->threshold.alloc(n_nodes);
+  fo->score.alloc(n_nodes);
+  fo->value.alloc(n_nodes);
+  fo->rowhash.alloc(n_nodes);
+  fo->class_counts.alloc(task == 0 ? (size_t)n_nodes * C : 1);
+  fo->sum_fixed.alloc(n_nodes);
+  FinalArgs fa{};
+  fa.n_nodes = n_nodes;
+  fa.C = C;
+  fa.tree = o_tree.p;
+  fa.id = o_id.p;
+  fa.tree_start = fo->tree_start.p;
+  fa.feature = o_feature.p;
+  fa.thr = o_thr.p;
+  fa.score = o_score.p;
+  fa.left = o_left.p;
+  fa.depth = o_depth.p;
+  fa.count = o_count.p;
+  fa.m = o_m.p;
+  fa.hash = o_hash.p;
+  fa.cls = task == 0 ? o_cls.p : nullptr;
+  fa.sum = task == 1 ? o_sum.p : nullptr;
+  fa.value = o_value.p;
+  fa.f_feature = fo->feature.p;
+  fa.f_thr = fo->threshold.p;
+  fa.f_score = fo->score.p;
+  fa.f_left = fo->left.p;
+  fa.f_depth = fo->depth.p;
+  fa.f_count = fo->count.p;
+  fa.f_m = fo->n_entries.p;
+  fa.f_hash = fo->rowhash.p;
+  fa.f_cls = fo->class_counts.p;
+  fa.f_sum = fo->sum_fixed.p;
+  fa.f_value = fo->value.p;
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(n_nodes)), dim3(kBlock), 0, s, fa);
+  build_predict_nodes(*fo, s);
+  DTE_HIP(hipStreamSynchronize(s));
+  return fo;
+}
+
+void build_predict_nodes(Forest& f, hipStream_t s) {
+  f.pnodes.alloc(f.n_nodes);
+  hipLaunchKernelGGL(k_predict_nodes, dim3(grid_for(f.n_nodes)), dim3(kBlock), 0, s, f.n_nodes, f.n_classes,
+                     f.task, f.feature.p, f.threshold.p, f.left.p, f.class_counts.p, f.value.p, f.pnodes.p);
+}
+
+}  // namespace dte

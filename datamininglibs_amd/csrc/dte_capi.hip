@@ -1,0 +1,553 @@
+// C-ABI glue of libdte: status codes, handles, host<->device staging,
+// kernel-time profiling.  Exceptions never cross this boundary.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <queue>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dte_internal.h"
+
+struct dte_forest {
+  dte::Forest f;
+};
+
+namespace dte {
+
+namespace {
+thread_local std::string t_err;
+thread_local int t_code = 0;
+
+struct DteException : std::exception {
+  int code;
+  std::string msg;
+  DteException(int c, std::string m) : code(c), msg(std::move(m)) {}
+  const char* what() const noexcept override { return msg.c_str(); }
+};
+
+std::mutex g_dev_mu;
+std::map<int, hipStream_t> g_streams;
+
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+struct Pending {
+  std::string name;
+  hipEvent_t a, b;
+  double bytes;
+};
+std::vector<Pending> g_pending;
+std::vector<KernelStat> g_stats;
+
+void drain_pending() {
+  for (auto& p : g_pending) {
+    float ms = 0.f;
+    (void)hipEventSynchronize(p.b);
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+    auto it = std::find_if(g_stats.begin(), g_stats.end(), [&](const KernelStat& k) { return k.name == p.name; });
+    if (it == g_stats.end()) {
+      g_stats.push_back(KernelStat{p.name, 0, 0.0, 0.0});
+      it = g_stats.end() - 1;
+    }
+    it->launches += 1;
+    it->total_ms += ms;
+    it->bytes += p.bytes;
+  }
+  g_pending.clear();
+}
+
+template <typename F>
+int guarded(F&& fn) {
+  try {
+    fn();
+    t_code = 0;
+    return DTE_OK;
+  } catch (const DteException& e) {
+    t_code = e.code;
+    t_err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    t_code = DTE_ERR_OUT_OF_MEMORY;
+    t_err = "host allocation failed";
+    return t_code;
+  } catch (const std::exception& e) {
+    t_code = DTE_ERR_INTERNAL;
+    t_err = e.what();
+    return t_code;
+  }
+}
+
+void check_params(const dte_params* p) {
+  if (!p) fail(DTE_ERR_INVALID_ARGUMENT, "params is NULL");
+  if (p->algorithm != DTE_RANDOM_FOREST && p->algorithm != DTE_EXTRA_TREES)
+    fail(DTE_ERR_INVALID_ARGUMENT, "unknown algorithm");
+  if (p->task != DTE_CLASSIFICATION && p->task != DTE_REGRESSION) fail(DTE_ERR_INVALID_ARGUMENT, "unknown task");
+  if (p->n_trees <= 0) fail(DTE_ERR_INVALID_ARGUMENT, "n_trees must be positive");
+  if (p->n_features_per_node < 0) fail(DTE_ERR_INVALID_ARGUMENT, "n_features_per_node must be >= 0");
+  if (p->max_depth < 0) fail(DTE_ERR_INVALID_ARGUMENT, "max_depth must be >= 0");
+  if (p->min_node_size < 0) fail(DTE_ERR_INVALID_ARGUMENT, "min_node_size must be >= 0");
+  if (p->task == DTE_CLASSIFICATION && (p->n_classes <= 0 || p->n_classes > 32))
+    fail(DTE_ERR_INVALID_ARGUMENT, "n_classes must be in [1, 32]");
+  if (p->build_mode != DTE_BUILD_BATCHED && p->build_mode != DTE_BUILD_PLAIN)
+    fail(DTE_ERR_INVALID_ARGUMENT, "unknown build_mode");
+  for (int i = 0; i < 7; ++i)
+    if (p->reserved[i] != 0) fail(DTE_ERR_INVALID_ARGUMENT, "reserved fields must be zero");
+}
+
+void set_device(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) fail(DTE_ERR_NO_DEVICE, "no HIP device available");
+  if (device < 0 || device >= n) fail(DTE_ERR_NO_DEVICE, "device id out of range");
+  DTE_HIP(hipSetDevice(device));
+  device_init(device);
+}
+
+}  // namespace
+
+void set_error(int code, const std::string& msg) {
+  t_code = code;
+  t_err = msg;
+}
+
+void fail(int code, const std::string& msg) { throw DteException(code, msg); }
+
+bool profiling_enabled() { return g_prof_on; }
+
+void profile_record(const char* name, hipEvent_t a, hipEvent_t b, double bytes) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_pending.push_back(Pending{name, a, b, bytes});
+}
+
+void upload_ln_table(int device);
+
+void device_init(int device) {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  if (g_streams.count(device)) return;
+  upload_ln_table(device);
+  hipStream_t s;
+  DTE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  g_streams[device] = s;
+}
+
+hipStream_t library_stream(int device) {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  auto it = g_streams.find(device);
+  if (it == g_streams.end()) fail(DTE_ERR_INTERNAL, "device not initialised");
+  return it->second;
+}
+
+// Add algorithmic bytes to a kernel family (known only after the launches
+// it covers have finished).
+void profile_add_bytes(const char* name, double bytes) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  drain_pending();
+  auto it = std::find_if(g_stats.begin(), g_stats.end(), [&](const KernelStat& k) { return k.name == name; });
+  if (it == g_stats.end()) {
+    g_stats.push_back(KernelStat{name, 0, 0.0, 0.0});
+    it = g_stats.end() - 1;
+  }
+  it->bytes += bytes;
+}
+
+}  // namespace dte
+
+using namespace dte;
+
+namespace {
+struct HostModel {
+  std::vector<int64_t> ts;
+  std::vector<int32_t> feature, left;
+  std::vector<float> thr, value;
+  std::vector<uint32_t> cls;
+};
+
+HostModel pull_model(const Forest& f) {
+  hipStream_t s = library_stream(f.device);
+  HostModel m;
+  const int64_t n = f.n_nodes;
+  m.ts.resize(f.n_trees + 1);
+  DTE_HIP(hipMemcpyAsync(m.ts.data(), f.tree_start.p, 8 * (f.n_trees + 1), hipMemcpyDeviceToHost, s));
+  m.feature.resize(n);
+  m.left.resize(n);
+  m.thr.resize(n);
+  m.value.resize(n);
+  DTE_HIP(hipMemcpyAsync(m.feature.data(), f.feature.p, 4 * n, hipMemcpyDeviceToHost, s));
+  DTE_HIP(hipMemcpyAsync(m.left.data(), f.left.p, 4 * n, hipMemcpyDeviceToHost, s));
+  DTE_HIP(hipMemcpyAsync(m.thr.data(), f.threshold.p, 4 * n, hipMemcpyDeviceToHost, s));
+  DTE_HIP(hipMemcpyAsync(m.value.data(), f.value.p, 4 * n, hipMemcpyDeviceToHost, s));
+  if (f.task == 0) {
+    m.cls.resize((size_t)n * f.n_classes);
+    DTE_HIP(hipMemcpyAsync(m.cls.data(), f.class_counts.p, 4 * m.cls.size(), hipMemcpyDeviceToHost, s));
+  }
+  DTE_HIP(hipStreamSynchronize(s));
+  return m;
+}
+
+int64_t count_leaves(const std::vector<int32_t>& feature) {
+  int64_t n = 0;
+  for (auto v : feature) n += v < 0;
+  return n;
+}
+}  // namespace
+
+extern "C" {
+
+int dte_last_error(char* buf, int64_t cap) {
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>((size_t)cap - 1, t_err.size());
+    std::memcpy(buf, t_err.data(), n);
+    buf[n] = 0;
+  }
+  return t_code;
+}
+
+int dte_device_count(int32_t* count) {
+  return guarded([&] {
+    if (!count) fail(DTE_ERR_INVALID_ARGUMENT, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+int dte_params_default(int32_t algorithm, dte_params* out) {
+  return guarded([&] {
+    if (!out) fail(DTE_ERR_INVALID_ARGUMENT, "out is NULL");
+    if (algorithm != DTE_RANDOM_FOREST && algorithm != DTE_EXTRA_TREES)
+      fail(DTE_ERR_INVALID_ARGUMENT, "unknown algorithm");
+    std::memset(out, 0, sizeof(*out));
+    out->algorithm = algorithm;
+    out->task = DTE_CLASSIFICATION;
+    out->n_trees = 100;
+    out->n_features_per_node = 0;
+    out->max_depth = 1000;
+    out->min_node_size = 5;
+    out->n_classes = 2;
+    out->device_id = 0;
+    out->seed = algorithm == DTE_RANDOM_FOREST ? 324123u : 123512u;
+    out->build_mode = DTE_BUILD_BATCHED;
+  });
+}
+
+static void fit_common(const float* samples, bool on_device, const float* targets, int64_t n_rows,
+                       int32_t n_features, const dte_params* params, dte_forest** out) {
+  check_params(params);
+  if (!samples || !targets || !out) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+  if (n_rows <= 0 || n_features <= 0) fail(DTE_ERR_INVALID_ARGUMENT, "empty data set");
+  if (n_rows > 0x7FFFFFFF) fail(DTE_ERR_INVALID_ARGUMENT, "n_rows above 2^31-1");
+  set_device(params->device_id);
+  hipStream_t s = library_stream(params->device_id);
+  DevBuf<float> X;
+  const float* Xd = samples;
+  if (!on_device) {
+    X.alloc((size_t)n_rows * n_features);
+    DTE_HIP(hipMemcpyAsync(X.p, samples, sizeof(float) * n_rows * n_features, hipMemcpyHostToDevice, s));
+    Xd = X.p;
+  }
+  std::unique_ptr<Forest> f(fit_forest(Xd, targets, n_rows, n_features, *params, s));
+  auto* h = new dte_forest();
+  h->f = std::move(*f);
+  *out = h;
+}
+
+int dte_fit(const float* samples, const float* targets, int64_t n_rows, int32_t n_features,
+            const dte_params* params, dte_forest** out) {
+  return guarded([&] { fit_common(samples, false, targets, n_rows, n_features, params, out); });
+}
+
+int dte_fit_device(const float* samples_dev, const float* targets, int64_t n_rows, int32_t n_features,
+                   const dte_params* params, dte_forest** out) {
+  return guarded([&] { fit_common(samples_dev, true, targets, n_rows, n_features, params, out); });
+}
+
+int dte_forest_free(dte_forest* forest) {
+  return guarded([&] {
+    if (forest) {
+      (void)hipSetDevice(forest->f.device);
+      delete forest;
+    }
+  });
+}
+
+
+int dte_forest_info_get(const dte_forest* forest, dte_forest_info* out) {
+  return guarded([&] {
+    if (!forest || !out) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    const Forest& f = forest->f;
+    DTE_HIP(hipSetDevice(f.device));
+    std::vector<int32_t> feat(f.n_nodes);
+    DTE_HIP(hipMemcpy(feat.data(), f.feature.p, 4 * f.n_nodes, hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->n_trees = f.n_trees;
+    out->n_classes = f.n_classes;
+    out->n_features = f.n_features;
+    out->task = f.task;
+    out->algorithm = f.algorithm;
+    out->fixed_shift = f.fixed_shift;
+    out->n_nodes = f.n_nodes;
+    out->n_leaves = count_leaves(feat);
+    out->n_probabilities = out->n_leaves * (f.task == 0 ? f.n_classes : 1);
+    out->device_bytes = (int64_t)(f.pnodes.bytes() + f.tree_start.bytes() + f.feature.bytes() + f.left.bytes() +
+                                  f.depth.bytes() + f.count.bytes() + f.n_entries.bytes() + f.threshold.bytes() +
+                                  f.score.bytes() + f.value.bytes() + f.rowhash.bytes() +
+                                  f.class_counts.bytes() + f.sum_fixed.bytes());
+  });
+}
+
+int dte_forest_export(const dte_forest* forest, dte_node* nodes, int64_t nodes_cap, float* probs,
+                      int64_t probs_cap) {
+  return guarded([&] {
+    if (!forest || !nodes || !probs) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    const Forest& f = forest->f;
+    DTE_HIP(hipSetDevice(f.device));
+    HostModel m = pull_model(f);
+    const int64_t n = f.n_nodes;
+    const int T = f.n_trees;
+    const int P = f.task == 0 ? f.n_classes : 1;
+    if (nodes_cap < n) fail(DTE_ERR_INVALID_ARGUMENT, "nodes_cap too small");
+    if (probs_cap < count_leaves(m.feature) * P) fail(DTE_ERR_INVALID_ARGUMENT, "probs_cap too small");
+    // canonical index -> export index: roots first (gpudte_algorithm.cc:349-357),
+    // then every tree's remaining nodes in level order.
+    std::vector<int64_t> ex(n);
+    int64_t next = T;
+    for (int t = 0; t < T; ++t) {
+      ex[m.ts[t]] = t;
+      for (int64_t g = m.ts[t] + 1; g < m.ts[t + 1]; ++g) ex[g] = next++;
+    }
+    std::vector<int64_t> inv(n);
+    for (int64_t g = 0; g < n; ++g) inv[ex[g]] = g;
+    int64_t pcur = 0;
+    for (int64_t e = 0; e < n; ++e) {
+      const int64_t g = inv[e];
+      dte_node& d = nodes[e];
+      if (m.feature[g] >= 0) {
+        d.child_start = (int32_t)ex[m.left[g]];
+        d.child_count = 2;
+        d.probability_start = -1;
+        d.attribute = m.feature[g];
+        d.split_point = m.thr[g];
+      } else {
+        d.child_start = -1;
+        d.child_count = 0;
+        d.probability_start = (int32_t)pcur;
+        d.attribute = -1;
+        d.split_point = -1.0f;
+        if (f.task == 0)
+          for (int c = 0; c < P; ++c) probs[pcur++] = (float)m.cls[g * P + c];
+        else
+          probs[pcur++] = m.value[g];
+      }
+    }
+  });
+}
+
+int dte_forest_import(const dte_node* nodes, int64_t n_nodes, const float* probs, int64_t n_probs, int32_t n_trees,
+                      int32_t n_classes, int32_t n_features, int32_t task, int32_t device_id, dte_forest** out) {
+  return guarded([&] {
+    if (!nodes || !probs || !out) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    if (n_trees <= 0 || n_nodes < n_trees) fail(DTE_ERR_INVALID_ARGUMENT, "bad tree / node counts");
+    if (task != 0 && task != 1) fail(DTE_ERR_INVALID_ARGUMENT, "unknown task");
+    const int P = task == 0 ? n_classes : 1;
+    if (P <= 0) fail(DTE_ERR_INVALID_ARGUMENT, "n_classes must be positive");
+    // Canonicalise: per tree, breadth-first from root t, children in order.
+    std::vector<int64_t> order;
+    order.reserve(n_nodes);
+    std::vector<int64_t> ts(n_trees + 1, 0);
+    std::vector<uint8_t> seen(n_nodes, 0);
+    for (int t = 0; t < n_trees; ++t) {
+      ts[t] = (int64_t)order.size();
+      size_t head = order.size();
+      order.push_back(t);
+      seen[t] = 1;
+      while (head < order.size()) {
+        const dte_node& d = nodes[order[head++]];
+        if (d.child_count == 0) {
+          if (d.probability_start < 0 || (int64_t)d.probability_start + P > n_probs)
+            fail(DTE_ERR_INVALID_ARGUMENT, "leaf probability index out of range");
+          continue;
+        }
+        if (d.child_count != 2) fail(DTE_ERR_INVALID_ARGUMENT, "child_count must be 0 or 2");
+        if (d.attribute < 0 || d.attribute >= n_features) fail(DTE_ERR_INVALID_ARGUMENT, "attribute out of range");
+        for (int c = 0; c < 2; ++c) {
+          const int64_t ch = (int64_t)d.child_start + c;
+          if (ch < 0 || ch >= n_nodes || seen[ch]) fail(DTE_ERR_INVALID_ARGUMENT, "bad child link");
+          seen[ch] = 1;
+          order.push_back(ch);
+        }
+      }
+    }
+    ts[n_trees] = (int64_t)order.size();
+    const int64_t n = (int64_t)order.size();
+    std::vector<int64_t> canon(n_nodes, -1);
+    for (int64_t g = 0; g < n; ++g) canon[order[g]] = g;
+    std::vector<int32_t> feature(n), left(n), depth(n, 0);
+    std::vector<float> thr(n), value(n, 0.0f);
+    std::vector<uint32_t> cls(task == 0 ? (size_t)n * P : 0, 0);
+    for (int64_t g = 0; g < n; ++g) {
+      const dte_node& d = nodes[order[g]];
+      if (d.child_count == 0) {
+        feature[g] = -1;
+        left[g] = -1;
+        thr[g] = -1.0f;
+        if (task == 0)
+          for (int c = 0; c < P; ++c) {
+            const float v = probs[d.probability_start + c];
+            cls[g * P + c] = v > 0 ? (uint32_t)v : 0u;
+          }
+        else
+          value[g] = probs[d.probability_start];
+      } else {
+        feature[g] = d.attribute;
+        thr[g] = d.split_point;
+        left[g] = (int32_t)canon[d.child_start];
+      }
+    }
+    for (int t = 0; t < n_trees; ++t)
+      for (int64_t g = ts[t]; g < ts[t + 1]; ++g)
+        if (feature[g] >= 0) depth[left[g]] = depth[left[g] + 1] = depth[g] + 1;
+    set_device(device_id);
+    hipStream_t s = library_stream(device_id);
+    auto* h = new dte_forest();
+    std::unique_ptr<dte_forest> guard(h);
+    Forest& f = h->f;
+    f.device = device_id;
+    f.task = task;
+    f.n_classes = P;
+    f.n_features = n_features;
+    f.n_trees = n_trees;
+    f.n_nodes = n;
+    f.has_stats = false;
+    f.tree_start.alloc(n_trees + 1);
+    f.feature.alloc(n);
+    f.left.alloc(n);
+    f.depth.alloc(n);
+    f.threshold.alloc(n);
+    f.value.alloc(n);
+    f.class_counts.alloc(task == 0 ? (size_t)n * P : 1);
+    DTE_HIP(hipMemcpyAsync(f.tree_start.p, ts.data(), 8 * ts.size(), hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(f.feature.p, feature.data(), 4 * n, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(f.left.p, left.data(), 4 * n, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(f.depth.p, depth.data(), 4 * n, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(f.threshold.p, thr.data(), 4 * n, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(f.value.p, value.data(), 4 * n, hipMemcpyHostToDevice, s));
+    if (task == 0) DTE_HIP(hipMemcpyAsync(f.class_counts.p, cls.data(), 4 * cls.size(), hipMemcpyHostToDevice, s));
+    build_predict_nodes(f, s);
+    DTE_HIP(hipStreamSynchronize(s));
+    *out = guard.release();
+  });
+}
+
+int dte_forest_canonical(const dte_forest* forest, int64_t* tree_start, int32_t* feature, float* threshold,
+                         int32_t* left, int32_t* depth, int32_t* count, int32_t* n_entries, float* score,
+                         uint64_t* rowhash, uint32_t* class_counts, int64_t* sum_fixed, float* value) {
+  return guarded([&] {
+    if (!forest) fail(DTE_ERR_INVALID_ARGUMENT, "forest is NULL");
+    const Forest& f = forest->f;
+    DTE_HIP(hipSetDevice(f.device));
+    hipStream_t s = library_stream(f.device);
+    const int64_t n = f.n_nodes;
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+      if (dst && src) DTE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    };
+    cp(tree_start, f.tree_start.p, 8 * (f.n_trees + 1));
+    cp(feature, f.feature.p, 4 * n);
+    cp(threshold, f.threshold.p, 4 * n);
+    cp(left, f.left.p, 4 * n);
+    cp(depth, f.depth.p, 4 * n);
+    cp(value, f.value.p, 4 * n);
+    if (f.has_stats) {
+      cp(count, f.count.p, 4 * n);
+      cp(n_entries, f.n_entries.p, 4 * n);
+      cp(score, f.score.p, 4 * n);
+      cp(rowhash, f.rowhash.p, 8 * n);
+      cp(sum_fixed, f.sum_fixed.p, 8 * n);
+    }
+    if (f.task == 0) cp(class_counts, f.class_counts.p, 4 * (size_t)n * f.n_classes);
+    DTE_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dte_predict(const dte_forest* forest, const float* samples, int64_t n_rows, float* out) {
+  return guarded([&] {
+    if (!forest || !samples || !out) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    if (n_rows < 0) fail(DTE_ERR_INVALID_ARGUMENT, "n_rows < 0");
+    const Forest& f = forest->f;
+    if (n_rows == 0) return;
+    set_device(f.device);
+    hipStream_t s = library_stream(f.device);
+    DevBuf<float> X((size_t)n_rows * f.n_features);
+    DTE_HIP(hipMemcpyAsync(X.p, samples, sizeof(float) * n_rows * f.n_features, hipMemcpyHostToDevice, s));
+    if (f.task == 0) {
+      DevBuf<int32_t> v((size_t)n_rows * f.n_classes);
+      predict_forest(f, X.p, n_rows, n_rows, 0, v.p, nullptr, s);
+      std::vector<int32_t> h((size_t)n_rows * f.n_classes);
+      DTE_HIP(hipMemcpyAsync(h.data(), v.p, 4 * h.size(), hipMemcpyDeviceToHost, s));
+      DTE_HIP(hipStreamSynchronize(s));
+      for (size_t i = 0; i < h.size(); ++i) out[i] = (float)h[i];
+    } else {
+      DevBuf<float> v(n_rows);
+      predict_forest(f, X.p, n_rows, n_rows, 0, nullptr, v.p, s);
+      DTE_HIP(hipMemcpyAsync(out, v.p, 4 * n_rows, hipMemcpyDeviceToHost, s));
+      DTE_HIP(hipStreamSynchronize(s));
+    }
+  });
+}
+
+int dte_predict_device(const dte_forest* forest, const float* samples_dev, int64_t n_rows, int64_t ld,
+                       int32_t row_major, int32_t* votes_dev, float* sums_dev, void* stream) {
+  return guarded([&] {
+    if (!forest || !samples_dev) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    const Forest& f = forest->f;
+    if (f.task == 0 && !votes_dev) fail(DTE_ERR_INVALID_ARGUMENT, "votes_dev is NULL");
+    if (f.task == 1 && !sums_dev) fail(DTE_ERR_INVALID_ARGUMENT, "sums_dev is NULL");
+    if (row_major ? ld < f.n_features : ld < n_rows) fail(DTE_ERR_INVALID_ARGUMENT, "leading dimension too small");
+    set_device(f.device);
+    hipStream_t s = stream ? (hipStream_t)stream : library_stream(f.device);
+    predict_forest(f, samples_dev, n_rows, ld, row_major, votes_dev, sums_dev, s);
+    if (!stream) DTE_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dte_profiling(int32_t enable) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_on = enable != 0;
+  });
+}
+
+int dte_profile_reset(void) {
+  return guarded([&] {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    drain_pending();
+    g_stats.clear();
+  });
+}
+
+int dte_profile_query(int32_t index, char* name, int32_t name_cap, int64_t* launches, double* total_ms,
+                      double* algorithmic_bytes) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  drain_pending();
+  const int n = (int)g_stats.size();
+  if (index >= 0 && index < n) {
+    const KernelStat& k = g_stats[index];
+    if (name && name_cap > 0) {
+      const size_t c = std::min<size_t>((size_t)name_cap - 1, k.name.size());
+      std::memcpy(name, k.name.data(), c);
+      name[c] = 0;
+    }
+    if (launches) *launches = k.launches;
+    if (total_ms) *total_ms = k.total_ms;
+    if (algorithmic_bytes) *algorithmic_bytes = k.bytes;
+  }
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// Internal declarations shared by the build, predict, serve and C-ABI units
+// of libdte.  Not part of the public ABI (include/dte.h is).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dte.h"
+#include "dte_numerics.h"
+
+namespace dte {
+
+// ------------------------------------------------------------- errors ----
+struct Error {
+  int code;
+  std::string msg;
+};
+void set_error(int code, const std::string& msg);
+[[noreturn]] void fail(int code, const std::string& msg);
+
+#define DTE_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      ::dte::fail(_e == hipErrorOutOfMemory ? DTE_ERR_OUT_OF_MEMORY : DTE_ERR_HIP,      \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+  } while (0)
+
+// Device buffer owned by RAII (the reference frees by hand in
+// GpuParams::finalize, gpudte_algorithm_shared.cc:30-52, and leaks on throw).
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    DTE_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  // Grow to at least `count` elements; contents are not preserved.
+  void reserve(size_t count) {
+    if (count > n) alloc(count + count / 4);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Kernel timing of one named kernel family, collected with HIP events on the
+// library's stream when profiling is on.
+struct KernelStat {
+  std::string name;
+  int64_t launches = 0;
+  double total_ms = 0.0;
+  double bytes = 0.0;  // algorithmic bytes
+};
+
+void profile_record(const char* name, hipEvent_t a, hipEvent_t b, double bytes);
+void profile_add_bytes(const char* name, double bytes);
+bool profiling_enabled();
+
+// Predict-side node record: 16 bytes, one 128-bit load per visited node.
+//   internal: feature >= 0, thr, left child index (right = left + 1)
+//   leaf:     feature = -1, thr = regression leaf value, left = vote class
+struct PNode {
+  int32_t feature;
+  float thr;
+  int32_t left;
+  int32_t pad;
+};
+
+// A fitted forest resident in HBM.  Node arrays are canonical: tree-major,
+// level order inside each tree, children adjacent.
+struct Forest {
+  int device = 0;
+  int task = 0;
+  int n_classes = 1;
+  int n_features = 0;
+  int n_trees = 0;
+  int fixed_shift = 0;
+  int algorithm = 0;
+  int64_t n_nodes = 0;
+  DevBuf<int64_t> tree_start;  // n_trees + 1
+  DevBuf<PNode> pnodes;        // n_nodes
+  // Build diagnostics (absent for imported forests).
+  bool has_stats = false;
+  DevBuf<int32_t> feature, left, depth, count, n_entries;
+  DevBuf<float> threshold, score, value;
+  DevBuf<uint64_t> rowhash;
+  DevBuf<uint32_t> class_counts;  // n_nodes * n_classes (classification)
+  DevBuf<int64_t> sum_fixed;
+};
+
+struct FitInput {
+  const float* X;  // device, feature-major, canonical (-0 -> +0)
+  const float* y;  // device or host targets (see caller)
+  int64_t n_rows;
+  int32_t n_features;
+};
+
+// Entry points implemented in dte_build.hip / dte_predict.hip.
+Forest* fit_forest(const float* X_dev, const float* y_host, int64_t n_rows, int32_t n_features,
+                   const dte_params& p, hipStream_t stream);
+void build_predict_nodes(Forest& f, hipStream_t stream);
+void predict_forest(const Forest& f, const float* X_dev, int64_t n_rows, int64_t ld,
+                    int row_major, int32_t* votes_dev, float* sums_dev, hipStream_t stream);
+void device_init(int device);
+hipStream_t library_stream(int device);
+const LnTable& host_ln_table();
+
+constexpr int kBlock = 256;
+
+}  // namespace dte

@@ -1,0 +1,202 @@
+/*
+ * libdte -- C ABI of the MI355X (gfx950) tree-ensemble build + predict path.
+ *
+ * This is the seam the reference's host orchestrator calls into.  In the
+ * reference, GpuDteAlgorithm<T>::Fit / ::Predict
+ * (lib_ensembles/source/gpudte_algorithm.cc:14-267, 269-331) drive the
+ * device through GpuDte<T>::CallCudaKernel / CopyIterationInfo /
+ * CopyDataStaticInfo (lib_ensembles/source/gpudte.h:18-24) and
+ * lib_gpu::GpuDevice alloc/copy/sync (lib_gpu/include/gpu_device.h:9-19).
+ * Here that whole device side sits behind the functions below: plain
+ * pointers and sizes, integer status codes, opaque handles, no C++ types.
+ *
+ * Conventions
+ *  - Every function returns DTE_OK (0) or a DTE_ERR_* code; the message of
+ *    the last failure on the calling thread is read with dte_last_error().
+ *    (The reference throws through CoreLib::ThrowException,
+ *    lib_core/source/core_interface.cc:28-33; the C++ layer above this ABI
+ *    rethrows.)
+ *  - Calls are blocking and re-entrant per device: per-launch state travels
+ *    as kernel arguments, not __constant__ globals (gpudte.cu:12-52).
+ *  - Matrices are fp32 and feature-major, X[f * n_rows + row], exactly the
+ *    layout of MlDataFrame<float>::GetSamples() (lib_parsing/source/
+ *    csv_parser.cc:92-96), unless a row-major variant says otherwise.
+ */
+#ifndef DTE_H_
+#define DTE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define DTE_OK 0
+#define DTE_ERR_INVALID_ARGUMENT 1
+#define DTE_ERR_HIP 2
+#define DTE_ERR_OUT_OF_MEMORY 3
+#define DTE_ERR_NO_DEVICE 4
+#define DTE_ERR_ADMISSION 5
+#define DTE_ERR_INTERNAL 6
+#define DTE_ERR_NOT_FOUND 7
+
+/* Algorithms: GpuRf / GpuErt (lib_ensembles/source/gpurf.h, gpuert.h). */
+#define DTE_RANDOM_FOREST 0
+#define DTE_EXTRA_TREES 1
+
+/* Tasks: AlgorithmsInterface::AlgorithmType (lib_algorithms/include/
+ * algorithms_interface.h:11). */
+#define DTE_CLASSIFICATION 0
+#define DTE_REGRESSION 1
+
+/* Build drivers.  BATCHED: all open nodes of every tree of a level in one
+ * set of launches.  PLAIN: the reference's launch structure (tree batches of
+ * <= 100, node chunks of <= 512, one block per node, host round trips per
+ * chunk; gpudte_algorithm.cc:105-253).  Both produce identical forests. */
+#define DTE_BUILD_BATCHED 0
+#define DTE_BUILD_PLAIN 1
+
+/* Fit parameters.  Mirrors the int-keyed param pack of
+ * EnsemblesInterface::CreateGpuRfParamPack / CreateGpuErtParamPack
+ * (lib_ensembles/source/ensembles_interface.cc:28-58): kNrTrees, kAlgoType,
+ * kNrFeatures, kMaxDepth, kMinNodeSize, kDevId. */
+typedef struct dte_params {
+  int32_t algorithm;           /* DTE_RANDOM_FOREST | DTE_EXTRA_TREES */
+  int32_t task;                /* kAlgoType: DTE_CLASSIFICATION | DTE_REGRESSION */
+  int32_t n_trees;             /* kNrTrees (default 100) */
+  int32_t n_features_per_node; /* kNrFeatures; 0 -> round(ln F) + 1 (gpudte_algorithm.cc:579-580) */
+  int32_t max_depth;           /* kMaxDepth (default 1000) */
+  int32_t min_node_size;       /* kMinNodeSize (default 5) */
+  int32_t n_classes;           /* MlDataFrame::GetNrTargets(); classification only */
+  int32_t device_id;           /* kDevId */
+  uint64_t seed;               /* RNG key; defaults 324123 (RF) / 123512 (ERT) as gpurf.cu:55, gpuert.cu:18 */
+  int32_t build_mode;          /* DTE_BUILD_BATCHED (default) | DTE_BUILD_PLAIN */
+  int32_t reserved[7];         /* must be zero */
+} dte_params;
+
+/* One node of the exported model: gpuDTE_NodeHeader_Classify<float>
+ * (lib_ensembles/source/gpudte_algorithm_shared.h:82-90).  Leaves have
+ * child_count == 0 and point at n_classes floats (class counts) or one
+ * float (regression mean) in the probability array. */
+typedef struct dte_node {
+  int32_t child_start;
+  int32_t child_count;
+  int32_t probability_start;
+  int32_t attribute;
+  float split_point;
+} dte_node;
+
+typedef struct dte_forest dte_forest;
+
+typedef struct dte_forest_info {
+  int32_t n_trees;
+  int32_t n_classes;      /* 1 for regression */
+  int32_t n_features;
+  int32_t task;
+  int32_t algorithm;
+  int32_t fixed_shift;    /* regression: fixed-point shift of the target sums */
+  int64_t n_nodes;
+  int64_t n_leaves;
+  int64_t n_probabilities; /* floats in the exported probability array */
+  int64_t device_bytes;    /* HBM held by the forest */
+} dte_forest_info;
+
+/* Defaults of CreateGpuRfParamPack / CreateGpuErtParamPack
+ * (ensembles_interface.cc:28-58). */
+int dte_params_default(int32_t algorithm, dte_params* out);
+
+/* Fit.  Replaces the device side of GpuDteAlgorithm<T>::Fit
+ * (gpudte_algorithm.cc:14-267: AllocateFit, kSetupKernel, kInitTreeBatch,
+ * kFindSplit, kPerformSplit, node streaming).  samples: host, feature-major
+ * n_features x n_rows; targets: host, n_rows (class ids as float, or
+ * regression targets). */
+int dte_fit(const float* samples, const float* targets, int64_t n_rows, int32_t n_features,
+            const dte_params* params, dte_forest** out);
+
+/* Fit with the samples already resident in HBM (device pointer on
+ * params->device_id); targets stay on the host. */
+int dte_fit_device(const float* samples_dev, const float* targets, int64_t n_rows,
+                   int32_t n_features, const dte_params* params, dte_forest** out);
+
+int dte_forest_info_get(const dte_forest* forest, dte_forest_info* out);
+
+/* Export in the reference model layout: MlModel kNodeArray / kProbArray
+ * (lib_models/include/models_interface.h:9-19), roots at [0, n_trees) as
+ * the reference's AggregateModels produces (gpudte_algorithm.cc:333-397),
+ * then every tree's other nodes in level order, children adjacent. */
+int dte_forest_export(const dte_forest* forest, dte_node* nodes, int64_t nodes_cap, float* probs,
+                      int64_t probs_cap);
+
+/* Import a model in that layout (any node order whose links are
+ * consistent) and make it resident for predict. */
+int dte_forest_import(const dte_node* nodes, int64_t n_nodes, const float* probs, int64_t n_probs,
+                      int32_t n_trees, int32_t n_classes, int32_t n_features, int32_t task,
+                      int32_t device_id, dte_forest** out);
+
+/* Canonical build arrays (tree-major, level order in each tree) for
+ * parity checks.  Any output pointer may be NULL.  class_counts has
+ * n_nodes * n_classes entries (classification only). */
+int dte_forest_canonical(const dte_forest* forest, int64_t* tree_start, int32_t* feature,
+                         float* threshold, int32_t* left, int32_t* depth, int32_t* count,
+                         int32_t* n_entries, float* score, uint64_t* rowhash,
+                         uint32_t* class_counts, int64_t* sum_fixed, float* value);
+
+int dte_forest_free(dte_forest* forest);
+
+/* Predict.  Replaces GpuDteAlgorithm<T>::Predict + kPredict
+ * (gpudte_algorithm.cc:269-331, gpudte.cu:311-359).  samples: host,
+ * feature-major n_features x n_rows.  out: n_rows x n_classes vote counts
+ * (classification) or n_rows sums over trees in tree order (regression;
+ * mean = sum / n_trees), as MlResultData receives them. */
+int dte_predict(const dte_forest* forest, const float* samples, int64_t n_rows, float* out);
+
+/* Predict on device buffers.  samples_dev: feature-major with leading
+ * dimension ld (>= n_rows) when row_major == 0, else row-major with leading
+ * dimension ld (>= n_features).  votes_dev (n_rows x n_classes int32) or
+ * sums_dev (n_rows fp32).  stream: hipStream_t or NULL for the library
+ * stream. */
+int dte_predict_device(const dte_forest* forest, const float* samples_dev, int64_t n_rows,
+                       int64_t ld, int32_t row_major, int32_t* votes_dev, float* sums_dev,
+                       void* stream);
+
+/* Serving: forests stay resident, requests of uneven row counts are
+ * batched continuously, admission by free HBM. */
+typedef struct dte_server dte_server;
+typedef struct dte_server_stats {
+  int64_t requests_done;
+  int64_t rows_done;
+  int64_t batches_launched;
+  int64_t requests_rejected;
+  double busy_ms;
+  int64_t hbm_in_use;
+} dte_server_stats;
+
+int dte_server_create(int32_t device_id, int64_t hbm_budget_bytes, int32_t max_batch_rows,
+                      dte_server** out);
+int dte_server_load_forest(dte_server* server, const dte_forest* forest, int32_t* slot);
+/* rows: host, row-major n_rows x n_features.  The server copies them. */
+int dte_server_submit(dte_server* server, int32_t slot, const float* rows, int64_t n_rows,
+                      uint64_t* ticket);
+/* done = 1 when finished; out receives n_rows x n_classes votes or n_rows sums. */
+int dte_server_poll(dte_server* server, uint64_t ticket, float* out, int32_t* done);
+int dte_server_wait(dte_server* server, uint64_t ticket, float* out);
+int dte_server_get_stats(dte_server* server, dte_server_stats* out);
+int dte_server_destroy(dte_server* server);
+
+/* Kernel timing: HIP events recorded around every launch of the library's
+ * kernels while profiling is on. */
+int dte_profiling(int32_t enable);
+int dte_profile_reset(void);
+/* Returns the number of kernel families; fills up to cap entries. */
+int dte_profile_query(int32_t index, char* name, int32_t name_cap, int64_t* launches,
+                      double* total_ms, double* algorithmic_bytes);
+
+int dte_device_count(int32_t* count);
+int dte_last_error(char* buf, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DTE_H_ */

@@ -1,0 +1,89 @@
+/*
+ * CPU oracle for the tree-ensemble build + predict hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is loaded by tests/, by
+ * __graft_entry__.smoke() and by the cpu_baseline leg of bench.py, and only
+ * as the checker.  The product (datamininglibs_amd, libdte.so) never links,
+ * loads or calls it.
+ *
+ * It restates the reference algorithm of novelari/DataminingLibs
+ * (lib_ensembles/source/gpurf.cu, gpuert.cu, gpudte.cu, gpudte_algorithm.cc)
+ * as plain sequential C++, with the reference's undefined / racy /
+ * order-dependent behaviour replaced by the evident intent (each such
+ * replacement is tagged FIX-Dn / KEEP-Kn / T1 in dte_oracle.cc).
+ *
+ * Pinning: Philox4x32-10 known-answer vectors (Random123), closed-form
+ * entropy / variance cases, brute-force best split on tiny nodes, the fp64
+ * libm recompute of every chosen split score, whole-forest invariants, the
+ * reference's own CSV fixtures (toy parity CSV, spambase).  See tests/.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_params {
+  int32_t algorithm;      /* 0 = random forest (bootstrap, exhaustive sorted split), 1 = extra trees */
+  int32_t task;           /* 0 = classification, 1 = regression */
+  int32_t n_trees;
+  int32_t k;              /* features drawn per node; 0 -> round(ln F) + 1 */
+  int32_t max_depth;
+  int32_t min_node_size;
+  int32_t n_classes;      /* classification only */
+  int32_t n_threads;      /* tree-parallel worker threads (0 -> hardware_concurrency) */
+  uint64_t seed;
+} oracle_params;
+
+typedef struct oracle_forest oracle_forest;
+
+/* Fit a forest.  X is feature-major X[f * n_rows + row], y has n_rows entries
+ * (class ids stored as float for classification, targets for regression).
+ * Returns 0 on success. */
+int oracle_fit(const float* X, const float* y, int64_t n_rows, int32_t n_features,
+               const oracle_params* p, oracle_forest** out);
+void oracle_forest_free(oracle_forest* f);
+
+int64_t oracle_forest_n_nodes(const oracle_forest* f);
+int32_t oracle_forest_n_trees(const oracle_forest* f);
+int32_t oracle_forest_stat_width(const oracle_forest* f); /* C (classification) or 1 */
+int32_t oracle_forest_fixed_shift(const oracle_forest* f);
+
+/* Canonical node arrays, tree-major, level order inside each tree.
+ * Any pointer may be NULL.  tree_start has n_trees + 1 entries. */
+void oracle_forest_get(const oracle_forest* f, int64_t* tree_start, int32_t* feature,
+                       float* threshold, int32_t* left, int32_t* depth, int32_t* count,
+                       int32_t* n_entries, float* score, uint64_t* rowhash,
+                       uint32_t* class_counts /* n_nodes * C */,
+                       int64_t* sum_fixed /* n_nodes */, float* value /* n_nodes */);
+
+/* fp64 recompute of each internal node's chosen split score with libm log,
+ * straight from the float inputs (no fixed point, no x ln x table). */
+int oracle_forest_recompute_scores(const oracle_forest* f, const float* X, const float* y,
+                                   int64_t n_rows, int32_t n_features, double* out);
+
+/* Predict: votes[row * C + c] (classification) or sums[row] (regression). */
+int oracle_predict(const oracle_forest* f, const float* X, int64_t n_rows,
+                   int32_t* votes, float* sums);
+
+/* Building blocks exposed for known-answer tests. */
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+uint32_t oracle_rand_u32(uint64_t seed, uint32_t stream, uint32_t tree, uint32_t node, uint64_t idx);
+double oracle_xlnx(uint64_t x);
+double oracle_info_gain(const uint32_t* node_counts, const uint32_t* left_counts, int32_t n_classes);
+double oracle_variance_gain(int64_t n_left, int64_t s_left, int64_t n_right, int64_t s_right,
+                            int32_t fixed_shift);
+int32_t oracle_fixed_shift(double max_abs_target, int64_t total_weight);
+float oracle_midpoint(float a, float b);
+/* Best RF split of one node on one feature (classification), exposed for the
+ * brute-force check.  values/labels/weights describe the node's entries.
+ * Returns the fp32 score; *thr receives the threshold. */
+float oracle_rf_best_split(const float* values, const int32_t* labels, const uint32_t* weights,
+                           int64_t m, int32_t n_classes, float* thr);
+
+const char* oracle_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,59 @@
+"""HIP build + predict vs the CPU oracle, bit for bit, on seeded inputs."""
+import numpy as np
+import pytest
+
+from datamininglibs_amd import capi, synth
+from parity import assert_same_forest, fit_both
+
+pytestmark = pytest.mark.gpu
+
+
+def small_classes(seed, n, f, c, informative=4):
+    return synth.gaussian_classes(seed=seed, n=n, f=f, n_informative=informative, n_classes=c)
+
+
+@pytest.mark.parametrize("algorithm", [capi.EXTRA_TREES, capi.RANDOM_FOREST])
+def test_spam_like_small(algorithm):
+    X, y = synth.spam_like(seed=0, n=600, f=20)
+    g, o = fit_both(X, y, algorithm=algorithm, n_trees=5, k=4, max_depth=25, min_node_size=10, n_classes=2)
+    assert_same_forest(g, o)
+    Xp, _ = synth.spam_like(seed=1, n=300, f=20)
+    assert np.array_equal(g.predict(Xp).astype(np.int64), o.predict(Xp).astype(np.int64))
+
+
+@pytest.mark.parametrize("algorithm", [capi.EXTRA_TREES, capi.RANDOM_FOREST])
+@pytest.mark.parametrize("c", [3, 10])
+def test_multiclass(algorithm, c):
+    X, y = small_classes(3, 3000, 16, c)
+    g, o = fit_both(X, y, algorithm=algorithm, n_trees=3, k=5, max_depth=20, min_node_size=5, n_classes=c)
+    assert_same_forest(g, o)
+
+
+def test_rf_big_nodes_bucket_path():
+    # 20k rows: root and first levels exceed the LDS chunk and take the
+    # bucketed path.
+    X, y = small_classes(5, 20000, 12, 4)
+    g, o = fit_both(X, y, algorithm=capi.RANDOM_FOREST, n_trees=2, k=3, max_depth=12, min_node_size=5,
+                    n_classes=4)
+    assert_same_forest(g, o)
+
+
+def test_rf_heavy_ties_histogram_path():
+    # Few distinct values: buckets overflow the sort capacity and use the
+    # LDS histogram.
+    rng = np.random.default_rng(11)
+    X = rng.integers(0, 6, size=(6, 12000)).astype(np.float32)
+    y = ((X[0] + X[1] + rng.integers(0, 2, 12000)) % 3).astype(np.float32)
+    g, o = fit_both(X, y, algorithm=capi.RANDOM_FOREST, n_trees=2, k=3, max_depth=8, min_node_size=5,
+                    n_classes=3)
+    assert_same_forest(g, o)
+
+
+@pytest.mark.parametrize("algorithm", [capi.EXTRA_TREES, capi.RANDOM_FOREST])
+def test_regression(algorithm):
+    X, y = synth.sine_regression(seed=2, n=5000, f=8)
+    g, o = fit_both(X, y, algorithm=algorithm, task=capi.REGRESSION, n_trees=3, k=3, max_depth=12,
+                    min_node_size=5)
+    assert_same_forest(g, o)
+    Xp, _ = synth.sine_regression(seed=3, n=500, f=8)
+    assert np.array_equal(g.predict(Xp), o.predict(Xp))
