@@ -126,6 +126,10 @@ def lib():
                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double)]
         L.dte_device_count.argtypes = [ctypes.POINTER(ctypes.c_int32)]
+        L.dte_device_alloc.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(P)]
+        L.dte_device_free.argtypes = [ctypes.c_int32, P]
+        L.dte_memcpy_to_device.argtypes = [ctypes.c_int32, P, P, ctypes.c_int64]
+        L.dte_memcpy_to_host.argtypes = [ctypes.c_int32, P, P, ctypes.c_int64]
         L.dte_last_error.argtypes = [ctypes.c_char_p, ctypes.c_int64]
         _lib = L
     return _lib
@@ -279,3 +283,37 @@ def declared_symbols():
     import re
     txt = open(HEADER_PATH).read()
     return sorted(set(re.findall(r"^int (dte_\w+)\(", txt, re.M)))
+
+
+class DeviceArray:
+    """A block of HBM owned through libdte (no torch needed)."""
+
+    def __init__(self, nbytes, device=0):
+        self.device = device
+        self.nbytes = int(nbytes)
+        p = P()
+        _check(lib().dte_device_alloc(device, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a, device=0):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes, device)
+        _check(lib().dte_memcpy_to_device(device, P(d.ptr), _ptr(a), a.nbytes))
+        return d
+
+    def to_numpy(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        _check(lib().dte_memcpy_to_host(self.device, _ptr(out), P(self.ptr), out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().dte_device_free(self.device, P(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,6 +1,8 @@
-// Block-level primitives for 256-thread (4 x wave64) workgroups on gfx950:
-// wave scans through DPP-backed shuffles, block scans through one LDS round,
-// and a stable LDS radix sort pass with 4-bit digits.
+// Team-level primitives for gfx950 (wave64).  A "team" is either one wave
+// (TEAM = 64: synchronised with a wavefront-scope fence, several teams per
+// workgroup work on independent items) or a whole 256-thread workgroup
+// (TEAM = 256: __syncthreads).  Scans go through DPP-backed wave shuffles
+// and one LDS round for the cross-wave part.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -11,21 +13,27 @@ namespace dte {
 constexpr int kWave = 64;
 constexpr int kWaves = 256 / kWave;
 
+template <int TEAM>
+__device__ __forceinline__ void team_sync() {
+  if constexpr (TEAM == kWave) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <int TEAM>
+__device__ __forceinline__ int team_tid() {
+  return TEAM == kWave ? (int)(threadIdx.x & (kWave - 1)) : (int)threadIdx.x;
+}
+
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll
   for (int off = 1; off < kWave; off <<= 1) {
     uint32_t y = __shfl_up(x, off, kWave);
-    if (lane >= off) x += y;
-  }
-  return x;
-}
-
-__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t x) {
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    uint64_t y = __shfl_up(x, off, kWave);
     if (lane >= off) x += y;
   }
   return x;
@@ -41,92 +49,131 @@ __device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t x) {
   return x;
 }
 
-// Exclusive scan over the block; s_w holds kWaves words.  Two barriers.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+// Exclusive scan over the team; s_w holds kWaves words (unused for a wave
+// team).
+template <int TEAM>
+__device__ __forceinline__ uint32_t team_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
   const uint32_t inc = wave_incl_scan(v);
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  if (lane == kWave - 1) s_w[w] = inc;
-  __syncthreads();
-  uint32_t pre = 0, tot = 0;
+  if constexpr (TEAM == kWave) {
+    if (total) *total = __shfl(inc, kWave - 1, kWave);
+    return inc - v;
+  } else {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == kWave - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
 #pragma unroll
-  for (int i = 0; i < kWaves; ++i) {
-    const uint32_t s = s_w[i];
-    pre += (i < w) ? s : 0u;
-    tot += s;
+    for (int i = 0; i < kWaves; ++i) {
+      const uint32_t s = s_w[i];
+      pre += (i < w) ? s : 0u;
+      tot += s;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return pre + inc - v;
   }
-  __syncthreads();
-  if (total) *total = tot;
-  return pre + inc - v;
 }
 
-__device__ __forceinline__ int64_t block_excl_scan_i64(int64_t v, int64_t* s_w, int64_t* total) {
+template <int TEAM>
+__device__ __forceinline__ int64_t team_excl_scan_i64(int64_t v, int64_t* s_w, int64_t* total) {
   const int64_t inc = wave_incl_scan_i64(v);
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  if (lane == kWave - 1) s_w[w] = inc;
-  __syncthreads();
-  int64_t pre = 0, tot = 0;
+  if constexpr (TEAM == kWave) {
+    if (total) *total = __shfl(inc, kWave - 1, kWave);
+    return inc - v;
+  } else {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == kWave - 1) s_w[w] = inc;
+    __syncthreads();
+    int64_t pre = 0, tot = 0;
 #pragma unroll
-  for (int i = 0; i < kWaves; ++i) {
-    const int64_t s = s_w[i];
-    pre += (i < w) ? s : 0;
-    tot += s;
+    for (int i = 0; i < kWaves; ++i) {
+      const int64_t s = s_w[i];
+      pre += (i < w) ? s : 0;
+      tot += s;
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return pre + inc - v;
   }
-  __syncthreads();
-  if (total) *total = tot;
-  return pre + inc - v;
 }
 
 // Exclusive scan of N lanes of counters at once; s_w holds N * kWaves words.
-template <int N>
-__device__ __forceinline__ void block_excl_scan_vec(uint32_t (&v)[N], uint32_t* s_w,
-                                                     uint32_t (&tot)[N]) {
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+template <int TEAM, int N>
+__device__ __forceinline__ void team_excl_scan_vec(uint32_t (&v)[N], uint32_t* s_w, uint32_t (&tot)[N]) {
   uint32_t inc[N];
 #pragma unroll
-  for (int c = 0; c < N; ++c) {
-    inc[c] = wave_incl_scan(v[c]);
-    if (lane == kWave - 1) s_w[c * kWaves + w] = inc[c];
-  }
-  __syncthreads();
+  for (int c = 0; c < N; ++c) inc[c] = wave_incl_scan(v[c]);
+  if constexpr (TEAM == kWave) {
 #pragma unroll
-  for (int c = 0; c < N; ++c) {
-    uint32_t pre = 0, t = 0;
-#pragma unroll
-    for (int i = 0; i < kWaves; ++i) {
-      const uint32_t s = s_w[c * kWaves + i];
-      pre += (i < w) ? s : 0u;
-      t += s;
+    for (int c = 0; c < N; ++c) {
+      tot[c] = __shfl(inc[c], kWave - 1, kWave);
+      v[c] = inc[c] - v[c];
     }
-    tot[c] = t;
-    v[c] = pre + inc[c] - v[c];
+  } else {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int c = 0; c < N; ++c)
+      if (lane == kWave - 1) s_w[c * kWaves + w] = inc[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      uint32_t pre = 0, t = 0;
+#pragma unroll
+      for (int i = 0; i < kWaves; ++i) {
+        const uint32_t s = s_w[c * kWaves + i];
+        pre += (i < w) ? s : 0u;
+        t += s;
+      }
+      tot[c] = t;
+      v[c] = pre + inc[c] - v[c];
+    }
+    __syncthreads();
   }
-  __syncthreads();
 }
 
-__device__ __forceinline__ uint32_t block_reduce_max(uint32_t v, uint32_t* s_w) {
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, kWave));
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  if (lane == 0) s_w[w] = v;
-  __syncthreads();
-  uint32_t r = s_w[0];
-#pragma unroll
-  for (int i = 1; i < kWaves; ++i) r = max(r, s_w[i]);
-  __syncthreads();
-  return r;
+  return v;
 }
-
-__device__ __forceinline__ uint32_t block_reduce_min(uint32_t v, uint32_t* s_w) {
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, kWave));
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  if (lane == 0) s_w[w] = v;
-  __syncthreads();
-  uint32_t r = s_w[0];
+  return v;
+}
+
+template <int TEAM>
+__device__ __forceinline__ uint32_t team_reduce_max(uint32_t v, uint32_t* s_w) {
+  v = wave_max_u32(v);
+  if constexpr (TEAM == kWave) {
+    return v;
+  } else {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) s_w[w] = v;
+    __syncthreads();
+    uint32_t r = s_w[0];
 #pragma unroll
-  for (int i = 1; i < kWaves; ++i) r = min(r, s_w[i]);
-  __syncthreads();
-  return r;
+    for (int i = 1; i < kWaves; ++i) r = max(r, s_w[i]);
+    __syncthreads();
+    return r;
+  }
+}
+
+template <int TEAM>
+__device__ __forceinline__ uint32_t team_reduce_min(uint32_t v, uint32_t* s_w) {
+  v = wave_min_u32(v);
+  if constexpr (TEAM == kWave) {
+    return v;
+  } else {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) s_w[w] = v;
+    __syncthreads();
+    uint32_t r = s_w[0];
+#pragma unroll
+    for (int i = 1; i < kWaves; ++i) r = min(r, s_w[i]);
+    __syncthreads();
+    return r;
+  }
 }
 
 // Best split candidate: larger gain wins, equal gain -> smaller key (lower
@@ -156,38 +203,42 @@ __device__ __forceinline__ BestCand wave_best(BestCand b) {
   return b;
 }
 
-// Block reduction of the best candidate; every thread gets the result.
-__device__ __forceinline__ BestCand block_best(BestCand b, double* s_g, uint32_t* s_k, uint32_t* s_p) {
+// Team reduction of the best candidate; every lane gets the result.
+template <int TEAM>
+__device__ __forceinline__ BestCand team_best(BestCand b, double* s_g, uint32_t* s_k, uint32_t* s_p) {
   b = wave_best(b);
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  if (lane == 0) {
-    s_g[w] = b.g;
-    s_k[w] = b.key;
-    s_p[w] = b.prev;
-  }
-  __syncthreads();
-  BestCand r{s_g[0], s_k[0], s_p[0]};
+  if constexpr (TEAM == kWave) {
+    return b;
+  } else {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) {
+      s_g[w] = b.g;
+      s_k[w] = b.key;
+      s_p[w] = b.prev;
+    }
+    __syncthreads();
+    BestCand r{s_g[0], s_k[0], s_p[0]};
 #pragma unroll
-  for (int i = 1; i < kWaves; ++i)
-    if (cand_better(s_g[i], s_k[i], r.g, r.key)) r = BestCand{s_g[i], s_k[i], s_p[i]};
-  __syncthreads();
-  return r;
+    for (int i = 1; i < kWaves; ++i)
+      if (cand_better(s_g[i], s_k[i], r.g, r.key)) r = BestCand{s_g[i], s_k[i], s_p[i]};
+    __syncthreads();
+    return r;
+  }
 }
 
-// One stable LSD pass over n <= 256*IPT (key, val) pairs held in LDS, on the
-// 4-bit digit at `shift`.  Thread t owns positions [t*IPT, t*IPT+IPT).
-// s_cnt: 16 * 256 uint16 counters.  Reads (kin, vin), writes (kout, vout).
-template <int IPT>
+// One stable LSD pass over n <= TEAM*IPT (key, val) pairs held in LDS, on the
+// 4-bit digit at `shift`.  Lane t owns positions [t*IPT, t*IPT+IPT).
+// s_cnt: 16 * TEAM uint16 counters.  Reads (kin, vin), writes (kout, vout).
+template <int IPT, int TEAM>
 __device__ __forceinline__ void lds_sort_pass(const uint32_t* kin, const uint16_t* vin, uint32_t* kout,
-                                              uint16_t* vout, int n, int shift, uint16_t* s_cnt,
-                                              uint32_t* s_w) {
-  const int t = threadIdx.x;
+                                              uint16_t* vout, int n, int shift, uint16_t* s_cnt, uint32_t* s_w) {
+  const int t = team_tid<TEAM>();
   const int lo = t * IPT;
   uint32_t k[IPT];
   uint16_t v[IPT];
   uint32_t d[IPT];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) s_cnt[i * 256 + t] = 0;
+  for (int i = 0; i < 16; ++i) s_cnt[i * TEAM + t] = 0;
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const int p = lo + i;
@@ -195,12 +246,12 @@ __device__ __forceinline__ void lds_sort_pass(const uint32_t* kin, const uint16_
       k[i] = kin[p];
       v[i] = vin[p];
       d[i] = (k[i] >> shift) & 15u;
-      s_cnt[d[i] * 256 + t] += 1;
+      s_cnt[d[i] * TEAM + t] += 1;
     }
   }
-  __syncthreads();
-  // Exclusive scan of the 16 x 256 counter matrix in digit-major order.
-  // Thread t covers flat entries [16t, 16t+16).
+  team_sync<TEAM>();
+  // Exclusive scan of the 16 x TEAM counter matrix in digit-major order; lane
+  // t covers flat entries [16t, 16t+16).
   uint32_t loc[16];
   uint32_t sum = 0;
 #pragma unroll
@@ -208,24 +259,33 @@ __device__ __forceinline__ void lds_sort_pass(const uint32_t* kin, const uint16_
     loc[i] = s_cnt[t * 16 + i];
     sum += loc[i];
   }
-  uint32_t base = block_excl_scan(sum, s_w, nullptr);
+  uint32_t base = team_excl_scan<TEAM>(sum, s_w, nullptr);
+  team_sync<TEAM>();
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     s_cnt[t * 16 + i] = (uint16_t)base;
     base += loc[i];
   }
-  __syncthreads();
+  team_sync<TEAM>();
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const int p = lo + i;
     if (p < n) {
-      const uint32_t pos = s_cnt[d[i] * 256 + t];
-      s_cnt[d[i] * 256 + t] = (uint16_t)(pos + 1);
+      const uint32_t pos = s_cnt[d[i] * TEAM + t];
+      s_cnt[d[i] * TEAM + t] = (uint16_t)(pos + 1);
       kout[pos] = k[i];
       vout[pos] = v[i];
     }
   }
-  __syncthreads();
+  team_sync<TEAM>();
+}
+
+// Compatibility names used by the whole-block kernels.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  return team_excl_scan<256>(v, s_w, total);
+}
+__device__ __forceinline__ int64_t block_excl_scan_i64(int64_t v, int64_t* s_w, int64_t* total) {
+  return team_excl_scan_i64<256>(v, s_w, total);
 }
 
 }  // namespace dte

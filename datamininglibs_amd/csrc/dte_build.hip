@@ -30,6 +30,8 @@
 #include <cstring>
 #include <numeric>
 #include <thread>
+#include <cstdlib>
+#include <chrono>
 
 #include "dte_block.h"
 #include "dte_internal.h"
@@ -37,6 +39,7 @@
 namespace dte {
 
 __device__ LnTable g_ln_table;
+const LnTable& host_ln_table();
 
 namespace {
 
@@ -66,6 +69,8 @@ struct DataArgs {
   uint64_t seed;
   int min_node;
   int max_depth;
+  int igF;           // fixed-point shift of the x ln x terms (per fit)
+  const int64_t* T;  // T[x] = xlnx_fixed(x, igF) for x < kTT
 };
 
 struct LevelArgs {
@@ -101,12 +106,26 @@ struct ItemArgs {
 };
 
 // ------------------------------------------------- per-node context -----
+// Fixed-point x ln x with the fit-wide shift F: table lookup for small
+// counts (LDS copy of T[x] = xlnx_fixed(x, F) for x < kTT), computed
+// otherwise -- the same bits either way.
+constexpr int kTT = 1024;
+
+struct LnCtx {
+  const double* inv;
+  const double* logc;
+  const int64_t* T;
+  int F;
+  __device__ __forceinline__ int64_t t(uint64_t x) const {
+    return x < (uint64_t)kTT ? T[x] : xlnx_fixed(x, F, inv, logc);
+  }
+};
+
 struct NodeCtx {
   int64_t n;        // sum of weights
   int64_t ssum;     // regression: fixed-point target sum
   int64_t P;        // classification: T(n) - sum_c T(n_c)
   double inv;       // 1 / (n ln 2)
-  int F;            // fixed-point shift of the x ln x terms
   int C;
 };
 
@@ -121,144 +140,155 @@ struct Carry {
   uint32_t best_prev;
 };
 
-// Shared scratch of the classification evaluators.
-template <int MAXC>
-struct ClsScratch {
-  uint32_t* pre;     // [MAXC][kBlock] per-thread class counters (columns)
-  uint32_t* ncls;    // [MAXC] node class counts
-  uint32_t* ctot;    // [MAXC] chunk totals
+// Per-team LDS of the sorted-chunk evaluator.
+template <int MAXC, int TASK, int TEAM, int IPT>
+struct TeamLds {
+  static constexpr int CAP = TEAM * IPT;
+  union {
+    struct {
+      uint32_t k[2][CAP];
+      uint16_t v[2][CAP];
+      uint16_t cnt[16 * TEAM];
+    } srt;
+    uint32_t hist[TEAM == 256 ? kHistWords : 1];
+    uint32_t bpos[TEAM == 256 ? kMaxBuckets : 1];
+  } u;
+  uint32_t aux[CAP];
+  int64_t y[TASK == 1 ? CAP : 1];
+  uint32_t pre[TASK == 0 ? MAXC * TEAM : 1];  // per-lane class columns
+  uint32_t ncls[MAXC];
+  uint32_t ctot[MAXC];
+  uint32_t last[TEAM];
+  uint32_t w[4 * kWaves + kWaves];
+  int64_t w64[kWaves];
+  double bg[kWaves];
+  uint32_t bk[kWaves];
+  uint32_t bp[kWaves];
+  Carry<MAXC, TASK> cr;
 };
 
-// ------------------------------------------------- chunk evaluation -----
-// Evaluate every boundary of a key-sorted chunk held in LDS.  keys[] are
-// relative to kbase; vals[] index the chunk's payload (s_aux, s_y).
-// Exclusive scan of the per-thread class columns pre[c][t] across the block,
+// Exclusive scan of the per-lane class columns pre[c][t] across the team,
 // plus the carry; chunk totals go to ctot[c].
-template <int MAXC, int TASK>
-__device__ void scan_class_columns(const ClsScratch<MAXC>& cs, int C, Carry<MAXC, TASK>* cr, uint32_t* s_w) {
-  const int t = threadIdx.x;
+template <int MAXC, int TASK, int TEAM, int IPT>
+__device__ void scan_class_columns(TeamLds<MAXC, TASK, TEAM, IPT>& L, int C) {
+  const int t = team_tid<TEAM>();
   for (int c0 = 0; c0 < C; c0 += 4) {
     uint32_t v[4], tot[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = (c0 + q < C) ? cs.pre[(c0 + q) * kBlock + t] : 0u;
-    block_excl_scan_vec<4>(v, s_w, tot);
+    for (int q = 0; q < 4; ++q) v[q] = (c0 + q < C) ? L.pre[(c0 + q) * TEAM + t] : 0u;
+    team_excl_scan_vec<TEAM, 4>(v, L.w, tot);
 #pragma unroll
     for (int q = 0; q < 4; ++q)
       if (c0 + q < C) {
-        cs.pre[(c0 + q) * kBlock + t] = v[q] + cr->run[c0 + q];
-        if (t == 0) cs.ctot[c0 + q] = tot[q];
+        L.pre[(c0 + q) * TEAM + t] = v[q] + L.cr.run[c0 + q];
+        if (t == 0) L.ctot[c0 + q] = tot[q];
       }
   }
 }
 
-// Incremental state of one thread's walk: exact fixed-point sums of the left
+// Incremental state of one lane's walk: exact fixed-point sums of the left
 // and right x ln x terms.
 struct WalkState {
   int64_t sl, sr;
   int64_t nl;
 };
 
-template <int MAXC>
-__device__ __forceinline__ WalkState walk_init(const ClsScratch<MAXC>& cs, const NodeCtx& nc,
-                                               const double* s_inv, const double* s_logc) {
-  const int t = threadIdx.x;
+template <int MAXC, int TASK, int TEAM, int IPT>
+__device__ __forceinline__ WalkState walk_init(const TeamLds<MAXC, TASK, TEAM, IPT>& L, const NodeCtx& nc,
+                                               const LnCtx& lx) {
+  const int t = team_tid<TEAM>();
   WalkState w{0, 0, 0};
   for (int c = 0; c < nc.C; ++c) {
-    const uint32_t l = cs.pre[c * kBlock + t];
+    const uint32_t l = L.pre[c * TEAM + t];
     w.nl += l;
-    w.sl += xlnx_fixed(l, nc.F, s_inv, s_logc);
-    w.sr += xlnx_fixed(cs.ncls[c] - l, nc.F, s_inv, s_logc);
+    w.sl += lx.t(l);
+    w.sr += lx.t(L.ncls[c] - l);
   }
   return w;
 }
 
-__device__ __forceinline__ double walk_gain(const WalkState& w, const NodeCtx& nc, const double* s_inv,
-                                            const double* s_logc) {
-  const int64_t q = (w.sl - xlnx_fixed((uint64_t)w.nl, nc.F, s_inv, s_logc)) +
-                    (w.sr - xlnx_fixed((uint64_t)(nc.n - w.nl), nc.F, s_inv, s_logc));
-  return ig_from_fixed(nc.P + q, nc.F, nc.inv);
+__device__ __forceinline__ double walk_gain(const WalkState& w, const NodeCtx& nc, const LnCtx& lx) {
+  const int64_t q = (w.sl - lx.t((uint64_t)w.nl)) + (w.sr - lx.t((uint64_t)(nc.n - w.nl)));
+  return ig_from_fixed(nc.P + q, lx.F, nc.inv);
 }
 
-template <int MAXC>
-__device__ __forceinline__ void walk_add(WalkState& w, const ClsScratch<MAXC>& cs, int lab, uint32_t add,
-                                         const NodeCtx& nc, const double* s_inv, const double* s_logc) {
-  const int t = threadIdx.x;
-  const uint32_t lo = cs.pre[lab * kBlock + t];
+template <int MAXC, int TASK, int TEAM, int IPT>
+__device__ __forceinline__ void walk_add(WalkState& w, TeamLds<MAXC, TASK, TEAM, IPT>& L, int lab, uint32_t add,
+                                         const LnCtx& lx) {
+  const int t = team_tid<TEAM>();
+  const uint32_t lo = L.pre[lab * TEAM + t];
   const uint32_t ln = lo + add;
-  cs.pre[lab * kBlock + t] = ln;
-  const uint32_t tot = cs.ncls[lab];
-  w.sl += xlnx_fixed(ln, nc.F, s_inv, s_logc) - xlnx_fixed(lo, nc.F, s_inv, s_logc);
-  w.sr += xlnx_fixed(tot - ln, nc.F, s_inv, s_logc) - xlnx_fixed(tot - lo, nc.F, s_inv, s_logc);
+  L.pre[lab * TEAM + t] = ln;
+  const uint32_t tot = L.ncls[lab];
+  w.sl += lx.t(ln) - lx.t(lo);
+  w.sr += lx.t(tot - ln) - lx.t(tot - lo);
   w.nl += add;
 }
 
 template <int MAXC, int TASK>
-__device__ __forceinline__ void carry_best(Carry<MAXC, TASK>* cr, const BestCand& b) {
-  if (b.key != kNone && cand_better(b.g, b.key, cr->best_g, cr->best_key)) {
-    cr->best_g = b.g;
-    cr->best_key = b.key;
-    cr->best_prev = b.prev;
+__device__ __forceinline__ void carry_best(Carry<MAXC, TASK>& cr, const BestCand& b) {
+  if (b.key != kNone && cand_better(b.g, b.key, cr.best_g, cr.best_key)) {
+    cr.best_g = b.g;
+    cr.best_key = b.key;
+    cr.best_prev = b.prev;
   }
 }
 
-// Evaluate every boundary of a key-sorted chunk held in LDS.  keys[] are
-// relative to kbase; vals[] index the chunk's payload (s_aux, s_y).
-template <int MAXC, int TASK>
-__device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, const uint32_t* s_aux,
-                                  const int64_t* s_y, int n, uint32_t kbase, const NodeCtx& nc,
-                                  Carry<MAXC, TASK>* cr, const ClsScratch<MAXC>& cs, const double* s_inv,
-                                  const double* s_logc, uint32_t* s_w, int64_t* s_w64, double* s_bg,
-                                  uint32_t* s_bk, uint32_t* s_bp) {
-  const int t = threadIdx.x;
-  const int lo = t * kIpt;
-  const int hi = min(lo + kIpt, n);
+// Evaluate every boundary of a key-sorted chunk (keys relative to kbase,
+// vals index the chunk's payload in L.aux / L.y).
+template <int MAXC, int TASK, int TEAM, int IPT>
+__device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, int n, uint32_t kbase,
+                                  const NodeCtx& nc, TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx) {
+  const int t = team_tid<TEAM>();
+  const int lo = t * IPT;
+  const int hi = min(lo + IPT, n);
   BestCand tb{0.0, kNone, kNone};
-  if (TASK == 0) {
-    for (int c = 0; c < nc.C; ++c) cs.pre[c * kBlock + t] = 0;
+  if constexpr (TASK == 0) {
+    for (int c = 0; c < nc.C; ++c) L.pre[c * TEAM + t] = 0;
     for (int p = lo; p < hi; ++p) {
-      const uint32_t a = s_aux[vals[p]];
-      cs.pre[(a >> 16) * kBlock + t] += a & 0xFFFFu;
+      const uint32_t a = L.aux[vals[p]];
+      L.pre[(a >> 16) * TEAM + t] += a & 0xFFFFu;
     }
-    scan_class_columns<MAXC, TASK>(cs, nc.C, cr, s_w);
+    scan_class_columns<MAXC, TASK, TEAM, IPT>(L, nc.C);
     if (lo < hi) {
-      WalkState w = walk_init<MAXC>(cs, nc, s_inv, s_logc);
-      uint32_t prevk = (lo > 0) ? keys[lo - 1] + kbase : cr->last_key;
+      WalkState w = walk_init<MAXC, TASK, TEAM, IPT>(L, nc, lx);
+      uint32_t prevk = (lo > 0) ? keys[lo - 1] + kbase : L.cr.last_key;
       for (int p = lo; p < hi; ++p) {
         const uint32_t k = keys[p] + kbase;
         if (w.nl > 0 && prevk < k) {
-          const double g = walk_gain(w, nc, s_inv, s_logc);
+          const double g = walk_gain(w, nc, lx);
           if (g > tb.g) tb = BestCand{g, k, prevk};
         }
-        const uint32_t a = s_aux[vals[p]];
-        walk_add<MAXC>(w, cs, (int)(a >> 16), a & 0xFFFFu, nc, s_inv, s_logc);
+        const uint32_t a = L.aux[vals[p]];
+        walk_add<MAXC, TASK, TEAM, IPT>(w, L, (int)(a >> 16), a & 0xFFFFu, lx);
         prevk = k;
       }
     }
-    const BestCand b = block_best(tb, s_bg, s_bk, s_bp);
+    const BestCand b = team_best<TEAM>(tb, L.bg, L.bk, L.bp);
     if (t == 0) {
-      carry_best(cr, b);
+      carry_best(L.cr, b);
       for (int c = 0; c < nc.C; ++c) {
-        cr->run[c] += cs.ctot[c];
-        cr->run_n += cs.ctot[c];
+        L.cr.run[c] += L.ctot[c];
+        L.cr.run_n += L.ctot[c];
       }
-      if (n > 0) cr->last_key = keys[n - 1] + kbase;
+      if (n > 0) L.cr.last_key = keys[n - 1] + kbase;
     }
-    __syncthreads();
+    team_sync<TEAM>();
   } else {
     int64_t ln = 0, ls = 0;
     for (int p = lo; p < hi; ++p) {
       const uint16_t v = vals[p];
-      const int64_t w = (int64_t)(s_aux[v] & 0xFFFFu);
+      const int64_t w = (int64_t)(L.aux[v] & 0xFFFFu);
       ln += w;
-      ls += w * s_y[v];
+      ls += w * L.y[v];
     }
     int64_t tot_n, tot_s;
-    int64_t pn = block_excl_scan_i64(ln, s_w64, &tot_n);
-    int64_t ps = block_excl_scan_i64(ls, s_w64, &tot_s);
-    pn += cr->run_n;
-    ps += cr->run_s;
+    int64_t pn = team_excl_scan_i64<TEAM>(ln, L.w64, &tot_n);
+    int64_t ps = team_excl_scan_i64<TEAM>(ls, L.w64, &tot_s);
+    pn += L.cr.run_n;
+    ps += L.cr.run_s;
     if (lo < hi) {
-      uint32_t prevk = (lo > 0) ? keys[lo - 1] + kbase : cr->last_key;
+      uint32_t prevk = (lo > 0) ? keys[lo - 1] + kbase : L.cr.last_key;
       for (int p = lo; p < hi; ++p) {
         const uint32_t k = keys[p] + kbase;
         if (pn > 0 && prevk < k) {
@@ -266,40 +296,39 @@ __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, co
           if (g > tb.g) tb = BestCand{g, k, prevk};
         }
         const uint16_t v = vals[p];
-        const int64_t w = (int64_t)(s_aux[v] & 0xFFFFu);
+        const int64_t w = (int64_t)(L.aux[v] & 0xFFFFu);
         pn += w;
-        ps += w * s_y[v];
+        ps += w * L.y[v];
         prevk = k;
       }
     }
-    const BestCand b = block_best(tb, s_bg, s_bk, s_bp);
+    const BestCand b = team_best<TEAM>(tb, L.bg, L.bk, L.bp);
     if (t == 0) {
-      carry_best(cr, b);
-      cr->run_n += tot_n;
-      cr->run_s += tot_s;
-      if (n > 0) cr->last_key = keys[n - 1] + kbase;
+      carry_best(L.cr, b);
+      L.cr.run_n += tot_n;
+      L.cr.run_s += tot_s;
+      if (n > 0) L.cr.last_key = keys[n - 1] + kbase;
     }
-    __syncthreads();
+    team_sync<TEAM>();
   }
 }
 
-// Evaluate one bucket of W key slots by an LDS histogram (used when the
-// bucket holds more elements than the sort capacity: heavy ties).
-// Classification: hist[slot * C + class]; regression: counts in words
-// [0, W), 64-bit sums from word W on.
-template <int MAXC, int TASK>
+// One bucket of W key slots through an LDS histogram (bucket holds more
+// elements than the sort capacity: heavy ties).  Classification:
+// hist[slot * C + class]; regression: counts in words [0, W), 64-bit sums
+// from word W on.  Whole-workgroup teams only.
+template <int MAXC, int TASK, int IPT>
 __device__ void eval_hist_bucket(const uint32_t* skey, const uint32_t* saux, const int64_t* sy, int64_t n_el,
-                                 uint32_t kbase, int W, uint32_t* s_hist, const NodeCtx& nc,
-                                 Carry<MAXC, TASK>* cr, const ClsScratch<MAXC>& cs, const double* s_inv,
-                                 const double* s_logc, uint32_t* s_w, int64_t* s_w64, double* s_bg,
-                                 uint32_t* s_bk, uint32_t* s_bp, uint32_t* s_last) {
+                                 uint32_t kbase, int W, const NodeCtx& nc, TeamLds<MAXC, TASK, 256, IPT>& L,
+                                 const LnCtx& lx) {
   const int t = threadIdx.x;
   const int C = nc.C;
+  uint32_t* s_hist = L.u.hist;
   const int zero_words = (TASK == 0) ? W * C : 3 * W;
-  for (int i = t; i < zero_words; i += kBlock) s_hist[i] = 0;
+  for (int i = t; i < zero_words; i += 256) s_hist[i] = 0;
   __syncthreads();
   unsigned long long* hsum = (unsigned long long*)(s_hist + W);
-  for (int64_t e = t; e < n_el; e += kBlock) {
+  for (int64_t e = t; e < n_el; e += 256) {
     const uint32_t slot = skey[e] - kbase;
     const uint32_t a = saux[e];
     const uint32_t w = a & 0xFFFFu;
@@ -311,54 +340,54 @@ __device__ void eval_hist_bucket(const uint32_t* skey, const uint32_t* saux, con
     }
   }
   __syncthreads();
-  const int spt = (W + kBlock - 1) / kBlock;
+  const int spt = (W + 255) / 256;
   const int s0 = t * spt, s1 = min(s0 + spt, W);
   uint32_t my_last = 0;  // last non-empty slot key + 1 in my range, 0 = none
   BestCand tb{0.0, kNone, kNone};
-  if (TASK == 0) {
-    for (int c = 0; c < C; ++c) cs.pre[c * kBlock + t] = 0;
+  if constexpr (TASK == 0) {
+    for (int c = 0; c < C; ++c) L.pre[c * 256 + t] = 0;
     for (int sl = s0; sl < s1; ++sl) {
       uint32_t any = 0;
       for (int c = 0; c < C; ++c) {
         const uint32_t h = s_hist[sl * C + c];
-        cs.pre[c * kBlock + t] += h;
+        L.pre[c * 256 + t] += h;
         any |= h;
       }
       if (any) my_last = kbase + sl + 1;
     }
-    s_last[t] = my_last;
-    scan_class_columns<MAXC, TASK>(cs, C, cr, s_w);
+    L.last[t] = my_last;
+    scan_class_columns<MAXC, TASK, 256, IPT>(L, C);
     uint32_t prev_last = 0;
-    for (int i = 0; i < t; ++i) prev_last = max(prev_last, s_last[i]);
+    for (int i = 0; i < t; ++i) prev_last = max(prev_last, L.last[i]);
     if (s0 < s1) {
-      WalkState w = walk_init<MAXC>(cs, nc, s_inv, s_logc);
-      uint32_t prevk = prev_last ? prev_last - 1 : cr->last_key;
+      WalkState w = walk_init<MAXC, TASK, 256, IPT>(L, nc, lx);
+      uint32_t prevk = prev_last ? prev_last - 1 : L.cr.last_key;
       for (int sl = s0; sl < s1; ++sl) {
         uint32_t any = 0;
         for (int c = 0; c < C; ++c) any |= s_hist[sl * C + c];
         if (!any) continue;
         const uint32_t k = kbase + sl;
         if (w.nl > 0 && prevk < k) {
-          const double g = walk_gain(w, nc, s_inv, s_logc);
+          const double g = walk_gain(w, nc, lx);
           if (g > tb.g) tb = BestCand{g, k, prevk};
         }
         for (int c = 0; c < C; ++c) {
           const uint32_t h = s_hist[sl * C + c];
-          if (h) walk_add<MAXC>(w, cs, c, h, nc, s_inv, s_logc);
+          if (h) walk_add<MAXC, TASK, 256, IPT>(w, L, c, h, lx);
         }
         prevk = k;
       }
     }
-    const BestCand b = block_best(tb, s_bg, s_bk, s_bp);
+    const BestCand b = team_best<256>(tb, L.bg, L.bk, L.bp);
     if (t == 0) {
-      carry_best(cr, b);
+      carry_best(L.cr, b);
       uint32_t last = 0;
-      for (int i = 0; i < kBlock; ++i) last = max(last, s_last[i]);
+      for (int i = 0; i < 256; ++i) last = max(last, L.last[i]);
       for (int c = 0; c < C; ++c) {
-        cr->run[c] += cs.ctot[c];
-        cr->run_n += cs.ctot[c];
+        L.cr.run[c] += L.ctot[c];
+        L.cr.run_n += L.ctot[c];
       }
-      if (last) cr->last_key = last - 1;
+      if (last) L.cr.last_key = last - 1;
     }
     __syncthreads();
   } else {
@@ -371,15 +400,15 @@ __device__ void eval_hist_bucket(const uint32_t* skey, const uint32_t* saux, con
         my_last = kbase + sl + 1;
       }
     }
-    s_last[t] = my_last;
+    L.last[t] = my_last;
     int64_t tot_n, tot_s;
-    int64_t pn = block_excl_scan_i64(ln, s_w64, &tot_n);
-    int64_t ps = block_excl_scan_i64(ls, s_w64, &tot_s);
+    int64_t pn = team_excl_scan_i64<256>(ln, L.w64, &tot_n);
+    int64_t ps = team_excl_scan_i64<256>(ls, L.w64, &tot_s);
     uint32_t prev_last = 0;
-    for (int i = 0; i < t; ++i) prev_last = max(prev_last, s_last[i]);
-    pn += cr->run_n;
-    ps += cr->run_s;
-    uint32_t prevk = prev_last ? prev_last - 1 : cr->last_key;
+    for (int i = 0; i < t; ++i) prev_last = max(prev_last, L.last[i]);
+    pn += L.cr.run_n;
+    ps += L.cr.run_s;
+    uint32_t prevk = prev_last ? prev_last - 1 : L.cr.last_key;
     for (int sl = s0; sl < s1; ++sl) {
       const uint32_t hc = s_hist[sl];
       if (!hc) continue;
@@ -392,114 +421,278 @@ __device__ void eval_hist_bucket(const uint32_t* skey, const uint32_t* saux, con
       ps += (int64_t)hsum[sl];
       prevk = k;
     }
-    const BestCand b = block_best(tb, s_bg, s_bk, s_bp);
+    const BestCand b = team_best<256>(tb, L.bg, L.bk, L.bp);
     if (t == 0) {
-      carry_best(cr, b);
+      carry_best(L.cr, b);
       uint32_t last = 0;
-      for (int i = 0; i < kBlock; ++i) last = max(last, s_last[i]);
-      cr->run_n += tot_n;
-      cr->run_s += tot_s;
-      if (last) cr->last_key = last - 1;
+      for (int i = 0; i < 256; ++i) last = max(last, L.last[i]);
+      L.cr.run_n += tot_n;
+      L.cr.run_s += tot_s;
+      if (last) L.cr.last_key = last - 1;
     }
     __syncthreads();
   }
 }
 
-// Sort a chunk (keys relative to kbase, span < 2^bits) held in LDS and
-// evaluate it.  On return the sorted order is in buffer `which`.
-template <int MAXC, int TASK>
-__device__ void sort_and_eval(uint32_t* s_k0, uint32_t* s_k1, uint16_t* s_v0, uint16_t* s_v1,
-                              uint16_t* s_cnt, int n, int bits, uint32_t kbase, const uint32_t* s_aux,
-                              const int64_t* s_y, const NodeCtx& nc, Carry<MAXC, TASK>* cr,
-                              const ClsScratch<MAXC>& cs, const double* s_inv, const double* s_logc,
-                              uint32_t* s_w, int64_t* s_w64, double* s_bg, uint32_t* s_bk, uint32_t* s_bp) {
-  uint32_t* ka = s_k0;
-  uint32_t* kb = s_k1;
-  uint16_t* va = s_v0;
-  uint16_t* vb = s_v1;
+// Sort a chunk (keys relative to kbase, span < 2^bits) held in LDS buffer 0
+// and evaluate it.
+template <int MAXC, int TASK, int TEAM, int IPT>
+__device__ void sort_and_eval(int n, int bits, uint32_t kbase, const NodeCtx& nc,
+                              TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx) {
+  int cur = 0;
   for (int sh = 0; sh < bits; sh += 4) {
-    lds_sort_pass<kIpt>(ka, va, kb, vb, n, sh, s_cnt, s_w);
-    uint32_t* tk = ka;
-    ka = kb;
-    kb = tk;
-    uint16_t* tv = va;
-    va = vb;
-    vb = tv;
+    lds_sort_pass<IPT, TEAM>(L.u.srt.k[cur], L.u.srt.v[cur], L.u.srt.k[cur ^ 1], L.u.srt.v[cur ^ 1], n, sh,
+                             L.u.srt.cnt, L.w);
+    cur ^= 1;
   }
-  eval_sorted_chunk<MAXC, TASK>(ka, va, s_aux, s_y, n, kbase, nc, cr, cs, s_inv, s_logc, s_w, s_w64, s_bg,
-                                s_bk, s_bp);
+  eval_sorted_chunk<MAXC, TASK, TEAM, IPT>(L.u.srt.k[cur], L.u.srt.v[cur], n, kbase, nc, L, lx);
 }
 
 __device__ __forceinline__ int bit_length(uint32_t v) { return v ? 32 - __clz((int)v) : 0; }
 
-// ------------------------------------------------- RF split search ------
-// One block per (node, feature draw) item, grid-stride over items.  Nodes of
-// up to kCap entries are sorted entirely in LDS; larger nodes are bucketed
-// by dense key into a per-block scratch slab first, then consecutive buckets
-// are packed into LDS-sized chunks (a bucket overfull with ties goes through
-// an LDS histogram).  Every boundary between distinct values is scored.
-template <int MAXC, int TASK>
-__global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, ItemArgs it) {
-  __shared__ double s_inv[128], s_logc[128];
-  __shared__ union alignas(16) {
-    struct {
-      uint32_t k[2][kCap];
-      uint16_t v[2][kCap];
-      uint16_t cnt[16 * kBlock];
-    } srt;
-    uint32_t hist[kHistWords];
-    uint32_t bpos[kMaxBuckets];
-  } u;
-  __shared__ uint32_t s_aux[kCap];
-  __shared__ int64_t s_y[TASK == 1 ? kCap : 1];
-  __shared__ uint32_t s_boff[kMaxBuckets + 1];
-  __shared__ uint32_t s_pre[TASK == 0 ? MAXC * kBlock : 1];
-  __shared__ uint32_t s_ncls[MAXC], s_ctot[MAXC];
-  __shared__ uint32_t s_last[kBlock];
-  __shared__ uint32_t s_w[4 * kWaves + kWaves];
-  __shared__ int64_t s_w64[kWaves];
-  __shared__ double s_bg[kWaves];
-  __shared__ uint32_t s_bk[kWaves], s_bp[kWaves];
-  __shared__ Carry<MAXC, TASK> s_cr;
+__device__ __forceinline__ void load_tables(double* s_inv, double* s_logc, int64_t* s_T, const int64_t* T) {
+  for (int i = threadIdx.x; i < 128; i += blockDim.x) {
+    s_inv[i] = g_ln_table.inv[i];
+    s_logc[i] = g_ln_table.logc[i];
+  }
+  if (s_T)
+    for (int i = threadIdx.x; i < kTT; i += blockDim.x) s_T[i] = T[i];
+}
 
-  load_ln_table(s_inv, s_logc);
+// Node statistics of one item, computed by every lane of the team.
+template <int MAXC, int TASK, int TEAM, int IPT>
+__device__ __forceinline__ bool node_context(const DataArgs& d, const LevelArgs& lv, uint32_t i, NodeCtx& nc,
+                                             TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx) {
+  const int t = team_tid<TEAM>();
+  nc.C = d.C;
+  nc.n = lv.count[i];
+  nc.ssum = 0;
+  nc.P = 0;
+  nc.inv = 0.0;
+  if constexpr (TASK == 0) {
+    const uint32_t my = t < d.C ? lv.cls[(int64_t)i * d.C + t] : 0u;
+    if (t < d.C) L.ncls[t] = my;
+    const uint32_t mx = team_reduce_max<TEAM>(my, L.w);
+    const int64_t tn = t < d.C ? lx.t(my) : 0;
+    int64_t tsum = 0;  // sum of T(n_c) over the classes (lanes < C)
+    (void)team_excl_scan_i64<TEAM>(tn, L.w64, &tsum);
+    nc.P = lx.t((uint64_t)nc.n) - tsum;
+    nc.inv = 1.0 / ((double)nc.n * kLn2);
+    return (int64_t)mx == nc.n;  // pure node
+  } else {
+    nc.ssum = lv.sum[i];
+    return false;
+  }
+}
+
+// Evaluate one RF item on a team; returns through L.cr.
+template <int MAXC, int TASK, int TEAM, int IPT>
+__device__ void rf_item(const DataArgs& d, const LevelArgs& lv, const ItemArgs& it, int f, uint32_t i, const NodeCtx& nc,
+                        TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx, uint32_t* s_boff, int team_index) {
+  constexpr int CAP = TeamLds<MAXC, TASK, TEAM, IPT>::CAP;
+  const int t = team_tid<TEAM>();
+  const int m = lv.m[i];
+  const int64_t st = lv.start[i];
+  const uint32_t* Kf = d.K + (int64_t)f * d.N;
+  if (t == 0) {
+    for (int c = 0; c < MAXC; ++c) L.cr.run[c] = 0;
+    L.cr.run_n = 0;
+    L.cr.run_s = 0;
+    L.cr.last_key = 0;
+    L.cr.best_g = 0.0;
+    L.cr.best_key = kNone;
+    L.cr.best_prev = kNone;
+  }
+  if (m <= CAP) {
+    // Gather: independent loads first (rows, payloads), then the keys.
+    uint32_t kmin = kNone, kmax = 0;
+    uint32_t rows[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const int e = t + q * TEAM;
+      if (e < m) {
+        rows[q] = lv.row[st + e];
+        L.aux[e] = lv.aux[st + e];
+        if (TASK == 1) L.y[e] = lv.ey[st + e];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const int e = t + q * TEAM;
+      if (e < m) {
+        const uint32_t key = Kf[rows[q]];
+        L.u.srt.k[0][e] = key;
+        L.u.srt.v[0][e] = (uint16_t)e;
+        kmin = min(kmin, key);
+        kmax = max(kmax, key);
+      }
+    }
+    kmin = team_reduce_min<TEAM>(kmin, L.w);
+    kmax = team_reduce_max<TEAM>(kmax, L.w);
+    for (int e = t; e < m; e += TEAM) L.u.srt.k[0][e] -= kmin;
+    team_sync<TEAM>();
+    sort_and_eval<MAXC, TASK, TEAM, IPT>(m, bit_length(kmax - kmin), kmin, nc, L, lx);
+    return;
+  }
+  if constexpr (TEAM == 256) {
+    // Big node: bucket by dense key into this block's scratch slab, then
+    // pack consecutive buckets into LDS-sized chunks.
+    const uint32_t S = d.S[f];
+    const int words = (TASK == 0) ? d.C : 3;
+    int wbits = 0;
+    while ((1 << (wbits + 1)) * words <= kHistWords) ++wbits;
+    while (((S + (1u << wbits) - 1) >> wbits) > (uint32_t)kMaxBuckets) ++wbits;
+    const int W = 1 << wbits;
+    const int NB = (int)((S + (uint32_t)W - 1) >> wbits);
+    uint32_t* skey = it.scr_key + (int64_t)team_index * it.scr_stride;
+    uint32_t* saux = it.scr_aux + (int64_t)team_index * it.scr_stride;
+    int64_t* sy = TASK == 1 ? it.scr_y + (int64_t)team_index * it.scr_stride : nullptr;
+    for (int b = t; b <= NB; b += 256) s_boff[b] = 0;
+    __syncthreads();
+    constexpr int U = 8;
+    for (int e0 = 0; e0 < m; e0 += 256 * U) {
+      uint32_t rows[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const int e = e0 + t + q * 256;
+        rows[q] = e < m ? lv.row[st + e] : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const int e = e0 + t + q * 256;
+        if (e < m) atomicAdd(&s_boff[Kf[rows[q]] >> wbits], 1u);
+      }
+    }
+    __syncthreads();
+    {
+      const int per = (NB + 255) / 256;
+      const int b0 = t * per, b1 = min(b0 + per, NB);
+      uint32_t loc = 0;
+      for (int b = b0; b < b1; ++b) loc += s_boff[b];
+      uint32_t base = team_excl_scan<256>(loc, L.w, nullptr);
+      for (int b = b0; b < b1; ++b) {
+        const uint32_t c = s_boff[b];
+        s_boff[b] = base;
+        base += c;
+      }
+      if (t == 0) s_boff[NB] = (uint32_t)m;
+      __syncthreads();
+      for (int b = t; b < NB; b += 256) L.u.bpos[b] = s_boff[b];
+      __syncthreads();
+    }
+    for (int e0 = 0; e0 < m; e0 += 256 * U) {
+      uint32_t rows[U], auxv[U], keys[U];
+      int64_t yv[U];
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const int e = e0 + t + q * 256;
+        if (e < m) {
+          rows[q] = lv.row[st + e];
+          auxv[q] = lv.aux[st + e];
+          if (TASK == 1) yv[q] = lv.ey[st + e];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const int e = e0 + t + q * 256;
+        if (e < m) keys[q] = Kf[rows[q]];
+      }
+#pragma unroll
+      for (int q = 0; q < U; ++q) {
+        const int e = e0 + t + q * 256;
+        if (e < m) {
+          const uint32_t pos = atomicAdd(&L.u.bpos[keys[q] >> wbits], 1u);
+          skey[pos] = keys[q];
+          saux[pos] = auxv[q];
+          if (TASK == 1) sy[pos] = yv[q];
+        }
+      }
+    }
+    __syncthreads();
+    int b = 0;
+    while (b < NB) {
+      const uint32_t cb = s_boff[b + 1] - s_boff[b];
+      if (cb == 0) {
+        ++b;
+        continue;
+      }
+      if (cb > (uint32_t)CAP) {
+        eval_hist_bucket<MAXC, TASK, IPT>(skey + s_boff[b], saux + s_boff[b], TASK == 1 ? sy + s_boff[b] : nullptr,
+                                          cb, (uint32_t)b << wbits, W, nc, L, lx);
+        ++b;
+        continue;
+      }
+      int be = b;
+      uint32_t tot = 0;
+      while (be < NB) {
+        const uint32_t c = s_boff[be + 1] - s_boff[be];
+        if (c > (uint32_t)CAP || tot + c > (uint32_t)CAP) break;
+        tot += c;
+        ++be;
+      }
+      const uint32_t e0 = s_boff[b];
+      const uint32_t kbase = (uint32_t)b << wbits;
+      for (int e = t; e < (int)tot; e += 256) {
+        L.u.srt.k[0][e] = skey[e0 + e] - kbase;
+        L.u.srt.v[0][e] = (uint16_t)e;
+        L.aux[e] = saux[e0 + e];
+        if (TASK == 1) L.y[e] = sy[e0 + e];
+      }
+      __syncthreads();
+      sort_and_eval<MAXC, TASK, 256, IPT>((int)tot, bit_length((uint32_t)(be - b) * (uint32_t)W - 1u), kbase, nc,
+                                          L, lx);
+      b = be;
+    }
+  }
+}
+
+template <int MAXC, int TASK, int TEAM, int IPT>
+__device__ __forceinline__ void rf_write_result(const DataArgs& d, const ItemArgs& it, int64_t ridx, int f,
+                                                const Carry<MAXC, TASK>& cr) {
+  float score = 0.0f, thr = 0.0f;
+  if (cr.best_key != kNone) {
+    const float* sv = d.SV + d.SVoff[f];
+    score = (float)(TASK == 0 ? cr.best_g : ldexp_d(cr.best_g, -2 * d.shift));
+    thr = midpoint(sv[cr.best_prev], sv[cr.best_key]);
+  }
+  it.res_score[ridx] = score;
+  it.res_thr[ridx] = thr;
+  it.res_feat[ridx] = f;
+}
+
+// ------------------------------------------------- RF split search ------
+// One team per (node, feature draw) item, grid-stride over items.  TEAM=64:
+// four independent wave teams per workgroup for medium nodes (up to 64*IPT
+// entries).  TEAM=256: whole workgroup; nodes up to 256*IPT entries are
+// sorted in LDS, larger ones are bucketed by dense key into a scratch slab
+// first.  Every boundary between distinct values is scored.
+template <int MAXC, int TASK, int TEAM>
+__global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, ItemArgs it) {
+  constexpr int IPT = kIpt;
+  constexpr int NT = kBlock / TEAM;
+  __shared__ double s_inv[128], s_logc[128];
+  __shared__ int64_t s_T[kTT];
+  __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[NT];
+  __shared__ uint32_t s_boff[TEAM == 256 ? kMaxBuckets + 1 : 1];
+  load_tables(s_inv, s_logc, TASK == 0 ? s_T : nullptr, d.T);
   __syncthreads();
-  const ClsScratch<MAXC> cs{s_pre, s_ncls, s_ctot};
-  const int t = threadIdx.x;
+  const LnCtx lx{s_inv, s_logc, s_T, d.igF};
+  const int team = TEAM == 256 ? 0 : (int)(threadIdx.x / kWave);
+  TeamLds<MAXC, TASK, TEAM, IPT>& L = s_team[team];
+  const int t = team_tid<TEAM>();
+  unsigned long long my_bytes = 0;
   const int64_t n_items = it.n_groups * it.per_group;
-  for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+  const int team_global = (int)blockIdx.x * NT + team;
+  for (int64_t item = team_global; item < n_items; item += (int64_t)gridDim.x * NT) {
     const int64_t grp = item / it.per_group;
     const int j = it.j0 + (int)(item % it.per_group);
     const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
     const uint32_t i = lv.open_idx[o];
-    const int tr = lv.tree[i];
-    const uint32_t nid = lv.id[i];
-    const int f = (int)(rand_u32(d.seed, kStreamFeat, (uint32_t)lv.tree_global[tr], nid, (uint64_t)j) %
+    const int f = (int)(rand_u32(d.seed, kStreamFeat, (uint32_t)lv.tree_global[lv.tree[i]], lv.id[i], (uint64_t)j) %
                         (uint32_t)d.F);
-    const int m = lv.m[i];
-    const int64_t st = lv.start[i];
-    NodeCtx nc;
-    nc.C = d.C;
-    nc.n = lv.count[i];
-    nc.ssum = 0;
-    nc.P = 0;
-    nc.F = 0;
-    nc.inv = 0.0;
-    bool pure = false;
-    if (TASK == 0) {
-      uint32_t mx = 0;
-      for (int c = 0; c < d.C; ++c) mx = max(mx, lv.cls[(int64_t)i * d.C + c]);
-      pure = (int64_t)mx == nc.n;
-      if (t < d.C) s_ncls[t] = lv.cls[(int64_t)i * d.C + t];
-      nc.F = ig_shift((uint64_t)nc.n, d.C, s_inv, s_logc);
-      nc.inv = 1.0 / ((double)nc.n * kLn2);
-      int64_t P = xlnx_fixed((uint64_t)nc.n, nc.F, s_inv, s_logc);
-      for (int c = 0; c < d.C; ++c) P -= xlnx_fixed(lv.cls[(int64_t)i * d.C + c], nc.F, s_inv, s_logc);
-      nc.P = P;
-    } else {
-      nc.ssum = lv.sum[i];
-    }
     const int64_t ridx = (int64_t)o * it.D + j;
+    NodeCtx nc;
+    const bool pure = node_context<MAXC, TASK, TEAM, IPT>(d, lv, i, nc, L, lx);
     if (pure) {
       // Every boundary of a pure node has gain exactly 0: no positive split.
       if (t == 0) {
@@ -507,134 +700,16 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
         it.res_thr[ridx] = 0.0f;
         it.res_feat[ridx] = f;
       }
-      __syncthreads();
+      team_sync<TEAM>();
       continue;
     }
-    if (t == 0) {
-      for (int c = 0; c < MAXC; ++c) s_cr.run[c] = 0;
-      s_cr.run_n = 0;
-      s_cr.run_s = 0;
-      s_cr.last_key = 0;
-      s_cr.best_g = 0.0;
-      s_cr.best_key = kNone;
-      s_cr.best_prev = kNone;
-      if (it.bytes) atomicAdd(it.bytes, (unsigned long long)m * (TASK == 0 ? 12ull : 20ull));
-    }
-    const uint32_t* Kf = d.K + (int64_t)f * d.N;
-    if (m <= kCap) {
-      // -------- LDS path: gather keys, sort, evaluate.
-      uint32_t kmin = kNone, kmax = 0;
-      for (int e = t; e < m; e += kBlock) {
-        const uint32_t row = lv.row[st + e];
-        const uint32_t key = Kf[row];
-        u.srt.k[0][e] = key;
-        u.srt.v[0][e] = (uint16_t)e;
-        s_aux[e] = lv.aux[st + e];
-        if (TASK == 1) s_y[e] = lv.ey[st + e];
-        kmin = min(kmin, key);
-        kmax = max(kmax, key);
-      }
-      kmin = block_reduce_min(kmin, s_w);
-      kmax = block_reduce_max(kmax, s_w);
-      for (int e = t; e < m; e += kBlock) u.srt.k[0][e] -= kmin;
-      __syncthreads();
-      const int bits = bit_length(kmax - kmin);
-      sort_and_eval<MAXC, TASK>(u.srt.k[0], u.srt.k[1], u.srt.v[0], u.srt.v[1], u.srt.cnt, m, bits, kmin,
-                                s_aux, s_y, nc, &s_cr, cs, s_inv, s_logc, s_w, s_w64, s_bg, s_bk, s_bp);
-    } else {
-      // -------- Big path: bucket by dense key into the scratch slab.
-      const uint32_t S = d.S[f];
-      const int words = (TASK == 0) ? d.C : 3;
-      int wbits = 0;
-      while ((1 << (wbits + 1)) * words <= kHistWords) ++wbits;  // W = 2^wbits slots fit the histogram
-      while (((S + (1u << wbits) - 1) >> wbits) > (uint32_t)kMaxBuckets) ++wbits;
-      const int W = 1 << wbits;
-      const int NB = (int)((S + (uint32_t)W - 1) >> wbits);
-      uint32_t* skey = it.scr_key + (int64_t)blockIdx.x * it.scr_stride;
-      uint32_t* saux = it.scr_aux + (int64_t)blockIdx.x * it.scr_stride;
-      int64_t* sy = TASK == 1 ? it.scr_y + (int64_t)blockIdx.x * it.scr_stride : nullptr;
-      for (int b = t; b <= NB; b += kBlock) s_boff[b] = 0;
-      __syncthreads();
-      for (int e = t; e < m; e += kBlock) atomicAdd(&s_boff[Kf[lv.row[st + e]] >> wbits], 1u);
-      __syncthreads();
-      {
-        // exclusive scan of the NB bucket counts in place
-        const int per = (NB + kBlock - 1) / kBlock;
-        const int b0 = t * per, b1 = min(b0 + per, NB);
-        uint32_t loc = 0;
-        for (int b = b0; b < b1; ++b) loc += s_boff[b];
-        uint32_t base = block_excl_scan(loc, s_w, nullptr);
-        for (int b = b0; b < b1; ++b) {
-          const uint32_t c = s_boff[b];
-          s_boff[b] = base;
-          base += c;
-        }
-        if (t == 0) s_boff[NB] = (uint32_t)m;
-        __syncthreads();
-        for (int b = t; b < NB; b += kBlock) u.bpos[b] = s_boff[b];
-        __syncthreads();
-      }
-      for (int e = t; e < m; e += kBlock) {
-        const uint32_t row = lv.row[st + e];
-        const uint32_t key = Kf[row];
-        const uint32_t pos = atomicAdd(&u.bpos[key >> wbits], 1u);
-        skey[pos] = key;
-        saux[pos] = lv.aux[st + e];
-        if (TASK == 1) sy[pos] = lv.ey[st + e];
-      }
-      __syncthreads();
-      int b = 0;
-      while (b < NB) {
-        const uint32_t cb = s_boff[b + 1] - s_boff[b];
-        if (cb == 0) {
-          ++b;
-          continue;
-        }
-        if (cb > (uint32_t)kCap) {
-          eval_hist_bucket<MAXC, TASK>(skey + s_boff[b], saux + s_boff[b], TASK == 1 ? sy + s_boff[b] : nullptr,
-                                       cb, (uint32_t)b << wbits, W, u.hist, nc, &s_cr, cs, s_inv, s_logc, s_w,
-                                       s_w64, s_bg, s_bk, s_bp, s_last);
-          ++b;
-          continue;
-        }
-        int be = b;
-        uint32_t tot = 0;
-        while (be < NB) {
-          const uint32_t c = s_boff[be + 1] - s_boff[be];
-          if (c > (uint32_t)kCap || tot + c > (uint32_t)kCap) break;
-          tot += c;
-          ++be;
-        }
-        const uint32_t e0 = s_boff[b];
-        const uint32_t kbase = (uint32_t)b << wbits;
-        for (int e = t; e < (int)tot; e += kBlock) {
-          u.srt.k[0][e] = skey[e0 + e] - kbase;
-          u.srt.v[0][e] = (uint16_t)e;
-          s_aux[e] = saux[e0 + e];
-          if (TASK == 1) s_y[e] = sy[e0 + e];
-        }
-        __syncthreads();
-        const int bits = bit_length((uint32_t)(be - b) * (uint32_t)W - 1u);
-        sort_and_eval<MAXC, TASK>(u.srt.k[0], u.srt.k[1], u.srt.v[0], u.srt.v[1], u.srt.cnt, (int)tot, bits,
-                                  kbase, s_aux, s_y, nc, &s_cr, cs, s_inv, s_logc, s_w, s_w64, s_bg, s_bk,
-                                  s_bp);
-        b = be;
-      }
-    }
-    __syncthreads();
-    if (t == 0) {
-      float score = 0.0f, thr = 0.0f;
-      if (s_cr.best_key != kNone) {
-        const float* sv = d.SV + d.SVoff[f];
-        score = (float)(TASK == 0 ? s_cr.best_g : ldexp_d(s_cr.best_g, -2 * d.shift));
-        thr = midpoint(sv[s_cr.best_prev], sv[s_cr.best_key]);
-      }
-      it.res_score[ridx] = score;
-      it.res_thr[ridx] = thr;
-      it.res_feat[ridx] = f;
-    }
-    __syncthreads();
+    if (t == 0) my_bytes += (unsigned long long)lv.m[i] * (TASK == 0 ? 12ull : 20ull);
+    rf_item<MAXC, TASK, TEAM, IPT>(d, lv, it, f, i, nc, L, lx, s_boff, (int)blockIdx.x);
+    team_sync<TEAM>();
+    if (t == 0) rf_write_result<MAXC, TASK, TEAM, IPT>(d, it, ridx, f, L.cr);
+    team_sync<TEAM>();
   }
+  if (t == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
 }
 
 // ------------------------------------------------ ERT split search ------
@@ -651,6 +726,7 @@ __global__ void __launch_bounds__(kBlock) k_search_ert(DataArgs d, LevelArgs lv,
   load_ln_table(s_inv, s_logc);
   __syncthreads();
   const int t = threadIdx.x;
+  unsigned long long my_bytes = 0;
   const int64_t n_items = it.n_groups * it.per_group;
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     const int64_t grp = item / it.per_group;
@@ -674,7 +750,7 @@ __global__ void __launch_bounds__(kBlock) k_search_ert(DataArgs d, LevelArgs lv,
         acc = acc + Xf[lv.row[st + pos]];
       }
       s_thr = acc / 10.0f;
-      if (it.bytes) atomicAdd(it.bytes, (unsigned long long)m * (TASK == 0 ? 12ull : 20ull));
+      my_bytes += (unsigned long long)m * (TASK == 0 ? 12ull : 20ull);
     }
     __syncthreads();
     const float thr = s_thr;
@@ -729,7 +805,7 @@ __global__ void __launch_bounds__(kBlock) k_search_ert(DataArgs d, LevelArgs lv,
         score = -FLT_MAX;
       } else if (TASK == 0) {
         const uint32_t* nclsp = lv.cls + (int64_t)i * d.C;
-        const int F = ig_shift((uint64_t)n, d.C, s_inv, s_logc);
+        const int F = d.igF;
         const double inv = 1.0 / ((double)n * kLn2);
         int64_t P = xlnx_fixed((uint64_t)n, F, s_inv, s_logc);
         int64_t sls = 0, srs = 0;
@@ -751,6 +827,270 @@ __global__ void __launch_bounds__(kBlock) k_search_ert(DataArgs d, LevelArgs lv,
       it.res_feat[ridx] = f;
     }
     __syncthreads();
+  }
+  if (t == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
+}
+
+// ------------------------------------------ wave-per-item split search --
+// Small nodes: one wave64 per (node, draw) item, several items per block, no
+// block barriers.  RF: each lane holds one entry (m <= 64); the rank of its
+// key is counted over the wave, ds_permute moves the entries into key order,
+// and the per-class running sums of the fixed-point x ln x terms become
+// wave exclusive scans of per-lane deltas.
+__device__ __forceinline__ uint32_t ds_permute_u32(uint32_t dst_lane, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_ds_permute((int)(dst_lane << 2), (int)v);
+}
+
+__device__ __forceinline__ int64_t wave_excl_i64(int64_t v) { return wave_incl_scan_i64(v) - v; }
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, kWave));
+  return v;
+}
+
+constexpr int kWaveItemMax = 64;  // RF wave path: one entry per lane
+
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_search_rf_wave(DataArgs d, LevelArgs lv, ItemArgs it) {
+  __shared__ double s_inv[128], s_logc[128];
+  __shared__ int64_t s_T[kTT];
+  load_tables(s_inv, s_logc, TASK == 0 ? s_T : nullptr, d.T);
+  __syncthreads();
+  const LnCtx lx{s_inv, s_logc, s_T, d.igF};
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  unsigned long long my_bytes = 0;
+  const int64_t n_items = it.n_groups * it.per_group;
+  for (int64_t item = (int64_t)blockIdx.x * kWaves + wv; item < n_items; item += (int64_t)gridDim.x * kWaves) {
+    const int64_t grp = item / it.per_group;
+    const int j = it.j0 + (int)(item % it.per_group);
+    const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
+    const uint32_t i = lv.open_idx[o];
+    const uint32_t nid = lv.id[i];
+    const int f = (int)(rand_u32(d.seed, kStreamFeat, (uint32_t)lv.tree_global[lv.tree[i]], nid, (uint64_t)j) %
+                        (uint32_t)d.F);
+    const int m = lv.m[i];
+    const int64_t st = lv.start[i];
+    const int64_t n = lv.count[i];
+    const int64_t ridx = (int64_t)o * it.D + j;
+    // node statistics
+    uint32_t ncls_lane = 0;
+    int F = 0;
+    int64_t P = 0, SR0 = 0;
+    double inv = 0.0;
+    if (TASK == 0) {
+      ncls_lane = lane < d.C ? lv.cls[(int64_t)i * d.C + lane] : 0u;
+      if ((int64_t)wave_max_u32(ncls_lane) == n) {
+        if (lane == 0) {
+          it.res_score[ridx] = 0.0f;
+          it.res_thr[ridx] = 0.0f;
+          it.res_feat[ridx] = f;
+        }
+        continue;
+      }
+      F = d.igF;
+      inv = 1.0 / ((double)n * kLn2);
+      SR0 = wave_sum_i64(lane < d.C ? lx.t(ncls_lane) : 0);
+      P = lx.t((uint64_t)n) - SR0;
+    }
+    if (lane == 0) my_bytes += (unsigned long long)m * (TASK == 0 ? 12ull : 20ull);
+    const bool valid = lane < m;
+    uint32_t key = kNone, pay = 0;
+    int64_t yv = 0;
+    if (valid) {
+      const uint32_t row = lv.row[st + lane];
+      pay = lv.aux[st + lane];
+      key = d.K[(int64_t)f * d.N + row];
+      if (TASK == 1) yv = lv.ey[st + lane];
+    }
+    uint32_t cnt = 0;
+    for (int r = 0; r < m; ++r) {
+      const uint32_t kr = __shfl(key, r, kWave);
+      cnt += (kr < key || (kr == key && r < lane)) ? 1u : 0u;
+    }
+    const uint32_t rank = valid ? cnt : (uint32_t)lane;
+    key = ds_permute_u32(rank, key);
+    pay = ds_permute_u32(rank, pay);
+    if (TASK == 1) {
+      const uint32_t lo = ds_permute_u32(rank, (uint32_t)(uint64_t)yv);
+      const uint32_t hi = ds_permute_u32(rank, (uint32_t)((uint64_t)yv >> 32));
+      yv = (int64_t)(((uint64_t)hi << 32) | lo);
+    }
+    const uint32_t w = valid ? (pay & 0xFFFFu) : 0u;
+    const uint32_t prevk = __shfl_up(key, 1, kWave);
+    const bool cand = valid && lane >= 1 && prevk < key;
+    double g = 0.0;
+    if (TASK == 0) {
+      const uint32_t lab = pay >> 16;
+      uint32_t pre = 0;
+      for (int r = 0; r < m; ++r) {
+        const uint32_t pr = __shfl(pay, r, kWave);
+        pre += (r < lane && (pr >> 16) == lab) ? (pr & 0xFFFFu) : 0u;
+      }
+      const uint32_t nc = __shfl(ncls_lane, (int)lab, kWave);
+      int64_t dl = 0, dr = 0;
+      if (valid) {
+        dl = lx.t(pre + w) - lx.t(pre);
+        dr = lx.t(nc - pre - w) - lx.t(nc - pre);
+      }
+      const int64_t sl = wave_excl_i64(dl);
+      const int64_t sr = SR0 + wave_excl_i64(dr);
+      const int64_t nl = wave_excl_i64((int64_t)w);
+      if (cand) {
+        const int64_t q = (sl - lx.t((uint64_t)nl)) + (sr - lx.t((uint64_t)(n - nl)));
+        g = ig_from_fixed(P + q, F, inv);
+      }
+    } else {
+      const int64_t pn = wave_excl_i64((int64_t)w);
+      const int64_t ps = wave_excl_i64((int64_t)w * yv);
+      if (cand) g = variance_gain(pn, ps, n - pn, lv.sum[i] - ps, 0);
+    }
+    BestCand b = (cand && g > 0.0) ? BestCand{g, key, prevk} : BestCand{0.0, kNone, kNone};
+    b = wave_best(b);
+    if (lane == 0) {
+      float score = 0.0f, thr = 0.0f;
+      if (b.key != kNone) {
+        const float* sv = d.SV + d.SVoff[f];
+        score = (float)(TASK == 0 ? b.g : ldexp_d(b.g, -2 * d.shift));
+        thr = midpoint(sv[b.prev], sv[b.key]);
+      }
+      it.res_score[ridx] = score;
+      it.res_thr[ridx] = thr;
+      it.res_feat[ridx] = f;
+    }
+  }
+  if (lane == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
+}
+
+// ERT, one wave per item (any node size; lanes stride over the entries).
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArgs lv, ItemArgs it) {
+  __shared__ double s_inv[128], s_logc[128];
+  load_ln_table(s_inv, s_logc);
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  unsigned long long my_bytes = 0;
+  const int64_t n_items = it.n_groups * it.per_group;
+  for (int64_t item = (int64_t)blockIdx.x * kWaves + wv; item < n_items; item += (int64_t)gridDim.x * kWaves) {
+    const int64_t grp = item / it.per_group;
+    const int j = it.j0 + (int)(item % it.per_group);
+    const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
+    const uint32_t i = lv.open_idx[o];
+    const uint32_t tg = (uint32_t)lv.tree_global[lv.tree[i]];
+    const uint32_t nid = lv.id[i];
+    const int f = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)j) % (uint32_t)d.F);
+    const int m = lv.m[i];
+    const int64_t st = lv.start[i];
+    const int64_t n = lv.count[i];
+    const float* Xf = d.X + (int64_t)f * d.N;
+    float xs = 0.0f;
+    if (lane < 10) {
+      const uint32_t pos = rand_u32(d.seed, kStreamThr, tg, nid, (uint64_t)j * 10 + lane) % (uint32_t)m;
+      xs = Xf[lv.row[st + pos]];
+    }
+    float acc = 0.0f;
+    for (int s = 0; s < 10; ++s) acc = acc + __shfl(xs, s, kWave);
+    const float thr = acc / 10.0f;
+    uint32_t loc[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) loc[c] = 0;
+    int64_t nl = 0, sl = 0;
+    for (int e = lane; e < m; e += kWave) {
+      const uint32_t row = lv.row[st + e];
+      if (side_of(Xf[row], thr) == 0) {
+        const uint32_t a = lv.aux[st + e];
+        const uint32_t w = a & 0xFFFFu;
+        nl += w;
+        if (TASK == 0) {
+          const int lab = (int)(a >> 16);
+#pragma unroll
+          for (int c = 0; c < MAXC; ++c) loc[c] += (c == lab) ? w : 0u;
+        } else {
+          sl += (int64_t)w * lv.ey[st + e];
+        }
+      }
+    }
+    nl = wave_sum_i64(nl);
+    if (lane == 0) my_bytes += (unsigned long long)m * (TASK == 0 ? 12ull : 20ull);
+    float score;
+    if (nl == 0 || nl == n) {
+      score = -FLT_MAX;
+    } else if (TASK == 0) {
+      // lane c keeps class c's left count
+      uint32_t lc = 0;
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c) {
+        uint32_t v = loc[c];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+        lc = (lane == c) ? v : lc;
+      }
+      const int F = d.igF;
+      const double inv = 1.0 / ((double)n * kLn2);
+      int64_t tn = 0, tl = 0, tr = 0;
+      if (lane < d.C) {
+        const uint32_t nc = lv.cls[(int64_t)i * d.C + lane];
+        tn = xlnx_fixed(nc, F, s_inv, s_logc);
+        tl = xlnx_fixed(lc, F, s_inv, s_logc);
+        tr = xlnx_fixed(nc - lc, F, s_inv, s_logc);
+      }
+      const int64_t P = xlnx_fixed((uint64_t)n, F, s_inv, s_logc) - wave_sum_i64(tn);
+      const int64_t Q = (wave_sum_i64(tl) - xlnx_fixed((uint64_t)nl, F, s_inv, s_logc)) +
+                        (wave_sum_i64(tr) - xlnx_fixed((uint64_t)(n - nl), F, s_inv, s_logc));
+      score = (float)ig_from_fixed(P + Q, F, inv);
+    } else {
+      sl = wave_sum_i64(sl);
+      score = (float)variance_gain(nl, sl, n - nl, lv.sum[i] - sl, d.shift);
+    }
+    if (lane == 0) {
+      const int64_t ridx = (int64_t)o * it.D + j;
+      it.res_score[ridx] = score;
+      it.res_thr[ridx] = thr;
+      it.res_feat[ridx] = f;
+    }
+  }
+  if (lane == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
+}
+
+// Split a list of open positions (or all of them) by node size into the
+// wave tier and the block tier.  Order inside a tier is irrelevant: every
+// item writes its own result slot.
+// Wave-aggregated: one global atomic per wave and tier.
+__global__ void k_tier_split(const uint32_t* list, int64_t n, const uint32_t* open_idx, const int32_t* node_m,
+                             int cut0, int cut1, uint32_t* t0, uint32_t* t1, uint32_t* t2, unsigned int* counts) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t base = (blockIdx.x * (int64_t)blockDim.x) & ~(int64_t)(kWave - 1); base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = base + (threadIdx.x & ~(kWave - 1)) + lane;
+    uint32_t o = 0;
+    int tier = -1;
+    if (g < n) {
+      o = list ? list[g] : (uint32_t)g;
+      const int m = node_m[open_idx[o]];
+      tier = m <= cut0 ? 0 : (m <= cut1 ? 1 : 2);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const uint64_t mask = __ballot(tier == k);
+      if (!mask) continue;
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      unsigned int basepos = 0;
+      if (lane == leader) basepos = atomicAdd(&counts[k], (unsigned int)__popcll(mask));
+      basepos = __shfl(basepos, leader, kWave);
+      if (tier == k) {
+        const unsigned int r = (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+        uint32_t* dst = k == 0 ? t0 : (k == 1 ? t1 : t2);
+        dst[basepos + r] = o;
+      }
+    }
   }
 }
 
@@ -877,6 +1217,7 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
   __shared__ unsigned long long s_cnt, s_sum, s_hash;
   __shared__ uint32_t s_ml;
   const int t = threadIdx.x;
+  unsigned long long my_bytes = 0;
   for (int64_t o = blockIdx.x; o < pa.n_open; o += gridDim.x) {
     const uint32_t i = lv.open_idx[o];
     const int m = lv.m[i];
@@ -941,7 +1282,7 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
     const int mL = (int)s_ml;
     const bool ok = mL > 0 && mL < m;
     if (t == 0) {
-      if (pa.bytes) atomicAdd(pa.bytes, (unsigned long long)m * (TASK == 0 ? 12ull : 20ull));
+      my_bytes += (unsigned long long)m * (TASK == 0 ? 12ull : 20ull);
       pa.split_ok[o] = ok ? 1 : 0;
       pa.l_m[o] = mL;
       pa.l_count[o] = (int32_t)s_cnt;
@@ -973,6 +1314,7 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
     }
     __syncthreads();
   }
+  if (t == 0 && my_bytes && pa.bytes) atomicAdd(pa.bytes, my_bytes);
 }
 
 // ------------------------------------------------------ children --------
@@ -1389,13 +1731,24 @@ struct Timer {
   }
 };
 
+// tier: 0 = one wave per item, one entry per lane; 1 = wave teams (four
+// items per workgroup); 2 = whole workgroup per item.
 template <int MAXC, int TASK>
-void launch_search(int algorithm, int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv,
+void launch_search(int algorithm, int tier, int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv,
                    const ItemArgs& it) {
-  if (algorithm == DTE_RANDOM_FOREST)
-    hipLaunchKernelGGL((k_search_rf<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
-  else
-    hipLaunchKernelGGL((k_search_ert<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
+  if (algorithm == DTE_RANDOM_FOREST) {
+    if (tier == 0)
+      hipLaunchKernelGGL((k_search_rf_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
+    else if (tier == 1)
+      hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
+    else
+      hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
+  } else {
+    if (tier == 0)
+      hipLaunchKernelGGL((k_search_ert_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
+    else
+      hipLaunchKernelGGL((k_search_ert<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
+  }
 }
 
 template <int MAXC, int TASK>
@@ -1403,7 +1756,7 @@ void launch_partition(int grid, hipStream_t s, const DataArgs& d, const LevelArg
   hipLaunchKernelGGL((k_partition<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, pa);
 }
 
-using SearchFn = void (*)(int, int, hipStream_t, const DataArgs&, const LevelArgs&, const ItemArgs&);
+using SearchFn = void (*)(int, int, int, hipStream_t, const DataArgs&, const LevelArgs&, const ItemArgs&);
 using PartFn = void (*)(int, hipStream_t, const DataArgs&, const LevelArgs&, const PartArgs&);
 
 SearchFn pick_search(int task, int C) {
@@ -1423,51 +1776,16 @@ PartFn pick_part(int task, int C) {
   return launch_partition<32, 0>;
 }
 
-// Dense ranks of every feature: K[f][row] = index of X[f][row] among the
-// sorted distinct values SV_f.  Computed on the host from the canonical
-// input (std::sort per feature, one thread per feature slice).
-void dense_ranks_host(const std::vector<float>& Xh, int64_t N, int F, std::vector<uint32_t>& K,
-                      std::vector<float>& SV, std::vector<int64_t>& SVoff, std::vector<uint32_t>& S) {
-  K.resize((size_t)N * F);
-  S.resize(F);
-  std::vector<std::vector<float>> sv(F);
-  int nt = (int)std::max(1u, std::thread::hardware_concurrency());
-  nt = std::min(nt, 32);
-  std::vector<std::thread> pool;
-  for (int w = 0; w < nt; ++w) {
-    pool.emplace_back([&, w]() {
-      std::vector<std::pair<float, uint32_t>> tmp((size_t)N);
-      for (int f = w; f < F; f += nt) {
-        const float* col = Xh.data() + (size_t)f * N;
-        for (int64_t r = 0; r < N; ++r) tmp[r] = {col[r], (uint32_t)r};
-        std::sort(tmp.begin(), tmp.end(),
-                  [](const std::pair<float, uint32_t>& a, const std::pair<float, uint32_t>& b) {
-                    return a.first < b.first;
-                  });
-        uint32_t rank = 0;
-        std::vector<float>& out = sv[f];
-        out.clear();
-        for (int64_t i = 0; i < N; ++i) {
-          if (i > 0 && tmp[i - 1].first < tmp[i].first) ++rank;
-          if (i == 0 || tmp[i - 1].first < tmp[i].first) out.push_back(tmp[i].first);
-          K[(size_t)f * N + tmp[i].second] = rank;
-        }
-        S[f] = (uint32_t)out.size();
-      }
-    });
-  }
-  for (auto& th : pool) th.join();
-  SVoff.resize(F);
-  int64_t off = 0;
-  for (int f = 0; f < F; ++f) {
-    SVoff[f] = off;
-    off += S[f];
-  }
-  SV.resize((size_t)off);
-  for (int f = 0; f < F; ++f) std::copy(sv[f].begin(), sv[f].end(), SV.begin() + SVoff[f]);
+}  // namespace
+
+void scan_u32_ext(const uint32_t* in, int64_t n, uint32_t* out, hipStream_t s,
+                  std::vector<DevBuf<uint32_t>>& tmp) {
+  scan_u32(in, n, out, s, tmp);
 }
 
-}  // namespace
+void dense_ranks_device(const float* X, int64_t N, int F, DevBuf<uint32_t>& K, DevBuf<float>& SV,
+                        DevBuf<int64_t>& SVoff, DevBuf<uint32_t>& S, hipStream_t s,
+                        std::vector<DevBuf<uint32_t>>& scan_tmp);
 
 const LnTable& host_ln_table() {
   static LnTable t = [] {
@@ -1530,22 +1848,14 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
   DevBuf<float> SV;
   DevBuf<int64_t> SVoff;
   DevBuf<uint32_t> Sdev;
+  std::vector<DevBuf<uint32_t>> scan_tmp;
   if (algorithm == DTE_RANDOM_FOREST) {
-    std::vector<float> Xh((size_t)N * F);
-    DTE_HIP(hipMemcpyAsync(Xh.data(), X.p, sizeof(float) * N * F, hipMemcpyDeviceToHost, s));
-    DTE_HIP(hipStreamSynchronize(s));
-    std::vector<uint32_t> Kh, Sh;
-    std::vector<float> SVh;
-    std::vector<int64_t> offh;
-    dense_ranks_host(Xh, N, F, Kh, SVh, offh, Sh);
-    K.alloc(Kh.size());
-    SV.alloc(SVh.size());
-    SVoff.alloc(F);
-    Sdev.alloc(F);
-    DTE_HIP(hipMemcpyAsync(K.p, Kh.data(), 4 * Kh.size(), hipMemcpyHostToDevice, s));
-    DTE_HIP(hipMemcpyAsync(SV.p, SVh.data(), 4 * SVh.size(), hipMemcpyHostToDevice, s));
-    DTE_HIP(hipMemcpyAsync(SVoff.p, offh.data(), 8 * F, hipMemcpyHostToDevice, s));
-    DTE_HIP(hipMemcpyAsync(Sdev.p, Sh.data(), 4 * F, hipMemcpyHostToDevice, s));
+    {
+      Timer tm("dense_ranks", 0.0, s);
+      dense_ranks_device(X.p, N, F, K, SV, SVoff, Sdev, s, scan_tmp);
+    }
+    std::vector<uint32_t> Sh(F);
+    DTE_HIP(hipMemcpyAsync(Sh.data(), Sdev.p, 4 * F, hipMemcpyDeviceToHost, s));
     DTE_HIP(hipStreamSynchronize(s));
     for (int f = 0; f < F; ++f) {
       int words = task == 0 ? C : 3;
@@ -1569,13 +1879,21 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
   d.seed = p.seed;
   d.min_node = p.min_node_size;
   d.max_depth = p.max_depth;
+  DevBuf<int64_t> Ttab(kTT);
+  {
+    const LnTable& lt = host_ln_table();
+    d.igF = ig_shift((uint64_t)N, task == 0 ? C : 1, lt.inv, lt.logc);
+    std::vector<int64_t> th(kTT);
+    for (int x = 0; x < kTT; ++x) th[x] = xlnx_fixed((uint64_t)x, d.igF, lt.inv, lt.logc);
+    DTE_HIP(hipMemcpyAsync(Ttab.p, th.data(), 8 * kTT, hipMemcpyHostToDevice, s));
+    d.T = Ttab.p;
+  }
 
   std::vector<int32_t> tg_h(T);
   std::iota(tg_h.begin(), tg_h.end(), 0);
   DevBuf<int32_t> tree_global(T);
   DTE_HIP(hipMemcpyAsync(tree_global.p, tg_h.data(), 4 * T, hipMemcpyHostToDevice, s));
 
-  std::vector<DevBuf<uint32_t>> scan_tmp;
   DevBuf<unsigned long long> search_bytes(1), part_bytes(1);
   DTE_HIP(hipMemsetAsync(search_bytes.p, 0, 8, s));
   DTE_HIP(hipMemsetAsync(part_bytes.p, 0, 8, s));
@@ -1755,6 +2073,39 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
   SearchFn search = pick_search(task, C);
   PartFn partition = pick_part(task, C);
 
+  // Size tiers of the split search: wave per item up to wave_cutoff entries,
+  // block per item above (the block kernel itself switches to the bucketed
+  // path above its LDS capacity).
+  const int cut0 = algorithm == DTE_RANDOM_FOREST ? kWaveItemMax : 4096;
+  const int cut1 = algorithm == DTE_RANDOM_FOREST ? 64 * kIpt : 4096;
+  DevBuf<uint32_t> tier_list[3];
+  DevBuf<unsigned int> tier_cnt(3);
+  const int wave_grid_cap = num_cu * 16;
+  LevelArgs lv{};
+  int64_t trace_tier[3] = {0, 0, 0};
+  auto run_phase = [&](ItemArgs it) {
+    for (auto& tl : tier_list) tl.reserve(it.n_groups);
+    DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 12, s));
+    hipLaunchKernelGGL(k_tier_split, dim3(grid_for(it.n_groups)), dim3(kBlock), 0, s, it.list, it.n_groups,
+                       lv.open_idx, lv.m, cut0, cut1, tier_list[0].p, tier_list[1].p, tier_list[2].p, tier_cnt.p);
+    unsigned int cnt[3];
+    DTE_HIP(hipMemcpyAsync(cnt, tier_cnt.p, 12, hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipStreamSynchronize(s));
+    Timer tm("split_search", 0.0, s);
+    for (int tier = 0; tier < 3; ++tier) {
+      if (!cnt[tier]) continue;
+      ItemArgs a = it;
+      a.list = tier_list[tier].p;
+      a.n_groups = cnt[tier];
+      const int64_t items = (int64_t)cnt[tier] * it.per_group;
+      const int per_block = tier == 2 ? 1 : kWaves;
+      const int cap = tier == 2 ? search_grid_cap : wave_grid_cap;
+      search(algorithm, tier, (int)std::min<int64_t>((items + per_block - 1) / per_block, cap), s, d, lv, a);
+      trace_tier[tier] += cnt[tier];
+    }
+  };
+  const bool trace = std::getenv("DTE_TRACE") != nullptr;
+
   int64_t off_level = 0;
   int64_t n_level = T;
   int64_t max_m = 0;
@@ -1768,6 +2119,12 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
   int64_t* nxt_ey = eyB.p;
 
   for (int depth = 0; n_level > 0; ++depth) {
+    double t_level0 = 0;
+    if (trace) {
+      DTE_HIP(hipStreamSynchronize(s));
+      t_level0 = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+      trace_tier[0] = trace_tier[1] = trace_tier[2] = 0;
+    }
     // ---- open list
     flag32.reserve(n_level);
     open_pos.reserve(n_level + 1);
@@ -1780,7 +2137,7 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
       hipLaunchKernelGGL(k_open_list, dim3(grid_for(n_level)), dim3(kBlock), 0, s, w_open.p,
                          open_pos.p, n_level, open_idx.p);
 
-    LevelArgs lv{};
+    lv = LevelArgs{};
     lv.tree = o_tree.p + off_level;
     lv.id = o_id.p + off_level;
     lv.count = o_count.p + off_level;
@@ -1827,10 +2184,7 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
       it.scr_aux = scr_aux.p;
       it.scr_y = scr_y.p;
       it.scr_stride = scr_stride;
-      {
-        Timer tm("split_search", 0.0, s);
-        search(algorithm, (int)std::min<int64_t>(n_open * D0, search_grid_cap), s, d, lv, it);
-      }
+      run_phase(it);
       DecideArgs da{};
       da.n_open = n_open;
       da.D0 = D0;
@@ -1854,10 +2208,7 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
           it.n_groups = n_retry;
           it.per_group = R;
           it.j0 = D0;
-          {
-            Timer tm("split_search", 0.0, s);
-            search(algorithm, (int)std::min<int64_t>(n_retry * R, search_grid_cap), s, d, lv, it);
-          }
+          run_phase(it);
           da.retry_list = retry_list.p;
           da.n_retry = n_retry;
           hipLaunchKernelGGL(k_decide2, dim3(grid_for(n_retry)), dim3(kBlock), 0, s, da);
@@ -1948,6 +2299,13 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
     std::swap(cur_row, nxt_row);
     std::swap(cur_aux, nxt_aux);
     std::swap(cur_ey, nxt_ey);
+    if (trace) {
+      DTE_HIP(hipStreamSynchronize(s));
+      fprintf(stderr, "[dte] depth %d nodes %lld open %lld tiers %lld/%lld/%lld : %.3f ms\n", depth,
+              (long long)n_level, (long long)n_open, (long long)trace_tier[0], (long long)trace_tier[1],
+              (long long)trace_tier[2],
+              1000.0 * (std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count() - t_level0));
+    }
     off_level += n_level;
     n_level = n_next;
     // largest node of the next level (for scratch sizing)

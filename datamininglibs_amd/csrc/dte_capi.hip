@@ -516,6 +516,39 @@ int dte_predict_device(const dte_forest* forest, const float* samples_dev, int64
   });
 }
 
+int dte_device_alloc(int32_t device_id, int64_t bytes, void** out) {
+  return guarded([&] {
+    if (!out || bytes < 0) fail(DTE_ERR_INVALID_ARGUMENT, "bad arguments");
+    set_device(device_id);
+    void* p = nullptr;
+    DTE_HIP(hipMalloc(&p, bytes > 0 ? (size_t)bytes : 1));
+    *out = p;
+  });
+}
+
+int dte_device_free(int32_t device_id, void* ptr) {
+  return guarded([&] {
+    set_device(device_id);
+    if (ptr) DTE_HIP(hipFree(ptr));
+  });
+}
+
+int dte_memcpy_to_device(int32_t device_id, void* dst, const void* src, int64_t bytes) {
+  return guarded([&] {
+    if ((!dst || !src) && bytes > 0) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    set_device(device_id);
+    if (bytes > 0) DTE_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+  });
+}
+
+int dte_memcpy_to_host(int32_t device_id, void* dst, const void* src, int64_t bytes) {
+  return guarded([&] {
+    if ((!dst || !src) && bytes > 0) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    set_device(device_id);
+    if (bytes > 0) DTE_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+  });
+}
+
 int dte_profiling(int32_t enable) {
   return guarded([&] {
     std::lock_guard<std::mutex> lk(g_prof_mu);

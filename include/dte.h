@@ -192,6 +192,14 @@ int dte_profile_reset(void);
 int dte_profile_query(int32_t index, char* name, int32_t name_cap, int64_t* launches,
                       double* total_ms, double* algorithmic_bytes);
 
+/* Device memory helpers for callers that stage inputs in HBM themselves
+ * (the reference's lib_gpu::GpuDevice AllocateMemory / CopyToDevice /
+ * CopyToHost, lib_gpu/include/gpu_device.h:9-19). */
+int dte_device_alloc(int32_t device_id, int64_t bytes, void** out);
+int dte_device_free(int32_t device_id, void* ptr);
+int dte_memcpy_to_device(int32_t device_id, void* dst, const void* src, int64_t bytes);
+int dte_memcpy_to_host(int32_t device_id, void* dst, const void* src, int64_t bytes);
+
 int dte_device_count(int32_t* count);
 int dte_last_error(char* buf, int64_t cap);
 

@@ -137,6 +137,8 @@ int bit_length64(uint64_t v) {
   return b;
 }
 
+/* F is fixed per fit from the total weight of a root (N rows, bootstrap or
+ * not), so every node of the fit uses the same T() table. */
 int ig_shift(uint64_t n, int C) {
   double b = (double)(2 * C + 2) * (xlnx(n) + 1.0);
   return 62 - bit_length64((uint64_t)b);
@@ -147,12 +149,11 @@ int64_t xlnx_fixed(uint64_t x, int F) {
   return std::llrint(std::ldexp(xlnx(x), F));
 }
 
-double info_gain(const uint32_t* node, const uint32_t* left, int C) {
+double info_gain(const uint32_t* node, const uint32_t* left, int C, int F) {
   uint64_t n = 0, nl = 0;
   for (int c = 0; c < C; ++c) { n += node[c]; nl += left[c]; }
   uint64_t nr = n - nl;
   if (n == 0) return 0.0;
-  int F = ig_shift(n, C);
   double inv = 1.0 / ((double)n * kLn2);
   int64_t P = xlnx_fixed(n, F);
   for (int c = 0; c < C; ++c) P -= xlnx_fixed(node[c], F);
@@ -231,6 +232,7 @@ struct Dataset {
   std::vector<int32_t> label;
   std::vector<int64_t> yfix;
   int shift;
+  int igF;  /* fixed-point shift of the x ln x terms (classification) */
 };
 
 struct Node {
@@ -276,34 +278,53 @@ Stats node_stats(const Dataset& d, const Node& nd) {
 Candidate eval_rf(const Dataset& d, const Node& nd, const Stats& st, int f) {
   const float* col = d.X + (int64_t)f * d.N;
   size_t m = nd.rows.size();
-  std::vector<uint32_t> ord(m);
-  for (size_t i = 0; i < m; ++i) ord[i] = (uint32_t)i;
-  std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) {
-    return col[nd.rows[a]] < col[nd.rows[b]];
+  std::vector<std::pair<float, uint32_t>> ord(m);
+  for (size_t i = 0; i < m; ++i) ord[i] = {col[nd.rows[i]], (uint32_t)i};
+  std::sort(ord.begin(), ord.end(), [](const std::pair<float, uint32_t>& a, const std::pair<float, uint32_t>& b) {
+    return a.first < b.first;  /* ties never meet at a boundary: their order is irrelevant */
   });
   Candidate best{0.0f, 0.0f, f};
   double best_g = 0.0;
   bool have = false;
   float best_a = 0, best_b = 0;
   if (d.task == 0) {
-    std::vector<uint32_t> left(d.C, 0);
+    /* Running left counts; the exact int64 sums of the x ln x terms are
+     * updated one class at a time, which equals info_gain()'s full
+     * recomputation bit for bit (integer sums are order-free). */
+    const int C = d.C;
+    const uint64_t n = (uint64_t)st.count;
+    const int F = d.igF;
+    const double inv = 1.0 / ((double)n * kLn2);
+    int64_t P = xlnx_fixed(n, F);
+    for (int c = 0; c < C; ++c) P -= xlnx_fixed(st.cls[c], F);
+    std::vector<uint32_t> left(C, 0);
+    int64_t sl = 0, sr = 0;
+    for (int c = 0; c < C; ++c) sr += xlnx_fixed(st.cls[c], F);
+    uint64_t nl = 0;
     for (size_t p = 0; p < m; ++p) {
-      uint32_t e = ord[p];
+      uint32_t e = ord[p].second;
       if (p > 0) {
-        float a = col[nd.rows[ord[p - 1]]], b = col[nd.rows[e]];
+        float a = ord[p - 1].first, b = ord[p].first;
         if (a < b) {
-          double g = info_gain(st.cls.data(), left.data(), d.C);
+          int64_t Q = (sl - xlnx_fixed(nl, F)) + (sr - xlnx_fixed(n - nl, F));
+          double g = std::ldexp((double)(P + Q), -F) * inv;
           if (g > 0.0 && (!have || g > best_g)) { have = true; best_g = g; best_a = a; best_b = b; }
         }
       }
-      left[d.label[nd.rows[e]]] += nd.w[e];
+      int lab = d.label[nd.rows[e]];
+      uint32_t w = nd.w[e];
+      uint32_t lo = left[lab], ln = lo + w, tot = st.cls[lab];
+      sl += xlnx_fixed(ln, F) - xlnx_fixed(lo, F);
+      sr += xlnx_fixed(tot - ln, F) - xlnx_fixed(tot - lo, F);
+      left[lab] = ln;
+      nl += w;
     }
   } else {
     int64_t nl = 0, sl = 0;
     for (size_t p = 0; p < m; ++p) {
-      uint32_t e = ord[p];
+      uint32_t e = ord[p].second;
       if (p > 0) {
-        float a = col[nd.rows[ord[p - 1]]], b = col[nd.rows[e]];
+        float a = ord[p - 1].first, b = ord[p].first;
         if (a < b) {
           double g = variance_gain(nl, sl, st.count - nl, st.sum - sl, d.shift);
           if (g > 0.0 && (!have || g > best_g)) { have = true; best_g = g; best_a = a; best_b = b; }
@@ -340,7 +361,7 @@ Candidate eval_ert(const Dataset& d, const Node& nd, const Stats& st, int f, uin
     for (size_t i = 0; i < m; ++i)
       if (side_of(col[nd.rows[i]], thr) == 0) { left[d.label[nd.rows[i]]] += nd.w[i]; nl += nd.w[i]; }
     if (nl == 0 || nl == st.count) c.score = -FLT_MAX;
-    else c.score = (float)info_gain(st.cls.data(), left.data(), d.C);
+    else c.score = (float)info_gain(st.cls.data(), left.data(), d.C, d.igF);
   } else {
     int64_t nl = 0, sl = 0;
     for (size_t i = 0; i < m; ++i)
@@ -470,9 +491,15 @@ uint32_t oracle_rand_u32(uint64_t seed, uint32_t stream, uint32_t tree, uint32_t
   return rand_u32(seed, stream, tree, node, idx);
 }
 double oracle_xlnx(uint64_t x) { return xlnx(x); }
-double oracle_info_gain(const uint32_t* node_counts, const uint32_t* left_counts, int32_t C) {
-  return info_gain(node_counts, left_counts, C);
+double oracle_info_gain(const uint32_t* node_counts, const uint32_t* left_counts, int32_t C, int32_t F) {
+  if (F <= 0) {
+    uint64_t n = 0;
+    for (int c = 0; c < C; ++c) n += node_counts[c];
+    F = ig_shift(n, C);
+  }
+  return info_gain(node_counts, left_counts, C, F);
 }
+int32_t oracle_ig_shift(uint64_t n, int32_t C) { return ig_shift(n, C); }
 double oracle_variance_gain(int64_t nl, int64_t sl, int64_t nr, int64_t sr, int32_t shift) {
   return variance_gain(nl, sl, nr, sr, shift);
 }
@@ -491,6 +518,11 @@ float oracle_rf_best_split(const float* values, const int32_t* labels, const uin
   d.C = C;
   d.label.assign(labels, labels + m);
   d.shift = 0;
+  {
+    uint64_t tot = 0;
+    for (int64_t i = 0; i < m; ++i) tot += weights[i];
+    d.igF = ig_shift(tot, C);
+  }
   Node nd;
   nd.depth = 0;
   nd.id = 0;
@@ -519,6 +551,7 @@ int oracle_fit(const float* X, const float* y, int64_t N, int32_t F, const oracl
   d.task = p.task;
   d.C = p.task == 0 ? p.n_classes : 1;
   d.shift = 0;
+  d.igF = ig_shift((uint64_t)N, d.C);
   if (p.task == 0) {
     d.label.resize(N);
     for (int64_t i = 0; i < N; ++i) {

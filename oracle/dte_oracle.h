@@ -71,7 +71,10 @@ int oracle_predict(const oracle_forest* f, const float* X, int64_t n_rows,
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 uint32_t oracle_rand_u32(uint64_t seed, uint32_t stream, uint32_t tree, uint32_t node, uint64_t idx);
 double oracle_xlnx(uint64_t x);
-double oracle_info_gain(const uint32_t* node_counts, const uint32_t* left_counts, int32_t n_classes);
+/* F <= 0: use the shift of the node's own total (as a fit of that size would). */
+double oracle_info_gain(const uint32_t* node_counts, const uint32_t* left_counts, int32_t n_classes,
+                        int32_t F);
+int32_t oracle_ig_shift(uint64_t total_weight, int32_t n_classes);
 double oracle_variance_gain(int64_t n_left, int64_t s_left, int64_t n_right, int64_t s_right,
                             int32_t fixed_shift);
 int32_t oracle_fixed_shift(double max_abs_target, int64_t total_weight);

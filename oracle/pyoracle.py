@@ -55,7 +55,9 @@ def lib():
         L.oracle_rand_u32.restype = ctypes.c_uint32
         L.oracle_xlnx.argtypes = [ctypes.c_uint64]
         L.oracle_xlnx.restype = ctypes.c_double
-        L.oracle_info_gain.argtypes = [P, P, ctypes.c_int32]
+        L.oracle_info_gain.argtypes = [P, P, ctypes.c_int32, ctypes.c_int32]
+        L.oracle_ig_shift.argtypes = [ctypes.c_uint64, ctypes.c_int32]
+        L.oracle_ig_shift.restype = ctypes.c_int32
         L.oracle_info_gain.restype = ctypes.c_double
         L.oracle_variance_gain.argtypes = [ctypes.c_int64] * 4 + [ctypes.c_int32]
         L.oracle_variance_gain.restype = ctypes.c_double
@@ -154,10 +156,14 @@ def xlnx(x):
     return lib().oracle_xlnx(int(x))
 
 
-def info_gain(node, left):
+def info_gain(node, left, F=0):
     node = np.asarray(node, np.uint32)
     left = np.asarray(left, np.uint32)
-    return lib().oracle_info_gain(_ptr(node), _ptr(left), node.size)
+    return lib().oracle_info_gain(_ptr(node), _ptr(left), node.size, F)
+
+
+def ig_shift(total_weight, n_classes):
+    return lib().oracle_ig_shift(int(total_weight), n_classes)
 
 
 def variance_gain(nl, sl, nr, sr, shift):
