@@ -712,6 +712,422 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
   if (t == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
 }
 
+// ------------------------------------- chunk-parallel big-node search ---
+// Nodes above the wave-team capacity are split into key chunks that are
+// evaluated independently, so every chunk of every big item of a level runs
+// in parallel:
+//   1. k_big_hist    per (item, tile): bucket histogram of the dense keys
+//   2. k_big_plan    per item: bucket offsets, greedy packing of consecutive
+//                    buckets into chunks of <= 512 entries (an overfull
+//                    bucket -- heavy ties -- becomes a histogram chunk)
+//   3. k_big_scatter per (item, tile): entries into bucket order
+//   4. k_big_ctot    per chunk: class totals / target sums, last key
+//   5. k_big_carry   per item: running totals before each chunk
+//   6. k_big_eval    per chunk: LDS sort + scored walk from the carry
+//   7. k_big_reduce  per item: best chunk -> (score, threshold)
+struct BigItem {
+  int32_t f;
+  int32_t m;
+  int64_t start;  // entry start of the node
+  int32_t wbits;  // bucket width 2^wbits keys
+  int32_t nb;     // buckets
+  int64_t boff;   // offset in the bucket arrays
+  int64_t soff;   // offset in the scratch element arrays
+  int64_t ridx;   // result slot
+  uint32_t node;  // level node index
+  uint32_t cbeg;  // first chunk (set by k_big_plan)
+  uint32_t ncnk;  // chunks (set by k_big_plan)
+  uint32_t pad;
+};
+
+struct BigTile {
+  uint32_t item;
+  uint32_t e0;
+};
+
+struct BigChunk {
+  uint32_t item;
+  uint32_t e0;     // first element (relative to the item's scratch base)
+  uint32_t n;      // elements
+  uint32_t kbase;  // first key of the chunk's first bucket
+  uint32_t bits;   // key span bits
+  uint32_t hist;   // 1: single overfull bucket, evaluated by histogram
+};
+
+struct BigArgs {
+  BigItem* items;
+  int64_t n_items;
+  const BigTile* tiles;
+  uint32_t* bcount;  // per bucket: count, then offset, then cursor
+  uint32_t* bcur;
+  BigChunk* chunks;
+  unsigned int* chunk_ctr;
+  uint32_t* skey;
+  uint32_t* saux;
+  int64_t* sy;
+  // per chunk
+  uint32_t* ctot_cls;  // [chunk * C]
+  int64_t* ctot_n;
+  int64_t* ctot_s;
+  uint32_t* clast;
+  uint32_t* carry_cls;
+  int64_t* carry_n;
+  int64_t* carry_s;
+  uint32_t* carry_last;
+  double* cb_g;
+  uint32_t* cb_key;
+  uint32_t* cb_prev;
+};
+
+constexpr int kBigTile = 4096;
+constexpr int kBigCap = 64 * kIpt;  // chunk capacity of a wave team (512)
+
+template <int TASK>
+__global__ void __launch_bounds__(kBlock) k_big_hist(DataArgs d, LevelArgs lv, BigArgs ba) {
+  __shared__ uint32_t s_h[kMaxBuckets];
+  const BigTile tl = ba.tiles[blockIdx.x];
+  const BigItem it = ba.items[tl.item];
+  const int t = threadIdx.x;
+  for (int b = t; b < it.nb; b += kBlock) s_h[b] = 0;
+  __syncthreads();
+  const uint32_t* Kf = d.K + (int64_t)it.f * d.N;
+  constexpr int U = kBigTile / kBlock;
+  uint32_t rows[U];
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int e = (int)tl.e0 + t + q * kBlock;
+    rows[q] = e < it.m ? lv.row[it.start + e] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int e = (int)tl.e0 + t + q * kBlock;
+    if (e < it.m) atomicAdd(&s_h[Kf[rows[q]] >> it.wbits], 1u);
+  }
+  __syncthreads();
+  for (int b = t; b < it.nb; b += kBlock)
+    if (s_h[b]) atomicAdd(&ba.bcount[it.boff + b], s_h[b]);
+}
+
+// One workgroup per item: exclusive scan of the bucket counts, cursor copy,
+// then thread 0 packs buckets into chunks (contiguous in the chunk list).
+__global__ void __launch_bounds__(kBlock) k_big_plan(BigArgs ba) {
+  __shared__ uint32_t s_c[kMaxBuckets + 1];
+  __shared__ uint32_t s_w[kWaves];
+  BigItem& it = ba.items[blockIdx.x];
+  const int t = threadIdx.x;
+  const int nb = it.nb;
+  for (int b = t; b < nb; b += kBlock) s_c[b] = ba.bcount[it.boff + b];
+  __syncthreads();
+  const int per = (nb + kBlock - 1) / kBlock;
+  const int b0 = t * per, b1 = min(b0 + per, nb);
+  uint32_t loc = 0;
+  for (int b = b0; b < b1; ++b) loc += s_c[b];
+  uint32_t base = team_excl_scan<256>(loc, s_w, nullptr);
+  __syncthreads();
+  for (int b = b0; b < b1; ++b) {
+    const uint32_t c = s_c[b];
+    s_c[b] = base;
+    base += c;
+  }
+  if (t == 0) s_c[nb] = (uint32_t)it.m;
+  __syncthreads();
+  for (int b = t; b < nb; b += kBlock) ba.bcur[it.boff + b] = s_c[b];
+  if (t == 0) {
+    // count chunks, reserve, write
+    uint32_t n = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      uint32_t cbeg = 0;
+      if (pass == 1) cbeg = atomicAdd(ba.chunk_ctr, n);
+      uint32_t k = 0;
+      int b = 0;
+      while (b < nb) {
+        const uint32_t cb = s_c[b + 1] - s_c[b];
+        if (cb == 0) {
+          ++b;
+          continue;
+        }
+        int be = b;
+        uint32_t tot = 0;
+        uint32_t hist = 0;
+        if (cb > (uint32_t)kBigCap) {
+          be = b + 1;
+          tot = cb;
+          hist = 1;
+        } else {
+          while (be < nb) {
+            const uint32_t c = s_c[be + 1] - s_c[be];
+            if (c > (uint32_t)kBigCap || tot + c > (uint32_t)kBigCap) break;
+            tot += c;
+            ++be;
+          }
+        }
+        if (pass == 1) {
+          BigChunk ch;
+          ch.item = blockIdx.x;
+          ch.e0 = s_c[b];
+          ch.n = tot;
+          ch.kbase = (uint32_t)b << it.wbits;
+          const uint32_t span = (uint32_t)(be - b) << it.wbits;
+          ch.bits = (uint32_t)bit_length(span - 1u);
+          ch.hist = hist;
+          ba.chunks[cbeg + k] = ch;
+        }
+        ++k;
+        b = be;
+      }
+      if (pass == 0) n = k;
+      else {
+        it.cbeg = cbeg;
+        it.ncnk = k;
+      }
+    }
+  }
+}
+
+template <int TASK>
+__global__ void __launch_bounds__(kBlock) k_big_scatter(DataArgs d, LevelArgs lv, BigArgs ba) {
+  __shared__ uint32_t s_h[kMaxBuckets];
+  __shared__ uint32_t s_base[kMaxBuckets];
+  const BigTile tl = ba.tiles[blockIdx.x];
+  const BigItem it = ba.items[tl.item];
+  const int t = threadIdx.x;
+  for (int b = t; b < it.nb; b += kBlock) s_h[b] = 0;
+  __syncthreads();
+  const uint32_t* Kf = d.K + (int64_t)it.f * d.N;
+  constexpr int U = kBigTile / kBlock;
+  uint32_t keys[U], auxv[U];
+  int64_t yv[U];
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int e = (int)tl.e0 + t + q * kBlock;
+    keys[q] = e < it.m ? lv.row[it.start + e] : 0u;
+    if (e < it.m) {
+      auxv[q] = lv.aux[it.start + e];
+      if (TASK == 1) yv[q] = lv.ey[it.start + e];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int e = (int)tl.e0 + t + q * kBlock;
+    if (e < it.m) {
+      keys[q] = Kf[keys[q]];
+      atomicAdd(&s_h[keys[q] >> it.wbits], 1u);
+    }
+  }
+  __syncthreads();
+  for (int b = t; b < it.nb; b += kBlock) {
+    const uint32_t c = s_h[b];
+    if (c) s_base[b] = atomicAdd(&ba.bcur[it.boff + b], c);
+    s_h[b] = 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int e = (int)tl.e0 + t + q * kBlock;
+    if (e < it.m) {
+      const uint32_t b = keys[q] >> it.wbits;
+      const int64_t pos = it.soff + s_base[b] + atomicAdd(&s_h[b], 1u);
+      ba.skey[pos] = keys[q];
+      ba.saux[pos] = auxv[q];
+      if (TASK == 1) ba.sy[pos] = yv[q];
+    }
+  }
+}
+
+// One wave per chunk: class totals (or count / target sum) and last key.
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_big_ctot(DataArgs d, BigArgs ba, int64_t n_chunks) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t c = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave; c < n_chunks;
+       c += (int64_t)gridDim.x * kWaves) {
+    const BigChunk ch = ba.chunks[c];
+    const BigItem& it = ba.items[ch.item];
+    const int64_t e0 = it.soff + ch.e0;
+    uint32_t loc[MAXC];
+#pragma unroll
+    for (int q = 0; q < MAXC; ++q) loc[q] = 0;
+    int64_t ln = 0, ls = 0;
+    uint32_t kmax = 0;
+    for (uint32_t e = lane; e < ch.n; e += kWave) {
+      const uint32_t a = ba.saux[e0 + e];
+      const uint32_t w = a & 0xFFFFu;
+      kmax = max(kmax, ba.skey[e0 + e]);
+      ln += w;
+      if (TASK == 0) {
+        const int lab = (int)(a >> 16);
+#pragma unroll
+        for (int q = 0; q < MAXC; ++q) loc[q] += (q == lab) ? w : 0u;
+      } else {
+        ls += (int64_t)w * ba.sy[e0 + e];
+      }
+    }
+    kmax = wave_max_u32(kmax);
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      ln += __shfl_xor(ln, off, kWave);
+      ls += __shfl_xor(ls, off, kWave);
+    }
+    if (TASK == 0) {
+#pragma unroll
+      for (int q = 0; q < MAXC; ++q) {
+        uint32_t v = loc[q];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+        if (lane == q && q < d.C) ba.ctot_cls[c * d.C + q] = v;
+      }
+    }
+    if (lane == 0) {
+      ba.ctot_n[c] = ln;
+      if (TASK == 1) ba.ctot_s[c] = ls;
+      ba.clast[c] = kmax;
+    }
+  }
+}
+
+// One thread per item: totals of all earlier chunks of the item.
+__global__ void k_big_carry(DataArgs d, BigArgs ba) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ba.n_items;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const BigItem it = ba.items[k];
+    uint32_t run[32];
+    for (int c = 0; c < d.C; ++c) run[c] = 0;
+    int64_t rn = 0, rs = 0;
+    uint32_t last = 0;
+    for (uint32_t q = 0; q < it.ncnk; ++q) {
+      const int64_t c = it.cbeg + q;
+      for (int j = 0; j < d.C; ++j) {
+        ba.carry_cls[c * d.C + j] = run[j];
+        run[j] += ba.ctot_cls[c * d.C + j];
+      }
+      ba.carry_n[c] = rn;
+      if (ba.carry_s) ba.carry_s[c] = rs;
+      ba.carry_last[c] = last;
+      rn += ba.ctot_n[c];
+      rs += ba.ctot_s ? ba.ctot_s[c] : 0;
+      last = ba.clast[c];
+    }
+  }
+}
+
+// Chunk evaluation.  TEAM = 64: sorted chunks (<= 512 entries), four per
+// workgroup.  TEAM = 256: histogram chunks (one overfull bucket).
+template <int MAXC, int TASK, int TEAM>
+__global__ void __launch_bounds__(kBlock) k_big_eval(DataArgs d, LevelArgs lv, BigArgs ba, const uint32_t* list,
+                                                    int64_t n_list) {
+  constexpr int IPT = kIpt;
+  constexpr int NT = kBlock / TEAM;
+  __shared__ double s_inv[128], s_logc[128];
+  __shared__ int64_t s_T[kTT];
+  __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[NT];
+  load_tables(s_inv, s_logc, TASK == 0 ? s_T : nullptr, d.T);
+  __syncthreads();
+  const LnCtx lx{s_inv, s_logc, s_T, d.igF};
+  const int team = TEAM == 256 ? 0 : (int)(threadIdx.x / kWave);
+  TeamLds<MAXC, TASK, TEAM, IPT>& L = s_team[team];
+  const int t = team_tid<TEAM>();
+  for (int64_t q = (int64_t)blockIdx.x * NT + team; q < n_list; q += (int64_t)gridDim.x * NT) {
+    const uint32_t c = list[q];
+    const BigChunk ch = ba.chunks[c];
+    const BigItem it = ba.items[ch.item];
+    NodeCtx nc;
+    (void)node_context<MAXC, TASK, TEAM, IPT>(d, lv, it.node, nc, L, lx);
+    if (t == 0) {
+      for (int j = 0; j < MAXC; ++j) L.cr.run[j] = (TASK == 0 && j < d.C) ? ba.carry_cls[(int64_t)c * d.C + j] : 0u;
+      L.cr.run_n = ba.carry_n[c];
+      L.cr.run_s = ba.carry_s ? ba.carry_s[c] : 0;
+      L.cr.last_key = ba.carry_last[c];
+      L.cr.best_g = 0.0;
+      L.cr.best_key = kNone;
+      L.cr.best_prev = kNone;
+    }
+    const int64_t e0 = it.soff + ch.e0;
+    if constexpr (TEAM == 256) {
+      team_sync<TEAM>();
+      eval_hist_bucket<MAXC, TASK, IPT>(ba.skey + e0, ba.saux + e0, TASK == 1 ? ba.sy + e0 : nullptr, ch.n,
+                                        ch.kbase, 1 << it.wbits, nc, L, lx);
+    } else {
+#pragma unroll
+      for (int k = 0; k < IPT; ++k) {
+        const int e = t + k * TEAM;
+        if (e < (int)ch.n) {
+          L.u.srt.k[0][e] = ba.skey[e0 + e] - ch.kbase;
+          L.u.srt.v[0][e] = (uint16_t)e;
+          L.aux[e] = ba.saux[e0 + e];
+          if (TASK == 1) L.y[e] = ba.sy[e0 + e];
+        }
+      }
+      team_sync<TEAM>();
+      sort_and_eval<MAXC, TASK, TEAM, IPT>((int)ch.n, (int)ch.bits, ch.kbase, nc, L, lx);
+    }
+    team_sync<TEAM>();
+    if (t == 0) {
+      ba.cb_g[c] = L.cr.best_g;
+      ba.cb_key[c] = L.cr.best_key;
+      ba.cb_prev[c] = L.cr.best_prev;
+    }
+    team_sync<TEAM>();
+  }
+}
+
+template <int TASK>
+__global__ void k_big_reduce(DataArgs d, BigArgs ba, ItemArgs it, int f_unused) {
+  (void)f_unused;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ba.n_items;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const BigItem bi = ba.items[k];
+    double g = 0.0;
+    uint32_t key = kNone, prev = kNone;
+    for (uint32_t q = 0; q < bi.ncnk; ++q) {
+      const int64_t c = bi.cbeg + q;
+      if (ba.cb_key[c] != kNone && cand_better(ba.cb_g[c], ba.cb_key[c], g, key)) {
+        g = ba.cb_g[c];
+        key = ba.cb_key[c];
+        prev = ba.cb_prev[c];
+      }
+    }
+    float score = 0.0f, thr = 0.0f;
+    if (key != kNone) {
+      const float* sv = d.SV + d.SVoff[bi.f];
+      score = (float)(TASK == 0 ? g : ldexp_d(g, -2 * d.shift));
+      thr = midpoint(sv[prev], sv[key]);
+    }
+    it.res_score[bi.ridx] = score;
+    it.res_thr[bi.ridx] = thr;
+    it.res_feat[bi.ridx] = bi.f;
+  }
+}
+
+// Fill the item table of a batch of big groups on the device.
+__global__ void k_big_items(DataArgs d, LevelArgs lv, const uint32_t* groups, int64_t g0, int64_t n_groups,
+                            int per_group, int j0, int D, BigItem* items) {
+  const int64_t n = n_groups * per_group;
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t o = groups[g0 + k / per_group];
+    const int j = j0 + (int)(k % per_group);
+    const uint32_t i = lv.open_idx[o];
+    BigItem b;
+    b.f = (int)(rand_u32(d.seed, kStreamFeat, (uint32_t)lv.tree_global[lv.tree[i]], lv.id[i], (uint64_t)j) %
+                (uint32_t)d.F);
+    b.m = lv.m[i];
+    b.start = lv.start[i];
+    const uint32_t S = d.S[b.f];
+    const int words = d.C > 0 ? d.C : 3;
+    int wbits = 0;
+    while ((1 << (wbits + 1)) * words <= kHistWords) ++wbits;
+    while (((S + (1u << wbits) - 1) >> wbits) > (uint32_t)kMaxBuckets) ++wbits;
+    b.wbits = wbits;
+    b.nb = (int)((S + (1u << wbits) - 1) >> wbits);
+    b.boff = 0;
+    b.soff = 0;
+    b.ridx = (int64_t)o * D + j;
+    b.node = i;
+    b.cbeg = 0;
+    b.ncnk = 0;
+    b.pad = 0;
+    items[k] = b;
+  }
+}
+
 // ------------------------------------------------ ERT split search ------
 // One block per item: threshold = fp32 mean of 10 node values drawn with
 // replacement (gpuert.cu:167-183, summed in draw order), then a 2 x C
@@ -1065,7 +1481,8 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArg
 // item writes its own result slot.
 // Wave-aggregated: one global atomic per wave and tier.
 __global__ void k_tier_split(const uint32_t* list, int64_t n, const uint32_t* open_idx, const int32_t* node_m,
-                             int cut0, int cut1, uint32_t* t0, uint32_t* t1, uint32_t* t2, unsigned int* counts) {
+                             int cut0, int cut1, int cut2, uint32_t* t0, uint32_t* t1, uint32_t* t2, uint32_t* t3,
+                             unsigned int* counts) {
   const int lane = threadIdx.x & (kWave - 1);
   for (int64_t base = (blockIdx.x * (int64_t)blockDim.x) & ~(int64_t)(kWave - 1); base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -1075,10 +1492,10 @@ __global__ void k_tier_split(const uint32_t* list, int64_t n, const uint32_t* op
     if (g < n) {
       o = list ? list[g] : (uint32_t)g;
       const int m = node_m[open_idx[o]];
-      tier = m <= cut0 ? 0 : (m <= cut1 ? 1 : 2);
+      tier = m <= cut0 ? 0 : (m <= cut1 ? 1 : (m < cut2 ? 2 : 3));
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
       const uint64_t mask = __ballot(tier == k);
       if (!mask) continue;
       const int leader = __ffsll((unsigned long long)mask) - 1;
@@ -1087,7 +1504,7 @@ __global__ void k_tier_split(const uint32_t* list, int64_t n, const uint32_t* op
       basepos = __shfl(basepos, leader, kWave);
       if (tier == k) {
         const unsigned int r = (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
-        uint32_t* dst = k == 0 ? t0 : (k == 1 ? t1 : t2);
+        uint32_t* dst = k == 0 ? t0 : (k == 1 ? t1 : (k == 2 ? t2 : t3));
         dst[basepos + r] = o;
       }
     }
@@ -1208,6 +1625,7 @@ struct PartArgs {
   uint64_t* l_hash;
   uint8_t* split_ok;
   unsigned long long* bytes;
+  const uint32_t* list;  // optional: which open positions to process
 };
 
 template <int MAXC, int TASK>
@@ -1218,7 +1636,8 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
   __shared__ uint32_t s_ml;
   const int t = threadIdx.x;
   unsigned long long my_bytes = 0;
-  for (int64_t o = blockIdx.x; o < pa.n_open; o += gridDim.x) {
+  for (int64_t q = blockIdx.x; q < pa.n_open; q += gridDim.x) {
+    const int64_t o = pa.list ? pa.list[q] : q;
     const uint32_t i = lv.open_idx[o];
     const int m = lv.m[i];
     const int64_t st = lv.start[i];
@@ -1315,6 +1734,101 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
     __syncthreads();
   }
   if (t == 0 && my_bytes && pa.bytes) atomicAdd(pa.bytes, my_bytes);
+}
+
+// Small nodes: one wave per node; stable compaction with ballots, tile by
+// tile (64 entries), statistics reduced across the wave.
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_partition_wave(DataArgs d, LevelArgs lv, PartArgs pa,
+                                                           const uint32_t* list, int64_t n_list) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const uint64_t lt_mask = (1ull << lane) - 1ull;
+  unsigned long long my_bytes = 0;
+  for (int64_t q = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave; q < n_list;
+       q += (int64_t)gridDim.x * kWaves) {
+    const uint32_t o = list[q];
+    const uint32_t i = lv.open_idx[o];
+    const int m = lv.m[i];
+    const int64_t st = lv.start[i];
+    const float thr = pa.ch_thr[o];
+    const float* Xf = d.X + (int64_t)pa.ch_feat[o] * d.N;
+    uint32_t loc[MAXC];
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) loc[c] = 0;
+    uint64_t cnt = 0, hsh = 0;
+    int64_t sm = 0;
+    uint32_t ml = 0;
+    for (int e0 = 0; e0 < m; e0 += kWave) {
+      const int e = e0 + lane;
+      bool isl = false;
+      if (e < m) {
+        const uint32_t row = lv.row[st + e];
+        isl = side_of(Xf[row], thr) == 0;
+        if (isl) {
+          const uint32_t a = lv.aux[st + e];
+          const uint32_t w = a & 0xFFFFu;
+          cnt += w;
+          hsh += (uint64_t)w * mix64(row);
+          if (TASK == 0) {
+            const int lab = (int)(a >> 16);
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) loc[c] += (c == lab) ? w : 0u;
+          } else {
+            sm += (int64_t)w * lv.ey[st + e];
+          }
+        }
+      }
+      ml += (uint32_t)__popcll(__ballot(isl));
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      cnt += __shfl_xor(cnt, off, kWave);
+      hsh += __shfl_xor(hsh, off, kWave);
+      sm += __shfl_xor(sm, off, kWave);
+    }
+    const int mL = (int)ml;
+    const bool ok = mL > 0 && mL < m;
+    if (TASK == 0) {
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c) {
+        uint32_t v = loc[c];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+        if (lane == c && c < d.C) pa.l_cls[(int64_t)o * d.C + c] = v;
+      }
+    }
+    if (lane == 0) {
+      my_bytes += (unsigned long long)m * (TASK == 0 ? 12ull : 20ull);
+      pa.split_ok[o] = ok ? 1 : 0;
+      pa.l_m[o] = mL;
+      pa.l_count[o] = (int32_t)cnt;
+      pa.l_hash[o] = hsh;
+      if (TASK == 1) pa.l_sum[o] = sm;
+    }
+    if (ok) {
+      uint32_t lbase = 0, rbase = (uint32_t)mL;
+      for (int e0 = 0; e0 < m; e0 += kWave) {
+        const int e = e0 + lane;
+        uint32_t row = 0;
+        bool isl = false;
+        if (e < m) {
+          row = lv.row[st + e];
+          isl = side_of(Xf[row], thr) == 0;
+        }
+        const uint64_t bl = __ballot(isl);
+        const uint64_t br = __ballot(e < m && !isl);
+        if (e < m) {
+          const uint32_t pos = isl ? lbase + (uint32_t)__popcll(bl & lt_mask) : rbase + (uint32_t)__popcll(br & lt_mask);
+          pa.nrow[st + pos] = row;
+          pa.naux[st + pos] = lv.aux[st + e];
+          if (TASK == 1) pa.ney[st + pos] = lv.ey[st + e];
+        }
+        lbase += (uint32_t)__popcll(bl);
+        rbase += (uint32_t)__popcll(br);
+      }
+    }
+  }
+  if (lane == 0 && my_bytes && pa.bytes) atomicAdd(pa.bytes, my_bytes);
 }
 
 // ------------------------------------------------------ children --------
@@ -1495,9 +2009,12 @@ __global__ void k_boot_compact(const uint32_t* mult, const uint32_t* pos, int64_
                                const int64_t* yfix, int C, uint32_t* row, uint32_t* aux, int64_t* ey,
                                uint32_t* root_cls, int64_t* root_sum, uint64_t* root_hash,
                                uint32_t* overflow) {
+  __shared__ uint32_t s_cls[32];
   const int tr = blockIdx.y;
   const uint32_t* mt = mult + (int64_t)tr * N;
   const uint32_t* pt = pos + (int64_t)tr * N;
+  if (threadIdx.x < 32) s_cls[threadIdx.x] = 0;
+  __syncthreads();
   uint64_t hsh = 0;
   int64_t sm = 0;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
@@ -1512,10 +2029,13 @@ __global__ void k_boot_compact(const uint32_t* mult, const uint32_t* pos, int64_
       ey[p] = yfix[r];
       sm += (int64_t)w * yfix[r];
     } else {
-      atomicAdd(&root_cls[(int64_t)tr * C + lab], w);
+      atomicAdd(&s_cls[lab], w);
     }
     hsh += (uint64_t)w * mix64((uint64_t)r);
   }
+  __syncthreads();
+  if (!yfix && (int)threadIdx.x < C && s_cls[threadIdx.x])
+    atomicAdd(&root_cls[(int64_t)tr * C + threadIdx.x], s_cls[threadIdx.x]);
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     hsh += __shfl_xor(hsh, off, kWave);
@@ -1661,6 +2181,13 @@ __global__ void k_compact_list(const uint32_t* flag, const uint32_t* pos, int64_
     if (flag[i]) list[pos[i]] = (uint32_t)i;
 }
 
+__global__ void k_add_u64(unsigned long long* p, unsigned long long v) { *p += v; }
+
+__global__ void k_strided_gather(const uint32_t* in, int64_t stride, int n, uint32_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[(int64_t)i * stride];
+}
+
 __global__ void k_max_m(const int32_t* m, int64_t n, uint32_t* out) {
   uint32_t v = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -1756,8 +2283,172 @@ void launch_partition(int grid, hipStream_t s, const DataArgs& d, const LevelArg
   hipLaunchKernelGGL((k_partition<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, pa);
 }
 
+template <int MAXC, int TASK>
+void launch_partition_wave(int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv, const PartArgs& pa,
+                           const uint32_t* list, int64_t n) {
+  hipLaunchKernelGGL((k_partition_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, pa, list, n);
+}
+
+__global__ void k_chunk_split(const BigChunk* chunks, int64_t n, uint32_t* sorted_list, uint32_t* hist_list,
+                              unsigned int* counts) {
+  const int lane = threadIdx.x & (kWave - 1);
+  for (int64_t base = (blockIdx.x * (int64_t)blockDim.x) & ~(int64_t)(kWave - 1); base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = base + (threadIdx.x & ~(kWave - 1)) + lane;
+    int k = -1;
+    if (c < n) k = chunks[c].hist ? 1 : 0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const uint64_t mask = __ballot(k == q);
+      if (!mask) continue;
+      const int leader = __ffsll((unsigned long long)mask) - 1;
+      unsigned int bp = 0;
+      if (lane == leader) bp = atomicAdd(&counts[q], (unsigned int)__popcll(mask));
+      bp = __shfl(bp, leader, kWave);
+      if (k == q) (q == 0 ? sorted_list : hist_list)[bp + __popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)c;
+    }
+  }
+}
+
+// Scratch and bookkeeping of the big-node pipeline, reused across levels.
+struct BigState {
+  DevBuf<BigItem> items;
+  DevBuf<BigTile> tiles;
+  DevBuf<uint32_t> bcount, bcur;
+  DevBuf<BigChunk> chunks;
+  DevBuf<unsigned int> ctr;
+  DevBuf<uint32_t> skey, saux;
+  DevBuf<int64_t> sy;
+  DevBuf<uint32_t> ctot_cls, clast, carry_cls, carry_last, cb_key, cb_prev, lists;
+  DevBuf<int64_t> ctot_n, ctot_s, carry_n, carry_s;
+  DevBuf<double> cb_g;
+  std::vector<BigItem> host_items;
+};
+
+template <int MAXC, int TASK>
+void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const ItemArgs& it, const uint32_t* groups,
+                  int64_t n_groups, hipStream_t s, int num_cu) {
+  const int64_t n_all = n_groups * it.per_group;
+  if (n_all == 0) return;
+  B.items.reserve(n_all);
+  hipLaunchKernelGGL(k_big_items, dim3(grid_for(n_all)), dim3(kBlock), 0, s, d, lv, groups, (int64_t)0, n_groups,
+                     it.per_group, it.j0, it.D, B.items.p);
+  B.host_items.resize(n_all);
+  DTE_HIP(hipMemcpyAsync(B.host_items.data(), B.items.p, sizeof(BigItem) * n_all, hipMemcpyDeviceToHost, s));
+  DTE_HIP(hipStreamSynchronize(s));
+  const int64_t budget = (int64_t)256 << 20;  // scratch elements per batch
+  const int C = d.C;
+  int64_t k0 = 0;
+  std::vector<BigTile> tiles;
+  while (k0 < n_all) {
+    int64_t k1 = k0, el = 0, nb = 0;
+    tiles.clear();
+    while (k1 < n_all && (k1 == k0 || el + B.host_items[k1].m <= budget)) {
+      BigItem& bi = B.host_items[k1];
+      bi.boff = nb;
+      bi.soff = el;
+      for (int e0 = 0; e0 < bi.m; e0 += kBigTile) tiles.push_back(BigTile{(uint32_t)(k1 - k0), (uint32_t)e0});
+      el += bi.m;
+      nb += bi.nb;
+      ++k1;
+    }
+    const int64_t n_items = k1 - k0;
+    // Greedy packing: two consecutive sorted chunks hold more than kBigCap
+    // entries, a histogram chunk holds more than kBigCap on its own.
+    const int64_t max_chunks = std::min<int64_t>(nb + n_items, 3 * (el / kBigCap) + 2 * n_items + 2);
+    B.items.reserve(n_all);
+    DTE_HIP(hipMemcpyAsync(B.items.p, B.host_items.data() + k0, sizeof(BigItem) * n_items, hipMemcpyHostToDevice, s));
+    B.tiles.reserve(tiles.size());
+    DTE_HIP(hipMemcpyAsync(B.tiles.p, tiles.data(), sizeof(BigTile) * tiles.size(), hipMemcpyHostToDevice, s));
+    B.bcount.reserve(nb);
+    B.bcur.reserve(nb);
+    B.chunks.reserve(max_chunks);
+    B.ctr.reserve(3);
+    B.skey.reserve(el);
+    B.saux.reserve(el);
+    if (TASK == 1) B.sy.reserve(el);
+    B.ctot_cls.reserve(TASK == 0 ? max_chunks * C : 1);
+    B.carry_cls.reserve(TASK == 0 ? max_chunks * C : 1);
+    B.ctot_n.reserve(max_chunks);
+    B.carry_n.reserve(max_chunks);
+    B.ctot_s.reserve(max_chunks);
+    B.carry_s.reserve(max_chunks);
+    B.clast.reserve(max_chunks);
+    B.carry_last.reserve(max_chunks);
+    B.cb_g.reserve(max_chunks);
+    B.cb_key.reserve(max_chunks);
+    B.cb_prev.reserve(max_chunks);
+    B.lists.reserve(2 * max_chunks);
+    DTE_HIP(hipMemsetAsync(B.bcount.p, 0, 4 * nb, s));
+    DTE_HIP(hipMemsetAsync(B.ctr.p, 0, 12, s));
+    BigArgs ba{};
+    ba.items = B.items.p;
+    ba.n_items = n_items;
+    ba.tiles = B.tiles.p;
+    ba.bcount = B.bcount.p;
+    ba.bcur = B.bcur.p;
+    ba.chunks = B.chunks.p;
+    ba.chunk_ctr = B.ctr.p;
+    ba.skey = B.skey.p;
+    ba.saux = B.saux.p;
+    ba.sy = TASK == 1 ? B.sy.p : nullptr;
+    ba.ctot_cls = B.ctot_cls.p;
+    ba.ctot_n = B.ctot_n.p;
+    ba.ctot_s = TASK == 1 ? B.ctot_s.p : nullptr;
+    ba.clast = B.clast.p;
+    ba.carry_cls = B.carry_cls.p;
+    ba.carry_n = B.carry_n.p;
+    ba.carry_s = TASK == 1 ? B.carry_s.p : nullptr;
+    ba.carry_last = B.carry_last.p;
+    ba.cb_g = B.cb_g.p;
+    ba.cb_key = B.cb_key.p;
+    ba.cb_prev = B.cb_prev.p;
+    const unsigned n_tiles = (unsigned)tiles.size();
+    hipLaunchKernelGGL(k_big_hist<TASK>, dim3(n_tiles), dim3(kBlock), 0, s, d, lv, ba);
+    hipLaunchKernelGGL(k_big_plan, dim3((unsigned)n_items), dim3(kBlock), 0, s, ba);
+    hipLaunchKernelGGL(k_big_scatter<TASK>, dim3(n_tiles), dim3(kBlock), 0, s, d, lv, ba);
+    unsigned int n_chunks = 0;
+    DTE_HIP(hipMemcpyAsync(&n_chunks, B.ctr.p, 4, hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipStreamSynchronize(s));
+    hipLaunchKernelGGL((k_big_ctot<MAXC, TASK>), dim3(grid_for(n_chunks, kWaves, num_cu * 16)), dim3(kBlock), 0, s, d,
+                       ba, (int64_t)n_chunks);
+    hipLaunchKernelGGL(k_big_carry, dim3(grid_for(n_items)), dim3(kBlock), 0, s, d, ba);
+    hipLaunchKernelGGL(k_chunk_split, dim3(grid_for(n_chunks)), dim3(kBlock), 0, s, B.chunks.p, (int64_t)n_chunks,
+                       B.lists.p, B.lists.p + max_chunks, B.ctr.p + 1);
+    unsigned int nl[2];
+    DTE_HIP(hipMemcpyAsync(nl, B.ctr.p + 1, 8, hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipStreamSynchronize(s));
+    if (nl[0])
+      hipLaunchKernelGGL((k_big_eval<MAXC, TASK, 64>), dim3(grid_for(nl[0], kWaves, num_cu * 16)), dim3(kBlock), 0,
+                         s, d, lv, ba, (const uint32_t*)B.lists.p, (int64_t)nl[0]);
+    if (nl[1])
+      hipLaunchKernelGGL((k_big_eval<MAXC, TASK, 256>), dim3(grid_for(nl[1], 1, num_cu * 4)), dim3(kBlock), 0, s,
+                         d, lv, ba, (const uint32_t*)(B.lists.p + max_chunks), (int64_t)nl[1]);
+    hipLaunchKernelGGL(k_big_reduce<TASK>, dim3(grid_for(n_items)), dim3(kBlock), 0, s, d, ba, it, 0);
+    if (it.bytes) {
+      // algorithmic bytes of the batch: 12 (20) bytes per entry
+      unsigned long long by = (unsigned long long)el * (TASK == 0 ? 12ull : 20ull);
+      hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(1), 0, s, it.bytes, by);
+    }
+    k0 = k1;
+  }
+}
+
+using BigFn = void (*)(BigState&, const DataArgs&, const LevelArgs&, const ItemArgs&, const uint32_t*, int64_t,
+                       hipStream_t, int);
+BigFn pick_big(int task, int C) {
+  if (task == 1) return big_pipeline<1, 1>;
+  if (C <= 2) return big_pipeline<2, 0>;
+  if (C <= 4) return big_pipeline<4, 0>;
+  if (C <= 8) return big_pipeline<8, 0>;
+  if (C <= 16) return big_pipeline<16, 0>;
+  return big_pipeline<32, 0>;
+}
+
 using SearchFn = void (*)(int, int, int, hipStream_t, const DataArgs&, const LevelArgs&, const ItemArgs&);
 using PartFn = void (*)(int, hipStream_t, const DataArgs&, const LevelArgs&, const PartArgs&);
+using PartWaveFn = void (*)(int, hipStream_t, const DataArgs&, const LevelArgs&, const PartArgs&, const uint32_t*,
+                            int64_t);
 
 SearchFn pick_search(int task, int C) {
   if (task == 1) return launch_search<1, 1>;
@@ -1767,6 +2458,15 @@ SearchFn pick_search(int task, int C) {
   if (C <= 16) return launch_search<16, 0>;
   return launch_search<32, 0>;
 }
+PartWaveFn pick_part_wave(int task, int C) {
+  if (task == 1) return launch_partition_wave<1, 1>;
+  if (C <= 2) return launch_partition_wave<2, 0>;
+  if (C <= 4) return launch_partition_wave<4, 0>;
+  if (C <= 8) return launch_partition_wave<8, 0>;
+  if (C <= 16) return launch_partition_wave<16, 0>;
+  return launch_partition_wave<32, 0>;
+}
+
 PartFn pick_part(int task, int C) {
   if (task == 1) return launch_partition<1, 1>;
   if (C <= 2) return launch_partition<2, 0>;
@@ -1921,10 +2621,12 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
     hipLaunchKernelGGL(k_nonzero, dim3(grid_for(TN)), dim3(kBlock), 0, s, mult.p, TN, flag.p);
     scan_u32(flag.p, TN, pos.p, s, scan_tmp);
     std::vector<uint32_t> starts(T + 1);
-    for (int t = 0; t <= T; ++t) {
-      DTE_HIP(hipMemcpyAsync(&starts[t], pos.p + (int64_t)t * N, 4, hipMemcpyDeviceToHost, s));
+    {
+      DevBuf<uint32_t> st(T + 1);
+      hipLaunchKernelGGL(k_strided_gather, dim3(grid_for(T + 1)), dim3(kBlock), 0, s, pos.p, N, T + 1, st.p);
+      DTE_HIP(hipMemcpyAsync(starts.data(), st.p, 4 * (T + 1), hipMemcpyDeviceToHost, s));
+      DTE_HIP(hipStreamSynchronize(s));
     }
-    DTE_HIP(hipStreamSynchronize(s));
     E = starts[T];
     for (int t = 0; t < T; ++t) {
       root_start[t] = starts[t];
@@ -2071,37 +2773,49 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
   const int search_grid_cap = num_cu * 4;
 
   SearchFn search = pick_search(task, C);
+  BigFn big = pick_big(task, C);
+  BigState bigst;
+  const bool use_big_pipeline = std::getenv("DTE_NO_BIG_PIPELINE") == nullptr;
   PartFn partition = pick_part(task, C);
+  PartWaveFn partition_wave = pick_part_wave(task, C);
 
   // Size tiers of the split search: wave per item up to wave_cutoff entries,
   // block per item above (the block kernel itself switches to the bucketed
   // path above its LDS capacity).
   const int cut0 = algorithm == DTE_RANDOM_FOREST ? kWaveItemMax : 4096;
   const int cut1 = algorithm == DTE_RANDOM_FOREST ? 64 * kIpt : 4096;
-  DevBuf<uint32_t> tier_list[3];
-  DevBuf<unsigned int> tier_cnt(3);
+  int big_min = 0x7FFFFFFF;  // entries from which a node goes through the chunk pipeline (off by default)
+  if (const char* e = std::getenv("DTE_BIG_MIN")) big_min = std::atoi(e);
+  DevBuf<uint32_t> tier_list[4];
+  DevBuf<unsigned int> tier_cnt(4);
   const int wave_grid_cap = num_cu * 16;
   LevelArgs lv{};
   int64_t trace_tier[3] = {0, 0, 0};
   auto run_phase = [&](ItemArgs it) {
     for (auto& tl : tier_list) tl.reserve(it.n_groups);
-    DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 12, s));
+    DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 16, s));
     hipLaunchKernelGGL(k_tier_split, dim3(grid_for(it.n_groups)), dim3(kBlock), 0, s, it.list, it.n_groups,
-                       lv.open_idx, lv.m, cut0, cut1, tier_list[0].p, tier_list[1].p, tier_list[2].p, tier_cnt.p);
-    unsigned int cnt[3];
-    DTE_HIP(hipMemcpyAsync(cnt, tier_cnt.p, 12, hipMemcpyDeviceToHost, s));
+                       lv.open_idx, lv.m, cut0, cut1,
+                       (algorithm == DTE_RANDOM_FOREST && use_big_pipeline) ? big_min : 0x7FFFFFFF,
+                       tier_list[0].p, tier_list[1].p, tier_list[2].p, tier_list[3].p, tier_cnt.p);
+    unsigned int cnt[4];
+    DTE_HIP(hipMemcpyAsync(cnt, tier_cnt.p, 16, hipMemcpyDeviceToHost, s));
     DTE_HIP(hipStreamSynchronize(s));
     Timer tm("split_search", 0.0, s);
-    for (int tier = 0; tier < 3; ++tier) {
+    for (int tier = 0; tier < 4; ++tier) {
       if (!cnt[tier]) continue;
       ItemArgs a = it;
       a.list = tier_list[tier].p;
       a.n_groups = cnt[tier];
+      trace_tier[std::min(tier, 2)] += cnt[tier];
+      if (tier == 3) {
+        big(bigst, d, lv, a, tier_list[3].p, cnt[3], s, num_cu);
+        continue;
+      }
       const int64_t items = (int64_t)cnt[tier] * it.per_group;
       const int per_block = tier == 2 ? 1 : kWaves;
       const int cap = tier == 2 ? search_grid_cap : wave_grid_cap;
       search(algorithm, tier, (int)std::min<int64_t>((items + per_block - 1) / per_block, cap), s, d, lv, a);
-      trace_tier[tier] += cnt[tier];
     }
   };
   const bool trace = std::getenv("DTE_TRACE") != nullptr;
@@ -2236,8 +2950,24 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
       pa.split_ok = split_ok.p;
       pa.bytes = part_bytes.p;
       {
+        for (auto& tl : tier_list) tl.reserve(n_open);
+        DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 16, s));
+        hipLaunchKernelGGL(k_tier_split, dim3(grid_for(n_open)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
+                           n_open, lv.open_idx, lv.m, 512, 0x7FFFFFFF, 0x7FFFFFFF, tier_list[0].p, tier_list[1].p,
+                           tier_list[2].p, tier_list[3].p, tier_cnt.p);
+        unsigned int pc[4];
+        DTE_HIP(hipMemcpyAsync(pc, tier_cnt.p, 16, hipMemcpyDeviceToHost, s));
+        DTE_HIP(hipStreamSynchronize(s));
         Timer tm("partition", 0.0, s);
-        partition((int)std::min<int64_t>(n_open, search_grid_cap), s, d, lv, pa);
+        if (pc[0])
+          partition_wave((int)std::min<int64_t>((pc[0] + kWaves - 1) / kWaves, wave_grid_cap), s, d, lv, pa,
+                         tier_list[0].p, pc[0]);
+        if (pc[1]) {
+          // workgroup partition over the remaining open positions
+          pa.list = tier_list[1].p;
+          pa.n_open = pc[1];
+          partition((int)std::min<int64_t>(pc[1], search_grid_cap), s, d, lv, pa);
+        }
       }
     }
 
