@@ -1385,6 +1385,180 @@ __global__ void __launch_bounds__(kBlock) k_search_rf_wave(DataArgs d, LevelArgs
   if (lane == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
 }
 
+// Segment-per-node RF search for tiny nodes: a wave holds 64/SEG nodes, one
+// per SEG-lane segment (one entry per lane, m <= SEG, C <= SEG), and loops
+// over the node's draws, so node statistics and entry loads are shared by
+// all draws.  Shuffles, scans and reductions stay inside the segment.
+template <int SEG>
+__device__ __forceinline__ int64_t seg_excl_i64(int64_t v) {
+  const int sl = threadIdx.x & (SEG - 1);
+  int64_t x = v;
+#pragma unroll
+  for (int off = 1; off < SEG; off <<= 1) {
+    const int64_t y = __shfl_up(x, off, SEG);
+    if (sl >= off) x += y;
+  }
+  return x - v;
+}
+
+template <int SEG>
+__device__ __forceinline__ int64_t seg_sum_i64(int64_t v) {
+#pragma unroll
+  for (int off = SEG / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, SEG);
+  return v;
+}
+
+template <int SEG>
+__device__ __forceinline__ uint32_t seg_max_u32(uint32_t v) {
+#pragma unroll
+  for (int off = SEG / 2; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, SEG));
+  return v;
+}
+
+template <int SEG>
+__device__ __forceinline__ BestCand seg_best(BestCand b) {
+#pragma unroll
+  for (int off = SEG / 2; off > 0; off >>= 1) {
+    const double g = __shfl_xor(b.g, off, SEG);
+    const uint32_t k = __shfl_xor(b.key, off, SEG);
+    const uint32_t p = __shfl_xor(b.prev, off, SEG);
+    if (cand_better(g, k, b.g, b.key)) {
+      b.g = g;
+      b.key = k;
+      b.prev = p;
+    }
+  }
+  return b;
+}
+
+template <int MAXC, int TASK, int SEG>
+__global__ void __launch_bounds__(kBlock) k_search_rf_seg(DataArgs d, LevelArgs lv, ItemArgs it) {
+  constexpr int NPW = kWave / SEG;  // nodes per wave
+  __shared__ double s_inv[128], s_logc[128];
+  __shared__ int64_t s_T[kTT];
+  load_tables(s_inv, s_logc, TASK == 0 ? s_T : nullptr, d.T);
+  __syncthreads();
+  const LnCtx lx{s_inv, s_logc, s_T, d.igF};
+  const int sl = threadIdx.x & (SEG - 1);
+  const int seg = (threadIdx.x & (kWave - 1)) / SEG;
+  unsigned long long my_bytes = 0;
+  const int64_t wave_id = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
+  const int64_t n_waves = (int64_t)gridDim.x * kWaves;
+  for (int64_t g0 = wave_id * NPW; g0 < it.n_groups; g0 += n_waves * NPW) {
+    const int64_t g = g0 + seg;
+    const bool node_ok = g < it.n_groups;
+    uint32_t o = 0, i = 0, tg = 0, nid = 0;
+    int m = 0;
+    int64_t st = 0, n = 0, ssum = 0;
+    if (node_ok) {
+      o = it.list ? it.list[g] : (uint32_t)g;
+      i = lv.open_idx[o];
+      tg = (uint32_t)lv.tree_global[lv.tree[i]];
+      nid = lv.id[i];
+      m = lv.m[i];
+      st = lv.start[i];
+      n = lv.count[i];
+      if (TASK == 1) ssum = lv.sum[i];
+    }
+    // node statistics (shared by all draws)
+    uint32_t ncls_lane = 0;
+    int64_t P = 0, SR0 = 0;
+    double inv = 0.0;
+    bool pure = false;
+    if (TASK == 0) {
+      ncls_lane = (node_ok && sl < d.C) ? lv.cls[(int64_t)i * d.C + sl] : 0u;
+      pure = (int64_t)seg_max_u32<SEG>(ncls_lane) == n;
+      SR0 = seg_sum_i64<SEG>(sl < d.C ? lx.t(ncls_lane) : 0);
+      P = lx.t((uint64_t)n) - SR0;
+      inv = 1.0 / ((double)n * kLn2);
+    }
+    const bool valid = node_ok && sl < m;
+    uint32_t row = 0, pay0 = 0;
+    int64_t y0 = 0;
+    if (valid) {
+      row = lv.row[st + sl];
+      pay0 = lv.aux[st + sl];
+      if (TASK == 1) y0 = lv.ey[st + sl];
+    }
+    if (sl == 0 && node_ok && !pure) my_bytes += (unsigned long long)m * it.per_group * (TASK == 0 ? 12ull : 20ull);
+    for (int jj = 0; jj < it.per_group; ++jj) {
+      const int j = it.j0 + jj;
+      const int f = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)j) % (uint32_t)d.F);
+      const int64_t ridx = (int64_t)o * it.D + j;
+      if (pure) {
+        if (sl == 0 && node_ok) {
+          it.res_score[ridx] = 0.0f;
+          it.res_thr[ridx] = 0.0f;
+          it.res_feat[ridx] = f;
+        }
+        continue;
+      }
+      uint32_t key = valid ? d.K[(int64_t)f * d.N + row] : kNone;
+      // rank inside the segment (ties by lane), then move entries into key order
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int r = 0; r < SEG; ++r) {
+        const uint32_t kr = __shfl(key, r, SEG);
+        cnt += (r < m && (kr < key || (kr == key && r < sl))) ? 1u : 0u;
+      }
+      const uint32_t rank = valid ? cnt : (uint32_t)sl;
+      const uint32_t dst = (uint32_t)(threadIdx.x & (kWave - 1) & ~(SEG - 1)) + rank;
+      key = ds_permute_u32(dst, key);
+      const uint32_t pay = ds_permute_u32(dst, pay0);
+      int64_t yv = 0;
+      if (TASK == 1) {
+        const uint32_t lo = ds_permute_u32(dst, (uint32_t)(uint64_t)y0);
+        const uint32_t hi = ds_permute_u32(dst, (uint32_t)((uint64_t)y0 >> 32));
+        yv = (int64_t)(((uint64_t)hi << 32) | lo);
+      }
+      const uint32_t w = valid ? (pay & 0xFFFFu) : 0u;
+      const uint32_t prevk = __shfl_up(key, 1, SEG);
+      const bool cand = valid && sl >= 1 && prevk < key;
+      double gval = 0.0;
+      if (TASK == 0) {
+        const uint32_t lab = pay >> 16;
+        uint32_t pre = 0;
+#pragma unroll
+        for (int r = 0; r < SEG; ++r) {
+          const uint32_t pr = __shfl(pay, r, SEG);
+          pre += (r < sl && (pr >> 16) == lab) ? (pr & 0xFFFFu) : 0u;
+        }
+        const uint32_t ncl = __shfl(ncls_lane, (int)lab, SEG);
+        int64_t dl = 0, dr = 0;
+        if (valid) {
+          dl = lx.t(pre + w) - lx.t(pre);
+          dr = lx.t(ncl - pre - w) - lx.t(ncl - pre);
+        }
+        const int64_t slv = seg_excl_i64<SEG>(dl);
+        const int64_t srv = SR0 + seg_excl_i64<SEG>(dr);
+        const int64_t nl = seg_excl_i64<SEG>((int64_t)w);
+        if (cand) {
+          const int64_t q = (slv - lx.t((uint64_t)nl)) + (srv - lx.t((uint64_t)(n - nl)));
+          gval = ig_from_fixed(P + q, lx.F, inv);
+        }
+      } else {
+        const int64_t pn = seg_excl_i64<SEG>((int64_t)w);
+        const int64_t ps = seg_excl_i64<SEG>((int64_t)w * yv);
+        if (cand) gval = variance_gain(pn, ps, n - pn, ssum - ps, 0);
+      }
+      BestCand b = (cand && gval > 0.0) ? BestCand{gval, key, prevk} : BestCand{0.0, kNone, kNone};
+      b = seg_best<SEG>(b);
+      if (sl == 0 && node_ok) {
+        float score = 0.0f, thr = 0.0f;
+        if (b.key != kNone) {
+          const float* sv = d.SV + d.SVoff[f];
+          score = (float)(TASK == 0 ? b.g : ldexp_d(b.g, -2 * d.shift));
+          thr = midpoint(sv[b.prev], sv[b.key]);
+        }
+        it.res_score[ridx] = score;
+        it.res_thr[ridx] = thr;
+        it.res_feat[ridx] = f;
+      }
+    }
+  }
+  if (sl == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
+}
+
 // ERT, one wave per item (any node size; lanes stride over the entries).
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArgs lv, ItemArgs it) {
@@ -1479,10 +1653,19 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArg
 // Split a list of open positions (or all of them) by node size into the
 // wave tier and the block tier.  Order inside a tier is irrelevant: every
 // item writes its own result slot.
-// Wave-aggregated: one global atomic per wave and tier.
-__global__ void k_tier_split(const uint32_t* list, int64_t n, const uint32_t* open_idx, const int32_t* node_m,
-                             int cut0, int cut1, int cut2, uint32_t* t0, uint32_t* t1, uint32_t* t2, uint32_t* t3,
-                             unsigned int* counts) {
+// Wave-aggregated: one global atomic per wave and tier.  Tier k holds the
+// groups with cuts[k-1] < m <= cuts[k].
+constexpr int kMaxTiers = 6;
+struct Cuts {
+  int c[kMaxTiers];  // ascending; the last tier takes everything above
+  int n;             // number of tiers
+};
+struct TierLists {
+  uint32_t* l[kMaxTiers];
+};
+
+__global__ void k_tier_split(const uint32_t* list, int64_t n, const uint32_t* open_idx, const int32_t* node_m, Cuts cuts,
+                             TierLists out, unsigned int* counts) {
   const int lane = threadIdx.x & (kWave - 1);
   for (int64_t base = (blockIdx.x * (int64_t)blockDim.x) & ~(int64_t)(kWave - 1); base < n;
        base += (int64_t)gridDim.x * blockDim.x) {
@@ -1492,21 +1675,21 @@ __global__ void k_tier_split(const uint32_t* list, int64_t n, const uint32_t* op
     if (g < n) {
       o = list ? list[g] : (uint32_t)g;
       const int m = node_m[open_idx[o]];
-      tier = m <= cut0 ? 0 : (m <= cut1 ? 1 : (m < cut2 ? 2 : 3));
+      tier = cuts.n - 1;
+      for (int k = 0; k < cuts.n - 1; ++k)
+        if (m <= cuts.c[k]) {
+          tier = k;
+          break;
+        }
     }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < cuts.n; ++k) {
       const uint64_t mask = __ballot(tier == k);
       if (!mask) continue;
       const int leader = __ffsll((unsigned long long)mask) - 1;
       unsigned int basepos = 0;
       if (lane == leader) basepos = atomicAdd(&counts[k], (unsigned int)__popcll(mask));
       basepos = __shfl(basepos, leader, kWave);
-      if (tier == k) {
-        const unsigned int r = (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
-        uint32_t* dst = k == 0 ? t0 : (k == 1 ? t1 : (k == 2 ? t2 : t3));
-        dst[basepos + r] = o;
-      }
+      if (tier == k) out.l[k][basepos + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull))] = o;
     }
   }
 }
@@ -2258,18 +2441,21 @@ struct Timer {
   }
 };
 
-// tier: 0 = one wave per item, one entry per lane; 1 = wave teams (four
-// items per workgroup); 2 = whole workgroup per item.
+// RF tiers: 0/1/2 = one 16/32/64-lane segment per node (one entry per lane,
+// draws looped in the kernel); 3 = wave teams (four items per workgroup);
+// 4 = whole workgroup per item.  ERT tiers: 0 = one wave per item,
+// 1 = workgroup per item.
 template <int MAXC, int TASK>
 void launch_search(int algorithm, int tier, int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv,
                    const ItemArgs& it) {
   if (algorithm == DTE_RANDOM_FOREST) {
-    if (tier == 0)
-      hipLaunchKernelGGL((k_search_rf_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
-    else if (tier == 1)
-      hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
-    else
-      hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
+    switch (tier) {
+      case 0: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 16>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 1: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 32>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 2: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 3: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      default: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+    }
   } else {
     if (tier == 0)
       hipLaunchKernelGGL((k_search_ert_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
@@ -2782,40 +2968,58 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
   // Size tiers of the split search: wave per item up to wave_cutoff entries,
   // block per item above (the block kernel itself switches to the bucketed
   // path above its LDS capacity).
-  const int cut0 = algorithm == DTE_RANDOM_FOREST ? kWaveItemMax : 4096;
-  const int cut1 = algorithm == DTE_RANDOM_FOREST ? 64 * kIpt : 4096;
   int big_min = 0x7FFFFFFF;  // entries from which a node goes through the chunk pipeline (off by default)
   if (const char* e = std::getenv("DTE_BIG_MIN")) big_min = std::atoi(e);
-  DevBuf<uint32_t> tier_list[4];
-  DevBuf<unsigned int> tier_cnt(4);
+  Cuts cuts{};
+  if (algorithm == DTE_RANDOM_FOREST) {
+    const int c16 = (task == 1 || C <= 16) ? 16 : 0;
+    const int c32 = (task == 1 || C <= 32) ? 32 : 0;
+    cuts = Cuts{{c16, std::max(c16, c32), 64, 64 * kIpt, std::max(64 * kIpt, big_min - 1), 0x7FFFFFFF}, 6};
+  } else {
+    cuts = Cuts{{4096, 0x7FFFFFFF, 0, 0, 0, 0}, 2};
+  }
+  DevBuf<uint32_t> tier_list[kMaxTiers];
+  DevBuf<unsigned int> tier_cnt(kMaxTiers);
+  auto tier_lists = [&]() {
+    TierLists tl{};
+    for (int k = 0; k < kMaxTiers; ++k) tl.l[k] = tier_list[k].p;
+    return tl;
+  };
   const int wave_grid_cap = num_cu * 16;
   LevelArgs lv{};
   int64_t trace_tier[3] = {0, 0, 0};
   auto run_phase = [&](ItemArgs it) {
     for (auto& tl : tier_list) tl.reserve(it.n_groups);
-    DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 16, s));
+    DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
     hipLaunchKernelGGL(k_tier_split, dim3(grid_for(it.n_groups)), dim3(kBlock), 0, s, it.list, it.n_groups,
-                       lv.open_idx, lv.m, cut0, cut1,
-                       (algorithm == DTE_RANDOM_FOREST && use_big_pipeline) ? big_min : 0x7FFFFFFF,
-                       tier_list[0].p, tier_list[1].p, tier_list[2].p, tier_list[3].p, tier_cnt.p);
-    unsigned int cnt[4];
-    DTE_HIP(hipMemcpyAsync(cnt, tier_cnt.p, 16, hipMemcpyDeviceToHost, s));
+                       lv.open_idx, lv.m, cuts, tier_lists(), tier_cnt.p);
+    unsigned int cnt[kMaxTiers];
+    DTE_HIP(hipMemcpyAsync(cnt, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
     DTE_HIP(hipStreamSynchronize(s));
     Timer tm("split_search", 0.0, s);
-    for (int tier = 0; tier < 4; ++tier) {
+    for (int tier = 0; tier < cuts.n; ++tier) {
       if (!cnt[tier]) continue;
       ItemArgs a = it;
       a.list = tier_list[tier].p;
       a.n_groups = cnt[tier];
-      trace_tier[std::min(tier, 2)] += cnt[tier];
-      if (tier == 3) {
-        big(bigst, d, lv, a, tier_list[3].p, cnt[3], s, num_cu);
+      const bool rf = algorithm == DTE_RANDOM_FOREST;
+      trace_tier[rf ? (tier <= 2 ? 0 : (tier == 3 ? 1 : 2)) : (tier == 0 ? 0 : 2)] += cnt[tier];
+      if (rf && tier == 5) {
+        big(bigst, d, lv, a, tier_list[5].p, cnt[5], s, num_cu);
         continue;
       }
-      const int64_t items = (int64_t)cnt[tier] * it.per_group;
-      const int per_block = tier == 2 ? 1 : kWaves;
-      const int cap = tier == 2 ? search_grid_cap : wave_grid_cap;
-      search(algorithm, tier, (int)std::min<int64_t>((items + per_block - 1) / per_block, cap), s, d, lv, a);
+      int grid;
+      if (rf && tier <= 2) {
+        const int npw = tier == 0 ? 4 : (tier == 1 ? 2 : 1);
+        grid = (int)std::min<int64_t>((cnt[tier] + npw * kWaves - 1) / (npw * kWaves), wave_grid_cap);
+      } else {
+        const int64_t items = (int64_t)cnt[tier] * it.per_group;
+        const bool whole = rf ? tier == 4 : tier == 1;
+        const int per_block = whole ? 1 : kWaves;
+        const int cap = whole ? search_grid_cap : wave_grid_cap;
+        grid = (int)std::min<int64_t>((items + per_block - 1) / per_block, cap);
+      }
+      search(algorithm, tier, grid, s, d, lv, a);
     }
   };
   const bool trace = std::getenv("DTE_TRACE") != nullptr;
@@ -2951,12 +3155,12 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
       pa.bytes = part_bytes.p;
       {
         for (auto& tl : tier_list) tl.reserve(n_open);
-        DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 16, s));
+        DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
         hipLaunchKernelGGL(k_tier_split, dim3(grid_for(n_open)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
-                           n_open, lv.open_idx, lv.m, 512, 0x7FFFFFFF, 0x7FFFFFFF, tier_list[0].p, tier_list[1].p,
-                           tier_list[2].p, tier_list[3].p, tier_cnt.p);
-        unsigned int pc[4];
-        DTE_HIP(hipMemcpyAsync(pc, tier_cnt.p, 16, hipMemcpyDeviceToHost, s));
+                           n_open, lv.open_idx, lv.m, Cuts{{512, 0x7FFFFFFF, 0, 0, 0, 0}, 2}, tier_lists(),
+                           tier_cnt.p);
+        unsigned int pc[kMaxTiers];
+        DTE_HIP(hipMemcpyAsync(pc, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
         DTE_HIP(hipStreamSynchronize(s));
         Timer tm("partition", 0.0, s);
         if (pc[0])
