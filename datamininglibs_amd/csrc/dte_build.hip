@@ -32,6 +32,7 @@
 #include <thread>
 #include <cstdlib>
 #include <chrono>
+#include <memory>
 
 #include "dte_block.h"
 #include "dte_internal.h"
@@ -99,6 +100,7 @@ struct ItemArgs {
   float* res_thr;
   int32_t* res_feat;
   unsigned long long* bytes;  // algorithmic bytes counter (may be NULL)
+  int node_major;        // 1: a grid of n_groups teams, each team walks its own group's draws in order
   uint32_t* scr_key;     // big-item scratch, one slab per block
   uint32_t* scr_aux;
   int64_t* scr_y;
@@ -684,8 +686,8 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
   const int64_t n_items = it.n_groups * it.per_group;
   const int team_global = (int)blockIdx.x * NT + team;
   for (int64_t item = team_global; item < n_items; item += (int64_t)gridDim.x * NT) {
-    const int64_t grp = item / it.per_group;
-    const int j = it.j0 + (int)(item % it.per_group);
+    const int64_t grp = it.node_major ? item % it.n_groups : item / it.per_group;
+    const int j = it.j0 + (int)(it.node_major ? item / it.n_groups : item % it.per_group);
     const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
     const uint32_t i = lv.open_idx[o];
     const int f = (int)(rand_u32(d.seed, kStreamFeat, (uint32_t)lv.tree_global[lv.tree[i]], lv.id[i], (uint64_t)j) %
@@ -1145,8 +1147,8 @@ __global__ void __launch_bounds__(kBlock) k_search_ert(DataArgs d, LevelArgs lv,
   unsigned long long my_bytes = 0;
   const int64_t n_items = it.n_groups * it.per_group;
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-    const int64_t grp = item / it.per_group;
-    const int j = it.j0 + (int)(item % it.per_group);
+    const int64_t grp = it.node_major ? item % it.n_groups : item / it.per_group;
+    const int j = it.j0 + (int)(it.node_major ? item / it.n_groups : item % it.per_group);
     const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
     const uint32_t i = lv.open_idx[o];
     const int tr = lv.tree[i];
@@ -2366,6 +2368,24 @@ __global__ void k_compact_list(const uint32_t* flag, const uint32_t* pos, int64_
 
 __global__ void k_add_u64(unsigned long long* p, unsigned long long v) { *p += v; }
 
+__global__ void k_iota(uint32_t* out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (uint32_t)i;
+}
+
+// Appends one tree batch to the merged canonical arrays: child links and
+// tree starts shift by the batch's node offset.
+__global__ void k_offset_links(const int32_t* left, int64_t n, int32_t off, int32_t* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t l = left[i];
+    out[i] = l < 0 ? l : l + off;
+  }
+}
+__global__ void k_offset_i64(const int64_t* in, int64_t n, int64_t off, int64_t* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = in[i] + off;
+}
+
 __global__ void k_strided_gather(const uint32_t* in, int64_t stride, int n, uint32_t* out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = in[(int64_t)i * stride];
@@ -2689,27 +2709,47 @@ void upload_ln_table(int device) {
 }
 
 // ------------------------------------------------------------ Builder ---
-Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_t F, const dte_params& p,
-                   hipStream_t s) {
+// Plain launch structure of the reference: at most 100 trees per batch
+// (gpudte_algorithm.cc:116-118), at most 512 open nodes per launch chunk
+// (gpudte_algorithm.cc:149-151).
+constexpr int kPlainTrees = 100;
+constexpr int64_t kPlainChunk = 512;
+
+// Device-resident inputs of one fit, shared by every tree batch.
+struct FitData {
+  DevBuf<float> X;         // canonical copy (-0 -> +0), feature-major
+  DevBuf<uint32_t> label;  // classification
+  DevBuf<int64_t> yfix;    // regression: fixed-point targets
+  int shift = 0;
+  DevBuf<uint32_t> K;      // dense ranks (random forest)
+  DevBuf<float> SV;
+  DevBuf<int64_t> SVoff;
+  DevBuf<uint32_t> Sdev;
+  DevBuf<int64_t> Ttab;    // fixed-point x ln x for small counts
+  DataArgs d{};
+};
+
+void prepare_data(FitData& fd, const float* X_dev_in, const float* y_host, int64_t N, int32_t F,
+                  const dte_params& p, hipStream_t s) {
   const int task = p.task;
   const int algorithm = p.algorithm;
   const int C = task == 0 ? p.n_classes : 1;
-  const int T = p.n_trees;
-  const int k = p.n_features_per_node > 0 ? p.n_features_per_node
-                                          : (int)std::round(std::log((double)F)) + 1;
-  const int D0 = std::min(k, F);
-  const int R = k < F ? std::min(10, F - k) : 0;
-  const int D = D0 + R;
   if (C > 32) fail(DTE_ERR_INVALID_ARGUMENT, "at most 32 classes are supported");
   if (N > (int64_t)0x7FFFFFFF) fail(DTE_ERR_INVALID_ARGUMENT, "too many rows");
-
+  DevBuf<float>& X = fd.X;
+  DevBuf<uint32_t>& label = fd.label;
+  DevBuf<int64_t>& yfix = fd.yfix;
+  int& shift = fd.shift;
+  DevBuf<uint32_t>& K = fd.K;
+  DevBuf<float>& SV = fd.SV;
+  DevBuf<int64_t>& SVoff = fd.SVoff;
+  DevBuf<uint32_t>& Sdev = fd.Sdev;
+  DevBuf<int64_t>& Ttab = fd.Ttab;
+  DataArgs& d = fd.d;
   // ---- dataset: canonical copy of X, labels / fixed-point targets.
-  DevBuf<float> X((size_t)N * F);
+  X.alloc((size_t)N * F);
   DTE_HIP(hipMemcpyAsync(X.p, X_dev_in, sizeof(float) * N * F, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_canon, dim3(grid_for((int64_t)N * F)), dim3(kBlock), 0, s, X.p, (int64_t)N * F);
-  DevBuf<uint32_t> label;
-  DevBuf<int64_t> yfix;
-  int shift = 0;
   if (task == 0) {
     std::vector<uint32_t> lab(N);
     for (int64_t i = 0; i < N; ++i) {
@@ -2730,10 +2770,6 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
     DTE_HIP(hipMemcpyAsync(yfix.p, yf.data(), 8 * N, hipMemcpyHostToDevice, s));
     DTE_HIP(hipStreamSynchronize(s));
   }
-  DevBuf<uint32_t> K;
-  DevBuf<float> SV;
-  DevBuf<int64_t> SVoff;
-  DevBuf<uint32_t> Sdev;
   std::vector<DevBuf<uint32_t>> scan_tmp;
   if (algorithm == DTE_RANDOM_FOREST) {
     {
@@ -2752,7 +2788,7 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
         fail(DTE_ERR_INVALID_ARGUMENT, "too many distinct values in a feature for the split-search buckets");
     }
   }
-  DataArgs d{};
+  d = DataArgs{};
   d.X = X.p;
   d.K = K.p;
   d.SV = SV.p;
@@ -2765,7 +2801,7 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
   d.seed = p.seed;
   d.min_node = p.min_node_size;
   d.max_depth = p.max_depth;
-  DevBuf<int64_t> Ttab(kTT);
+  Ttab.alloc(kTT);
   {
     const LnTable& lt = host_ln_table();
     d.igF = ig_shift((uint64_t)N, task == 0 ? C : 1, lt.inv, lt.logc);
@@ -2774,9 +2810,30 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
     DTE_HIP(hipMemcpyAsync(Ttab.p, th.data(), 8 * kTT, hipMemcpyHostToDevice, s));
     d.T = Ttab.p;
   }
+}
 
+// Builds trees [t0, t0 + T) of the fit level by level.  plain = 1 is the
+// per-node launch structure of the reference (gpudte_algorithm.cc:116-217):
+// open nodes in chunks of at most 512, one workgroup per node walking its
+// feature draws in order, a host round trip after every chunk; the same
+// device functions compute every result, so both modes give the same bits.
+Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_params& p, hipStream_t s,
+                  bool plain) {
+  const int task = p.task;
+  const int algorithm = p.algorithm;
+  const int C = task == 0 ? p.n_classes : 1;
+  const int k = p.n_features_per_node > 0 ? p.n_features_per_node
+                                          : (int)std::round(std::log((double)F)) + 1;
+  const int D0 = std::min(k, F);
+  const int R = k < F ? std::min(10, F - k) : 0;
+  const int D = D0 + R;
+  const DataArgs d = fd.d;
+  const uint32_t* label = fd.label.p;
+  const int64_t* yfix = fd.yfix.p;
+  const int shift = fd.shift;
+  std::vector<DevBuf<uint32_t>> scan_tmp;
   std::vector<int32_t> tg_h(T);
-  std::iota(tg_h.begin(), tg_h.end(), 0);
+  std::iota(tg_h.begin(), tg_h.end(), t0);
   DevBuf<int32_t> tree_global(T);
   DTE_HIP(hipMemcpyAsync(tree_global.p, tg_h.data(), 4 * T, hipMemcpyHostToDevice, s));
 
@@ -2829,7 +2886,7 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
     DevBuf<uint32_t> overflow(1);
     DTE_HIP(hipMemsetAsync(overflow.p, 0, 4, s));
     hipLaunchKernelGGL(k_boot_compact, dim3(grid_for(N, kBlock, 4096), T), dim3(kBlock), 0, s, mult.p, pos.p, N,
-                       label.p, yfix.p, C, rowA.p, auxA.p, eyA.p, root_cls.p, root_sum.p, root_hash.p,
+                       label, yfix, C, rowA.p, auxA.p, eyA.p, root_cls.p, root_sum.p, root_hash.p,
                        overflow.p);
     if (read_one(overflow.p, s)) fail(DTE_ERR_INTERNAL, "bootstrap multiplicity above 65535");
   } else {
@@ -2846,16 +2903,16 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
       eyA.alloc(E);
       eyB.alloc(E);
     }
-    hipLaunchKernelGGL(k_identity_entries, dim3(grid_for(N, kBlock, 4096), T), dim3(kBlock), 0, s, N, T, label.p,
-                       yfix.p, rowA.p, auxA.p, eyA.p);
+    hipLaunchKernelGGL(k_identity_entries, dim3(grid_for(N, kBlock, 4096), T), dim3(kBlock), 0, s, N, T, label,
+                       yfix, rowA.p, auxA.p, eyA.p);
     DevBuf<uint32_t> c1(std::max(C, 1));
     DevBuf<int64_t> s1(1);
     DevBuf<uint64_t> h1(1);
     DTE_HIP(hipMemsetAsync(c1.p, 0, c1.bytes(), s));
     DTE_HIP(hipMemsetAsync(s1.p, 0, 8, s));
     DTE_HIP(hipMemsetAsync(h1.p, 0, 8, s));
-    hipLaunchKernelGGL(k_root_stats_identity, dim3(grid_for(N, kBlock, 4096)), dim3(kBlock), 0, s, N, label.p,
-                       yfix.p, C, c1.p, s1.p, h1.p);
+    hipLaunchKernelGGL(k_root_stats_identity, dim3(grid_for(N, kBlock, 4096)), dim3(kBlock), 0, s, N, label,
+                       yfix, C, c1.p, s1.p, h1.p);
     std::vector<uint32_t> c1h(std::max(C, 1));
     int64_t s1h;
     uint64_t h1h;
@@ -3022,6 +3079,27 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
       search(algorithm, tier, grid, s, d, lv, a);
     }
   };
+  // Plain launch structure: chunks of at most kPlainChunk open nodes, one
+  // workgroup per node, a host round trip after every chunk.
+  DevBuf<uint32_t> iota_list;
+  auto identity_list = [&](int64_t n) {
+    iota_list.reserve(n);
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(kBlock), 0, s, iota_list.p, n);
+    return (const uint32_t*)iota_list.p;
+  };
+  auto plain_phase = [&](ItemArgs it) {
+    const uint32_t* lst = it.list ? it.list : identity_list(it.n_groups);
+    const int tier = algorithm == DTE_RANDOM_FOREST ? 4 : 1;
+    Timer tm("split_search", 0.0, s);
+    for (int64_t c0 = 0; c0 < it.n_groups; c0 += kPlainChunk) {
+      ItemArgs a = it;
+      a.list = lst + c0;
+      a.n_groups = std::min<int64_t>(kPlainChunk, it.n_groups - c0);
+      a.node_major = 1;
+      search(algorithm, tier, (int)a.n_groups, s, d, lv, a);
+      DTE_HIP(hipStreamSynchronize(s));
+    }
+  };
   const bool trace = std::getenv("DTE_TRACE") != nullptr;
 
   int64_t off_level = 0;
@@ -3102,7 +3180,7 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
       it.scr_aux = scr_aux.p;
       it.scr_y = scr_y.p;
       it.scr_stride = scr_stride;
-      run_phase(it);
+      if (plain) plain_phase(it); else run_phase(it);
       DecideArgs da{};
       da.n_open = n_open;
       da.D0 = D0;
@@ -3126,7 +3204,7 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
           it.n_groups = n_retry;
           it.per_group = R;
           it.j0 = D0;
-          run_phase(it);
+          if (plain) plain_phase(it); else run_phase(it);
           da.retry_list = retry_list.p;
           da.n_retry = n_retry;
           hipLaunchKernelGGL(k_decide2, dim3(grid_for(n_retry)), dim3(kBlock), 0, s, da);
@@ -3153,7 +3231,16 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
       pa.l_hash = l_hash.p;
       pa.split_ok = split_ok.p;
       pa.bytes = part_bytes.p;
-      {
+      if (plain) {
+        const uint32_t* lst = identity_list(n_open);
+        Timer tm("partition", 0.0, s);
+        for (int64_t c0 = 0; c0 < n_open; c0 += kPlainChunk) {
+          pa.list = lst + c0;
+          pa.n_open = std::min<int64_t>(kPlainChunk, n_open - c0);
+          partition((int)pa.n_open, s, d, lv, pa);
+          DTE_HIP(hipStreamSynchronize(s));
+        }
+      } else {
         for (auto& tl : tier_list) tl.reserve(n_open);
         DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
         hipLaunchKernelGGL(k_tier_split, dim3(grid_for(n_open)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
@@ -3322,6 +3409,82 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
   build_predict_nodes(*fo, s);
   DTE_HIP(hipStreamSynchronize(s));
   return fo;
+}
+
+// Concatenates tree batches into one forest (the reference's
+// AggregateModels, gpudte_algorithm.cc:333-397, on canonical arrays).
+Forest* merge_forests(std::vector<std::unique_ptr<Forest>>& parts, hipStream_t s) {
+  auto* fo = new Forest();
+  std::unique_ptr<Forest> guard(fo);
+  const Forest& p0 = *parts[0];
+  fo->device = p0.device;
+  fo->task = p0.task;
+  fo->algorithm = p0.algorithm;
+  fo->n_classes = p0.n_classes;
+  fo->n_features = p0.n_features;
+  fo->fixed_shift = p0.fixed_shift;
+  fo->has_stats = true;
+  int64_t n = 0;
+  int T = 0;
+  for (auto& q : parts) {
+    n += q->n_nodes;
+    T += q->n_trees;
+  }
+  fo->n_nodes = n;
+  fo->n_trees = T;
+  const int C = fo->n_classes;
+  const size_t W = fo->task == 0 ? (size_t)C : 1;
+  fo->tree_start.alloc(T + 1);
+  fo->feature.alloc(n);
+  fo->left.alloc(n);
+  fo->depth.alloc(n);
+  fo->count.alloc(n);
+  fo->n_entries.alloc(n);
+  fo->threshold.alloc(n);
+  fo->score.alloc(n);
+  fo->value.alloc(n);
+  fo->rowhash.alloc(n);
+  fo->class_counts.alloc(fo->task == 0 ? (size_t)n * C : 1);
+  fo->sum_fixed.alloc(n);
+  int64_t off = 0;
+  int t_off = 0;
+  for (auto& q : parts) {
+    const int64_t m = q->n_nodes;
+    auto cp = [&](auto* dst, const auto* src, size_t count) {
+      DTE_HIP(hipMemcpyAsync(dst, src, sizeof(*src) * count, hipMemcpyDeviceToDevice, s));
+    };
+    cp(fo->feature.p + off, q->feature.p, m);
+    cp(fo->depth.p + off, q->depth.p, m);
+    cp(fo->count.p + off, q->count.p, m);
+    cp(fo->n_entries.p + off, q->n_entries.p, m);
+    cp(fo->threshold.p + off, q->threshold.p, m);
+    cp(fo->score.p + off, q->score.p, m);
+    cp(fo->value.p + off, q->value.p, m);
+    cp(fo->rowhash.p + off, q->rowhash.p, m);
+    cp(fo->sum_fixed.p + off, q->sum_fixed.p, m);
+    if (fo->task == 0) cp(fo->class_counts.p + off * W, q->class_counts.p, m * W);
+    hipLaunchKernelGGL(k_offset_links, dim3(grid_for(m)), dim3(kBlock), 0, s, q->left.p, m, (int32_t)off,
+                       fo->left.p + off);
+    hipLaunchKernelGGL(k_offset_i64, dim3(grid_for(q->n_trees + 1)), dim3(kBlock), 0, s, q->tree_start.p,
+                       (int64_t)q->n_trees + 1, off, fo->tree_start.p + t_off);
+    off += m;
+    t_off += q->n_trees;
+  }
+  build_predict_nodes(*fo, s);
+  DTE_HIP(hipStreamSynchronize(s));
+  return guard.release();
+}
+
+Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_t F, const dte_params& p,
+                   hipStream_t s) {
+  FitData fd;
+  prepare_data(fd, X_dev_in, y_host, N, F, p, s);
+  if (p.build_mode != DTE_BUILD_PLAIN) return fit_batch(fd, 0, p.n_trees, N, F, p, s, false);
+  std::vector<std::unique_ptr<Forest>> parts;
+  for (int t0 = 0; t0 < p.n_trees; t0 += kPlainTrees)
+    parts.emplace_back(fit_batch(fd, t0, std::min(kPlainTrees, p.n_trees - t0), N, F, p, s, true));
+  if (parts.size() == 1) return parts[0].release();
+  return merge_forests(parts, s);
 }
 
 void build_predict_nodes(Forest& f, hipStream_t s) {

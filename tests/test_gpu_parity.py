@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 from datamininglibs_amd import capi, synth
+import pyoracle
 from parity import assert_same_forest, fit_both
 
 pytestmark = pytest.mark.gpu
@@ -57,3 +58,37 @@ def test_regression(algorithm):
     assert_same_forest(g, o)
     Xp, _ = synth.sine_regression(seed=3, n=500, f=8)
     assert np.array_equal(g.predict(Xp), o.predict(Xp))
+
+
+def _params(algorithm, mode, **kw):
+    return capi.default_params(algorithm, build_mode=mode, **kw)
+
+
+@pytest.mark.parametrize("algorithm", [capi.EXTRA_TREES, capi.RANDOM_FOREST])
+def test_plain_equals_batched_and_oracle_over_tree_batches(algorithm):
+    """130 trees: the plain path builds them as batches of 100 + 30 and
+    concatenates; the batched path builds all at once.  Both equal the
+    oracle."""
+    X, y = small_classes(7, 1500, 10, 3)
+    kw = dict(n_trees=130, n_features_per_node=3, max_depth=14, min_node_size=4, n_classes=3, seed=77)
+    gb = capi.fit(X, y, _params(algorithm, capi.BUILD_BATCHED, **kw))
+    gp = capi.fit(X, y, _params(algorithm, capi.BUILD_PLAIN, **kw))
+    cb, cp = gb.canonical(), gp.canonical()
+    for k in cb:
+        assert np.array_equal(cb[k], cp[k]), k
+    nb, pb = gb.export()
+    npl, ppl = gp.export()
+    assert np.array_equal(nb, npl) and np.array_equal(pb, ppl)
+    o = pyoracle.fit(X, y, algorithm=algorithm, n_trees=130, k=3, max_depth=14, min_node_size=4, n_classes=3,
+                     seed=77)
+    assert_same_forest(gp, o)
+    Xp, _ = small_classes(8, 400, 10, 3)
+    assert np.array_equal(gp.predict(Xp).astype(np.int64), o.predict(Xp).astype(np.int64))
+
+
+@pytest.mark.parametrize("algorithm", [capi.EXTRA_TREES, capi.RANDOM_FOREST])
+def test_plain_regression(algorithm):
+    X, y = synth.sine_regression(seed=4, n=3000, f=6)
+    g, o = fit_both(X, y, algorithm=algorithm, task=capi.REGRESSION, n_trees=3, k=2, max_depth=10,
+                    min_node_size=5, build_mode=capi.BUILD_PLAIN)
+    assert_same_forest(g, o)
