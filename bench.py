@@ -7,17 +7,27 @@ a random linear map plus Gumbel noise; 200 bootstrap trees, entropy
 criterion, 11 features per node, max depth 20, min node size 5.
 
 A step is one full fit of the forest from samples already resident in HBM
-(dense-rank preprocessing, bootstrap, every level of every tree).  `value` is
-trees/s over the timed steps.  Beside it: predict rows/s of the fitted forest
-on the 1M training rows, the split-search roofline (algorithmic bytes over
-the split-search kernel time, HIP events on the library stream) and the CPU
-oracle timed on the host cores on a bounded sample (one tree per core).
+(dense-rank preprocessing, bootstrap, every level of every tree) with the
+batched level-wise build.  `value` is trees/s over the timed steps.  Beside
+it:
+  - the plain per-node build (the reference's launch structure) timed on
+    the same input, and the batched/plain speed-up (the forests must be
+    identical);
+  - predict rows/s of the fitted forest on the 1M training rows;
+  - `roofline`: algorithmic bytes of the dominant split-search kernel over
+    its average launch time, from HIP events on the library stream;
+    `traffic` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    of the same kernel (profiles/traffic_rf_1m128.json) when present;
+  - `cpu_baseline`: the CPU oracle on the host cores, one tree per core, on
+    the same data (a bounded sample of the same workload).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
 import argparse
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -26,17 +36,76 @@ sys.path.insert(0, ROOT)
 
 METRIC = "RF build trees/s at 1M×128 + predict rows/s; p99 latency at fixed QPS"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_rf_1m128.json")
+# family -> kernel symbol prefix as rocprofv3 names it (10 classes -> MAXC 16)
+KERNEL_SYMBOL = {
+    "search_seg16": "dte::(anonymous namespace)::k_search_rf_seg<16, 0, 16>",
+    "search_seg32": "dte::(anonymous namespace)::k_search_rf_seg<16, 0, 32>",
+    "search_seg64": "dte::(anonymous namespace)::k_search_rf_seg<16, 0, 64>",
+    "search_team64": "dte::(anonymous namespace)::k_search_rf<16, 0, 64>",
+    "search_block": "dte::(anonymous namespace)::k_search_rf<16, 0, 256>",
+}
 
 
-def cpu_baseline(X, y, n_trees_cfg, cores):
+def log(msg):
+    print(f"# {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(X, y, cores):
+    """The oracle fits `cores` trees of the same workload on `cores` threads
+    (trees are independent, so trees/s scales to the full forest)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     t0 = time.perf_counter()
-    pyoracle.fit(X, y, algorithm=0, n_trees=cores, k=11, max_depth=20, min_node_size=5, n_classes=10, seed=11,
-                 n_threads=cores)
+    f = pyoracle.fit(X, y, algorithm=0, n_trees=cores, k=11, max_depth=20, min_node_size=5, n_classes=10,
+                     seed=324123, n_threads=cores)
     dt = time.perf_counter() - t0
-    return {"value": cores / dt, "unit": "trees/s", "cores": cores, "kind": "port",
-            "sample": f"{cores} trees (one per core) of the same 1M x 128 RF fit, {dt:.1f} s wall"}
+    return {"value": round(cores / dt, 4), "unit": "trees/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} of the 200 trees (one per thread) of the same 1M x 128 RF fit, same seed; "
+                      f"{dt:.1f} s wall, {f.n_nodes} nodes; CPU: {cpu_model()}"}
+
+
+def fit_steps(capi, Xd, n_rows, n_features, y, params, steps, warmup, label):
+    for i in range(warmup):
+        t = time.perf_counter()
+        f = capi.fit_device(Xd.ptr, n_rows, n_features, y, params)
+        log(f"{label} warmup {i}: {time.perf_counter() - t:.3f} s, {f.n_nodes} nodes")
+        f.close()
+    capi.profile_reset()
+    capi.profiling(True)
+    times, forest = [], None
+    for i in range(steps):
+        if forest is not None:
+            forest.close()
+        t = time.perf_counter()
+        forest = capi.fit_device(Xd.ptr, n_rows, n_features, y, params)
+        times.append(time.perf_counter() - t)
+        log(f"{label} step {i}: {times[-1]:.3f} s")
+    capi.profiling(False)
+    return times, capi.profile_stats(), forest
+
+
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain copy kernel of known size, for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters (run under rocprofv3 --pmc)."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")  # fill kernel: writes nbytes
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        torch.mul(a, 1.0, out=b)  # elementwise copy kernel: reads nbytes, writes nbytes
+    torch.cuda.synchronize()
+    log(f"copy calibration: {reps} x {nbytes} bytes")
 
 
 def main():
@@ -47,79 +116,108 @@ def main():
     ap.add_argument("--trees", type=int, default=200)
     ap.add_argument("--rows", type=int, default=1_000_000)
     ap.add_argument("--features", type=int, default=128)
+    ap.add_argument("--plain-steps", type=int, default=1, help="timed fits of the plain per-node path (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cores", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true", help="also run a 1 GiB copy kernel")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU tree sharding is design-only "
-                                   "(see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "one GPU only: this build measures the single-GPU share of the workload; "
+                                   "multi-GPU tree sharding is a design note, not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    import ctypes
+
     import numpy as np
+
     from datamininglibs_amd import capi, synth
 
+    if args.copy_calibration:
+        copy_calibration()
     t0 = time.perf_counter()
     X, y = synth.gaussian_classes(seed=0, n=args.rows, f=args.features, n_informative=16, n_classes=10)
-    print(f"# data generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    log(f"data generated in {time.perf_counter() - t0:.1f} s")
     Xd = capi.DeviceArray.from_numpy(X)
-    params = capi.default_params(capi.RANDOM_FOREST, n_trees=args.trees, n_features_per_node=11, max_depth=20,
-                                 min_node_size=5, n_classes=10, seed=324123)
+    common = dict(n_trees=args.trees, n_features_per_node=11, max_depth=20, min_node_size=5, n_classes=10,
+                  seed=324123)
 
-    for i in range(args.warmup):
-        t = time.perf_counter()
-        f = capi.fit_device(Xd.ptr, args.rows, args.features, y, params)
-        print(f"# warmup {i}: {time.perf_counter() - t:.3f} s, {f.n_nodes} nodes", file=sys.stderr, flush=True)
-        f.close()
-
-    capi.profile_reset()
-    capi.profiling(True)
-    times = []
-    forest = None
-    for i in range(args.steps):
-        if forest is not None:
-            forest.close()
-        t = time.perf_counter()
-        forest = capi.fit_device(Xd.ptr, args.rows, args.features, y, params)
-        times.append(time.perf_counter() - t)
-        print(f"# step {i}: {times[-1]:.3f} s", file=sys.stderr, flush=True)
-    capi.profiling(False)
-    stats = capi.profile_stats()
+    times, stats, forest = fit_steps(capi, Xd, args.rows, args.features, y,
+                                     capi.default_params(capi.RANDOM_FOREST, **common), args.steps, args.warmup,
+                                     "batched")
     total = sum(times)
     trees_per_s = args.trees * args.steps / total
 
+    plain = None
+    if args.plain_steps > 0:
+        ptimes, _, pforest = fit_steps(capi, Xd, args.rows, args.features, y,
+                                       capi.default_params(capi.RANDOM_FOREST, build_mode=capi.BUILD_PLAIN,
+                                                           **common), args.plain_steps, 0, "plain")
+        a, b = forest.canonical(), pforest.canonical()
+        identical = all(np.array_equal(a[k], b[k]) for k in a)
+        pforest.close()
+        p_tps = args.trees * args.plain_steps / sum(ptimes)
+        plain = {"trees_per_s": round(p_tps, 3), "ms_per_step": round(1000 * sum(ptimes) / args.plain_steps, 1),
+                 "speedup_batched_vs_plain": round(trees_per_s / p_tps, 2), "identical_forest": identical}
+        if not identical:
+            raise SystemExit("plain and batched forests differ")
+
     # Predict throughput on the training rows (feature-major, resident).
+    lib = capi.lib()
     n_pred = args.rows
     votes = capi.DeviceArray(4 * n_pred * 10)
-    lib = capi.lib()
-    import ctypes
-    for _ in range(2):
+
+    def predict_once():
         capi._check(lib.dte_predict_device(forest.handle, ctypes.c_void_p(Xd.ptr), n_pred, args.rows, 0,
                                            ctypes.c_void_p(votes.ptr), None, None))
+
+    predict_once()
+    capi.profile_reset()
+    capi.profiling(True)
     reps = 3
     t = time.perf_counter()
     for _ in range(reps):
-        capi._check(lib.dte_predict_device(forest.handle, ctypes.c_void_p(Xd.ptr), n_pred, args.rows, 0,
-                                           ctypes.c_void_p(votes.ptr), None, None))
+        predict_once()
     pred_dt = (time.perf_counter() - t) / reps
+    capi.profiling(False)
+    pstats = capi.profile_stats().get("predict", {"total_ms": 0.0, "launches": 1})
     v = votes.to_numpy(np.int32, (n_pred, 10))
     assert (v.sum(1) == args.trees).all()
     acc = float((v.argmax(1) == y.astype(np.int64)).mean())
+    pred_kernel_ms = pstats["total_ms"] / max(1, pstats["launches"])
+    pred_bytes = n_pred * args.features * 4 + forest.n_nodes * 16 + n_pred * 10 * 4
 
-    ss = stats.get("split_search", {"total_ms": 0.0, "bytes": 0.0, "launches": 0})
-    achieved = ss["bytes"] / (ss["total_ms"] * 1e-3) / 1e9 if ss["total_ms"] > 0 else 0.0
+    # Roofline of the dominant split-search kernel.
+    fam = {k: v_ for k, v_ in stats.items() if k.startswith("search_")}
+    dom = max(fam, key=lambda k: fam[k]["total_ms"])
+    ds = fam[dom]
+    avg_ms = ds["total_ms"] / max(1, ds["launches"])
+    bytes_per_launch = ds["bytes"] / max(1, ds["launches"])
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = None
+    traffic_src = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            tj = json.load(fh)
+        if tj.get("family") == dom and tj.get("rows") == args.rows and tj.get("trees_per_fit") == args.trees:
+            traffic = tj["hbm_bytes_per_dispatch"]
+            traffic_src = os.path.relpath(TRAFFIC_FILE, ROOT)
+    ss = stats.get("split_search", {"total_ms": 0.0, "bytes": 0.0})
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_search_rf (split search)", "launches": ss["launches"],
-                "avg_launch_ms": round(ss["total_ms"] / max(1, ss["launches"]), 4),
-                "bytes_per_launch": ss["bytes"] / max(1, ss["launches"])}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": KERNEL_SYMBOL.get(dom, dom), "family": dom, "launches": ds["launches"],
+                "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": round(bytes_per_launch),
+                "traffic_source": traffic_src,
+                "split_search_all_tiers_GBps": round(ss["bytes"] / (ss["total_ms"] * 1e-3) / 1e9, 1)
+                if ss["total_ms"] > 0 else None}
     cpu = None
     if not args.no_cpu_baseline:
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(X, y, args.trees, cores)
+        log(f"cpu baseline on {cores} threads")
+        cpu = cpu_baseline(X, y, cores)
 
     line = {
         "metric": METRIC,
-        "value": round(trees_per_s, 2),
+        "value": round(trees_per_s, 3),
         "unit": "trees/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -128,14 +226,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 inputs, u32 counts, int64 fixed-point + fp64 scores",
+        "dtype": "int64",
         "data": "synthetic",
-        "config": {"workload": "random forest classifier 1,000,000 x 128 fp32, 10 classes, 200 bootstrap trees, "
-                               "entropy, 11 features/node, max depth 20, min node 5",
+        "config": {"workload": "random forest classifier, 1,000,000 x 128 fp32 N(0,1), 16 informative, 10 classes, "
+                               "200 bootstrap trees, entropy, 11 features/node, max depth 20, min node 5",
                    "rows": args.rows, "features": args.features, "trees": args.trees, "parallelism": "1 GPU"},
-        "predict_rows_per_s": round(n_pred / pred_dt, 1),
-        "predict_train_accuracy": round(acc, 4),
-        "kernel_ms": {k: round(v_["total_ms"] / args.steps, 3) for k, v_ in stats.items()},
+        "plain_path": plain,
+        "predict": {"rows_per_s": round(n_pred / pred_dt, 1), "kernel_ms": round(pred_kernel_ms, 3),
+                    "compulsory_GBps": round(pred_bytes / (pred_kernel_ms * 1e-3) / 1e9, 1) if pred_kernel_ms else None,
+                    "train_accuracy": round(acc, 4)},
+        "kernel_ms_per_step": {k: round(v_["total_ms"] / args.steps, 3) for k, v_ in stats.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -2441,25 +2441,14 @@ T read_one(const T* p, hipStream_t s) {
   return v;
 }
 
-struct Timer {
-  hipEvent_t a = nullptr, b = nullptr;
-  const char* name;
-  double bytes;
-  hipStream_t s;
-  Timer(const char* nm, double by, hipStream_t st) : name(nm), bytes(by), s(st) {
-    if (profiling_enabled()) {
-      DTE_HIP(hipEventCreate(&a));
-      DTE_HIP(hipEventCreate(&b));
-      DTE_HIP(hipEventRecord(a, s));
-    }
-  }
-  ~Timer() {
-    if (a) {
-      (void)hipEventRecord(b, s);
-      profile_record(name, a, b, bytes);
-    }
-  }
-};
+
+// Profile family of a split-search tier (one kernel per family).
+const char* search_family(int algorithm, int tier) {
+  static const char* rf[] = {"search_seg16", "search_seg32", "search_seg64", "search_team64", "search_block",
+                             "search_chunked"};
+  static const char* ert[] = {"search_ert_wave", "search_ert_block"};
+  return algorithm == DTE_RANDOM_FOREST ? rf[tier] : ert[tier];
+}
 
 // RF tiers: 0/1/2 = one 16/32/64-lane segment per node (one entry per lane,
 // draws looped in the kernel); 3 = wave teams (four items per workgroup);
@@ -2837,8 +2826,8 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   DevBuf<int32_t> tree_global(T);
   DTE_HIP(hipMemcpyAsync(tree_global.p, tg_h.data(), 4 * T, hipMemcpyHostToDevice, s));
 
-  DevBuf<unsigned long long> search_bytes(1), part_bytes(1);
-  DTE_HIP(hipMemsetAsync(search_bytes.p, 0, 8, s));
+  DevBuf<unsigned long long> search_bytes(kMaxTiers), part_bytes(1);  // search: one counter per tier
+  DTE_HIP(hipMemsetAsync(search_bytes.p, 0, 8 * kMaxTiers, s));
   DTE_HIP(hipMemsetAsync(part_bytes.p, 0, 8, s));
 
   // ---- root entries.
@@ -3059,6 +3048,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       ItemArgs a = it;
       a.list = tier_list[tier].p;
       a.n_groups = cnt[tier];
+      a.bytes = it.bytes ? it.bytes + tier : nullptr;
       const bool rf = algorithm == DTE_RANDOM_FOREST;
       trace_tier[rf ? (tier <= 2 ? 0 : (tier == 3 ? 1 : 2)) : (tier == 0 ? 0 : 2)] += cnt[tier];
       if (rf && tier == 5) {
@@ -3076,6 +3066,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         const int cap = whole ? search_grid_cap : wave_grid_cap;
         grid = (int)std::min<int64_t>((items + per_block - 1) / per_block, cap);
       }
+      Timer tk(search_family(algorithm, tier), 0.0, s);
       search(algorithm, tier, grid, s, d, lv, a);
     }
   };
@@ -3096,6 +3087,8 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       a.list = lst + c0;
       a.n_groups = std::min<int64_t>(kPlainChunk, it.n_groups - c0);
       a.node_major = 1;
+      a.bytes = it.bytes ? it.bytes + tier : nullptr;
+      Timer tk(search_family(algorithm, tier), 0.0, s);
       search(algorithm, tier, (int)a.n_groups, s, d, lv, a);
       DTE_HIP(hipStreamSynchronize(s));
     }
@@ -3340,7 +3333,16 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   }
   const int64_t n_nodes = off_level;
   if (profiling_enabled()) {
-    profile_add_bytes("split_search", (double)read_one(search_bytes.p, s));
+    unsigned long long sb[kMaxTiers];
+    DTE_HIP(hipMemcpyAsync(sb, search_bytes.p, 8 * kMaxTiers, hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipStreamSynchronize(s));
+    double total = 0;
+    for (int tier = 0; tier < kMaxTiers; ++tier) {
+      if (!sb[tier]) continue;
+      profile_add_bytes(search_family(algorithm, tier), (double)sb[tier]);
+      total += (double)sb[tier];
+    }
+    profile_add_bytes("split_search", total);
     profile_add_bytes("partition", (double)read_one(part_bytes.p, s));
   }
 

@@ -75,6 +75,28 @@ void profile_record(const char* name, hipEvent_t a, hipEvent_t b, double bytes);
 void profile_add_bytes(const char* name, double bytes);
 bool profiling_enabled();
 
+// HIP events around the launches of one kernel family on stream s (only
+// while profiling is on); recorded under `name` with `bytes` algorithmic bytes.
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+  const char* name;
+  double bytes;
+  hipStream_t s;
+  Timer(const char* nm, double by, hipStream_t st) : name(nm), bytes(by), s(st) {
+    if (profiling_enabled()) {
+      DTE_HIP(hipEventCreate(&a));
+      DTE_HIP(hipEventCreate(&b));
+      DTE_HIP(hipEventRecord(a, s));
+    }
+  }
+  ~Timer() {
+    if (a) {
+      (void)hipEventRecord(b, s);
+      profile_record(name, a, b, bytes);
+    }
+  }
+};
+
 // Predict-side node record: 16 bytes, one 128-bit load per visited node.
 //   internal: feature >= 0, thr, left child index (right = left + 1)
 //   leaf:     feature = -1, thr = regression leaf value, left = vote class

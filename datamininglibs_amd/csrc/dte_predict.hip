@@ -89,10 +89,12 @@ void predict_forest(const Forest& f, const float* X, int64_t n_rows, int64_t ld,
     while (row_blocks * chunks < 2048 && chunks * 32 < f.n_trees) chunks *= 2;
     const int tpb = (f.n_trees + chunks - 1) / chunks;
     const dim3 grid((unsigned)row_blocks, (unsigned)((f.n_trees + tpb - 1) / tpb));
+    Timer tm("predict", 0.0, s);
     hipLaunchKernelGGL(k_predict_cls, grid, dim3(kBlock), sizeof(uint32_t) * kRowsPerBlock * f.n_classes, s,
                        f.pnodes.p, f.tree_start.p, f.n_trees, X, n_rows, ld, row_major, f.n_classes, tpb, votes);
   } else {
     const int grid = (int)std::min<int64_t>((n_rows + kBlock - 1) / kBlock, 65535);
+    Timer tm("predict", 0.0, s);
     hipLaunchKernelGGL(k_predict_reg, dim3(grid), dim3(kBlock), 0, s, f.pnodes.p, f.tree_start.p, f.n_trees, X,
                        n_rows, ld, row_major, sums);
   }
