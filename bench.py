@@ -108,6 +108,81 @@ def copy_calibration(nbytes=1 << 30, reps=3):
     log(f"copy calibration: {reps} x {nbytes} bytes")
 
 
+def serve_bench(capi, synth, np, n_requests, seed=0):
+    """Four resident forests of 500 trees (depth <= 20, 256 features, 10
+    classes, trained on a 200,000-row set of the same generator), requests
+    of log-uniform 1..65,536 rows of N(0,1) features, Poisson arrivals.
+    Capacity is measured first (all requests submitted at once); then two
+    fixed QPS points at 30% and 70% of it."""
+    import threading
+    t0 = time.perf_counter()
+    X, y = synth.gaussian_classes(seed=5, n=200_000, f=256, n_informative=16, n_classes=10)
+    Xd = capi.DeviceArray.from_numpy(X)
+    forests = []
+    for i in range(4):
+        p = capi.default_params(capi.RANDOM_FOREST, n_trees=500, max_depth=20, min_node_size=5, n_classes=10,
+                                seed=1000 + i)
+        forests.append(capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p))
+    Xd.free()
+    log(f"serving: 4 x 500-tree forests trained in {time.perf_counter() - t0:.1f} s")
+    srv = capi.Server(hbm_budget_bytes=0, max_batch_rows=65536)
+    slots = [srv.load(f) for f in forests]
+    rng = np.random.default_rng(seed)
+    pool = rng.standard_normal((1 << 17, 256), dtype=np.float32)
+    sizes = synth.request_sizes(seed=seed + 1, n_requests=n_requests, max_rows=65536)
+    offs = rng.integers(0, pool.shape[0] - 65536, n_requests)
+    which = rng.integers(0, 4, n_requests)
+    reqs = [(int(which[i]), pool[offs[i]:offs[i] + sizes[i]]) for i in range(n_requests)]
+    total_rows = int(sizes.sum())
+
+    def run(arrivals):
+        done_t = [0.0] * n_requests
+        tickets = [None] * n_requests
+        ready = threading.Semaphore(0)
+
+        def waiter():
+            for i in range(n_requests):
+                ready.acquire()
+                srv.wait(tickets[i])
+                done_t[i] = time.perf_counter()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        arr_abs = [start + a for a in arrivals]
+        for i, (slot, rows) in enumerate(reqs):
+            dt = arr_abs[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets[i] = srv.submit(slots[slot], rows)
+            ready.release()
+        th.join()
+        lat = np.array([done_t[i] - arr_abs[i] for i in range(n_requests)]) * 1e3
+        span = max(done_t) - start
+        return lat, span
+
+    lat, span = run([0.0] * n_requests)
+    cap_rows = total_rows / span
+    cap_qps = n_requests / span
+    points = []
+    for frac in (0.3, 0.7):
+        qps = cap_qps * frac
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, n_requests)).tolist()
+        lat, span = run(arrivals)
+        points.append({"qps": round(qps, 1), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat, 99)), 3), "rows_per_s": round(total_rows / span, 1)})
+        log(f"serving at {qps:.0f} QPS: p50 {points[-1]['p50_ms']} ms, p99 {points[-1]['p99_ms']} ms")
+    st = srv.stats()
+    srv.close()
+    nodes = sum(f.n_nodes for f in forests)
+    for f in forests:
+        f.close()
+    return {"forests": "4 x 500 trees, 256 features, 10 classes, depth <= 20, trained on 200,000 rows",
+            "requests_per_point": n_requests, "rows_per_request": "log-uniform 1..65536",
+            "capacity_rows_per_s": round(cap_rows, 1), "capacity_qps": round(cap_qps, 1), "points": points,
+            "batches": st["batches_launched"], "resident_nodes": nodes, "hbm_in_use_bytes": st["hbm_in_use"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +195,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true", help="also run a 1 GiB copy kernel")
+    ap.add_argument("--serve-requests", type=int, default=1000, help="requests per serving QPS point (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: this build measures the single-GPU share of the workload; "
@@ -209,6 +285,9 @@ def main():
                 "traffic_source": traffic_src,
                 "split_search_all_tiers_GBps": round(ss["bytes"] / (ss["total_ms"] * 1e-3) / 1e9, 1)
                 if ss["total_ms"] > 0 else None}
+    serving = None
+    if args.serve_requests > 0:
+        serving = serve_bench(capi, synth, np, args.serve_requests)
     cpu = None
     if not args.no_cpu_baseline:
         cores = args.cpu_cores or min(16, os.cpu_count() or 1)
@@ -235,6 +314,7 @@ def main():
         "predict": {"rows_per_s": round(n_pred / pred_dt, 1), "kernel_ms": round(pred_kernel_ms, 3),
                     "compulsory_GBps": round(pred_bytes / (pred_kernel_ms * 1e-3) / 1e9, 1) if pred_kernel_ms else None,
                     "train_accuracy": round(acc, 4)},
+        "serving": serving,
         "kernel_ms_per_step": {k: round(v_["total_ms"] / args.steps, 3) for k, v_ in stats.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

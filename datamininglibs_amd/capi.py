@@ -317,3 +317,58 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class Server:
+    """Prediction server: resident forests, continuous batching (dte_server_*)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, max_batch_rows=65536):
+        h = P()
+        _check(lib().dte_server_create(device, hbm_budget_bytes, max_batch_rows, ctypes.byref(h)))
+        self._h = h
+        self._meta = {}
+
+    def load(self, forest):
+        slot = ctypes.c_int32()
+        _check(lib().dte_server_load_forest(self._h, forest.handle, ctypes.byref(slot)))
+        self._meta[slot.value] = (forest.n_features, forest.n_classes if forest.task == CLASSIFICATION else 1)
+        return slot.value
+
+    def submit(self, slot, rows):
+        """rows: (n, F) row-major float32.  Returns a ticket."""
+        rows = np.ascontiguousarray(rows, np.float32)
+        t = ctypes.c_uint64()
+        _check(lib().dte_server_submit(self._h, slot, _ptr(rows), rows.shape[0], ctypes.byref(t)))
+        return t.value, rows.shape[0], slot
+
+    def _out(self, req):
+        _, n, slot = req
+        width = self._meta[slot][1]
+        return np.zeros((n, width) if width > 1 else n, np.float32)
+
+    def wait(self, req):
+        out = self._out(req)
+        _check(lib().dte_server_wait(self._h, req[0], _ptr(out)))
+        return out
+
+    def poll(self, req):
+        out = self._out(req)
+        done = ctypes.c_int32()
+        _check(lib().dte_server_poll(self._h, req[0], _ptr(out), ctypes.byref(done)))
+        return out if done.value else None
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().dte_server_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            _check(lib().dte_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -14,10 +15,6 @@
 #include <vector>
 
 #include "dte_internal.h"
-
-struct dte_forest {
-  dte::Forest f;
-};
 
 namespace dte {
 
@@ -116,6 +113,10 @@ void set_error(int code, const std::string& msg) {
   t_code = code;
   t_err = msg;
 }
+
+int run_guarded(const std::function<void()>& fn) { return guarded(fn); }
+
+void select_device(int device) { set_device(device); }
 
 void fail(int code, const std::string& msg) { throw DteException(code, msg); }
 

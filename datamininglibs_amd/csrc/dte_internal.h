@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,10 @@ struct Error {
   std::string msg;
 };
 void set_error(int code, const std::string& msg);
+// Runs fn, turning exceptions into a status code and the thread's last error.
+int run_guarded(const std::function<void()>& fn);
+// hipSetDevice + one-time library initialisation of that device.
+void select_device(int device);
 [[noreturn]] void fail(int code, const std::string& msg);
 
 #define DTE_HIP(expr)                                                                  \
@@ -149,3 +154,8 @@ const LnTable& host_ln_table();
 constexpr int kBlock = 256;
 
 }  // namespace dte
+
+// Opaque handle of the C ABI.
+struct dte_forest {
+  dte::Forest f;
+};

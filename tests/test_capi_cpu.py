@@ -91,6 +91,9 @@ def test_no_device_is_reported():
     p = capi.default_params(capi.EXTRA_TREES)
     assert _fit_rc(p) == capi.ERR_NO_DEVICE
     assert "device" in _last_error()
+    s = ctypes.c_void_p()
+    assert capi.lib().dte_server_create(0, 1 << 30, 65536, ctypes.byref(s)) == capi.ERR_NO_DEVICE
+    assert capi.lib().dte_server_create(0, 1 << 30, 0, ctypes.byref(s)) == capi.ERR_INVALID_ARGUMENT
 
 
 def test_error_state_is_per_call():
