@@ -48,7 +48,8 @@ class Params(ctypes.Structure):
         ("device_id", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
         ("build_mode", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("tree_offset", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 6),
     ]
 
 

@@ -1273,7 +1273,6 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   return v;
 }
 
-constexpr int kWaveItemMax = 64;  // RF wave path: one entry per lane
 
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_search_rf_wave(DataArgs d, LevelArgs lv, ItemArgs it) {
@@ -3007,7 +3006,6 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   SearchFn search = pick_search(task, C);
   BigFn big = pick_big(task, C);
   BigState bigst;
-  const bool use_big_pipeline = std::getenv("DTE_NO_BIG_PIPELINE") == nullptr;
   PartFn partition = pick_part(task, C);
   PartWaveFn partition_wave = pick_part_wave(task, C);
 
@@ -3481,10 +3479,11 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
                    hipStream_t s) {
   FitData fd;
   prepare_data(fd, X_dev_in, y_host, N, F, p, s);
-  if (p.build_mode != DTE_BUILD_PLAIN) return fit_batch(fd, 0, p.n_trees, N, F, p, s, false);
+  if (p.build_mode != DTE_BUILD_PLAIN) return fit_batch(fd, p.tree_offset, p.n_trees, N, F, p, s, false);
   std::vector<std::unique_ptr<Forest>> parts;
   for (int t0 = 0; t0 < p.n_trees; t0 += kPlainTrees)
-    parts.emplace_back(fit_batch(fd, t0, std::min(kPlainTrees, p.n_trees - t0), N, F, p, s, true));
+    parts.emplace_back(
+        fit_batch(fd, p.tree_offset + t0, std::min(kPlainTrees, p.n_trees - t0), N, F, p, s, true));
   if (parts.size() == 1) return parts[0].release();
   return merge_forests(parts, s);
 }

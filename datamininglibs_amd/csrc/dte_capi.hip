@@ -95,7 +95,9 @@ void check_params(const dte_params* p) {
     fail(DTE_ERR_INVALID_ARGUMENT, "n_classes must be in [1, 32]");
   if (p->build_mode != DTE_BUILD_BATCHED && p->build_mode != DTE_BUILD_PLAIN)
     fail(DTE_ERR_INVALID_ARGUMENT, "unknown build_mode");
-  for (int i = 0; i < 7; ++i)
+  if (p->tree_offset < 0 || (int64_t)p->tree_offset + p->n_trees > 0x7FFFFFFF)
+    fail(DTE_ERR_INVALID_ARGUMENT, "tree_offset out of range");
+  for (int i = 0; i < 6; ++i)
     if (p->reserved[i] != 0) fail(DTE_ERR_INVALID_ARGUMENT, "reserved fields must be zero");
 }
 

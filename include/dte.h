@@ -72,7 +72,11 @@ typedef struct dte_params {
   int32_t device_id;           /* kDevId */
   uint64_t seed;               /* RNG key; defaults 324123 (RF) / 123512 (ERT) as gpurf.cu:55, gpuert.cu:18 */
   int32_t build_mode;          /* DTE_BUILD_BATCHED (default) | DTE_BUILD_PLAIN */
-  int32_t reserved[7];         /* must be zero */
+  int32_t tree_offset;         /* index of the first tree: the tree counter of every random draw.  Parts
+                                  built with offsets 0, n0, n0+n1, ... concatenate to the forest of one
+                                  build (what SplitParameterPack + AggregateModels need, gpu_algorithm.cc:
+                                  17-31; the reference restarts its RNG per batch instead, D7) */
+  int32_t reserved[6];         /* must be zero */
 } dte_params;
 
 /* One node of the exported model: gpuDTE_NodeHeader_Classify<float>

@@ -11,6 +11,7 @@ for p in (ROOT, os.path.join(ROOT, "oracle")):
 
 ORACLE_LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 DTE_LIB = os.path.join(ROOT, "datamininglibs_amd", "libdte.so")
+HOST_TEST = os.path.join(ROOT, "tests", "cpp", "test_host_api")
 
 
 def pytest_configure(config):
@@ -29,4 +30,5 @@ def _built_libraries():
     __graft_entry__.build() and only rebuilt here when absent."""
     _make(os.path.join(ROOT, "oracle"), ORACLE_LIB)
     _make(os.path.join(ROOT, "datamininglibs_amd"), DTE_LIB)
+    _make(os.path.join(ROOT, "datamininglibs_amd"), HOST_TEST)
     yield

@@ -66,7 +66,7 @@ def _last_error():
 
 @pytest.mark.parametrize("field,value", [("algorithm", 7), ("task", 3), ("n_trees", 0), ("max_depth", -1),
                                          ("min_node_size", -2), ("n_classes", 0), ("n_classes", 33),
-                                         ("build_mode", 9), ("n_features_per_node", -1)])
+                                         ("build_mode", 9), ("n_features_per_node", -1), ("tree_offset", -1)])
 def test_invalid_params_rejected_before_device(field, value):
     p = capi.default_params(capi.RANDOM_FOREST)
     setattr(p, field, value)
