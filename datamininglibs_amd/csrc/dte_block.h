@@ -131,6 +131,41 @@ __device__ __forceinline__ void team_excl_scan_vec(uint32_t (&v)[N], uint32_t* s
   }
 }
 
+// Exclusive scan of N int64 lanes at once, with totals; s_w holds
+// N * kWaves words (unused for a wave team).
+template <int TEAM, int N>
+__device__ __forceinline__ void team_excl_scan_i64_vec(int64_t (&v)[N], int64_t* s_w, int64_t (&tot)[N]) {
+  int64_t inc[N];
+#pragma unroll
+  for (int c = 0; c < N; ++c) inc[c] = wave_incl_scan_i64(v[c]);
+  if constexpr (TEAM == kWave) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      tot[c] = __shfl(inc[c], kWave - 1, kWave);
+      v[c] = inc[c] - v[c];
+    }
+  } else {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int c = 0; c < N; ++c)
+      if (lane == kWave - 1) s_w[c * kWaves + w] = inc[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      int64_t pre = 0, t = 0;
+#pragma unroll
+      for (int i = 0; i < kWaves; ++i) {
+        const int64_t x = s_w[c * kWaves + i];
+        pre += (i < w) ? x : 0;
+        t += x;
+      }
+      tot[c] = t;
+      v[c] = pre + inc[c] - v[c];
+    }
+    __syncthreads();
+  }
+}
+
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, kWave));
@@ -226,14 +261,22 @@ __device__ __forceinline__ BestCand team_best(BestCand b, double* s_g, uint32_t*
   }
 }
 
+// Positions per lane when n elements are spread over the whole team.
+template <int TEAM>
+__device__ __forceinline__ int lane_span(int n) {
+  return (n + TEAM - 1) / TEAM;
+}
+
 // One stable LSD pass over n <= TEAM*IPT (key, val) pairs held in LDS, on the
-// 4-bit digit at `shift`.  Lane t owns positions [t*IPT, t*IPT+IPT).
+// 4-bit digit at `shift`.  Lane t owns positions [t*ipt, t*ipt+ipt) with
+// ipt = ceil(n / TEAM), so small inputs keep every lane busy.
 // s_cnt: 16 * TEAM uint16 counters.  Reads (kin, vin), writes (kout, vout).
 template <int IPT, int TEAM>
 __device__ __forceinline__ void lds_sort_pass(const uint32_t* kin, const uint16_t* vin, uint32_t* kout,
                                               uint16_t* vout, int n, int shift, uint16_t* s_cnt, uint32_t* s_w) {
   const int t = team_tid<TEAM>();
-  const int lo = t * IPT;
+  const int ipt = lane_span<TEAM>(n);
+  const int lo = t * ipt;
   uint32_t k[IPT];
   uint16_t v[IPT];
   uint32_t d[IPT];
@@ -242,7 +285,7 @@ __device__ __forceinline__ void lds_sort_pass(const uint32_t* kin, const uint16_
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const int p = lo + i;
-    if (p < n) {
+    if (i < ipt && p < n) {
       k[i] = kin[p];
       v[i] = vin[p];
       d[i] = (k[i] >> shift) & 15u;
@@ -270,11 +313,52 @@ __device__ __forceinline__ void lds_sort_pass(const uint32_t* kin, const uint16_
 #pragma unroll
   for (int i = 0; i < IPT; ++i) {
     const int p = lo + i;
-    if (p < n) {
+    if (i < ipt && p < n) {
       const uint32_t pos = s_cnt[d[i] * TEAM + t];
       s_cnt[d[i] * TEAM + t] = (uint16_t)(pos + 1);
       kout[pos] = k[i];
       vout[pos] = v[i];
+    }
+  }
+  team_sync<TEAM>();
+}
+
+// Counting sort of n <= TEAM*IPT (key, val) pairs whose keys lie in
+// [0, span), span <= 2 * 16 * TEAM / 2 words of `cnt` (the radix counters
+// of lds_sort_pass, reused as span u32 counters).  Equal keys come out in
+// any order: callers only use the order of distinct keys.
+template <int IPT, int TEAM>
+__device__ __forceinline__ void lds_count_sort(const uint32_t* kin, const uint16_t* vin, uint32_t* kout,
+                                               uint16_t* vout, int n, int span, uint32_t* cnt, uint32_t* s_w) {
+  const int t = team_tid<TEAM>();
+  for (int i = t; i < span; i += TEAM) cnt[i] = 0;
+  team_sync<TEAM>();
+  uint32_t r[IPT];
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const int e = t + q * TEAM;
+    if (e < n) r[q] = atomicAdd(&cnt[kin[e]], 1u);
+  }
+  team_sync<TEAM>();
+  const int per = (span + TEAM - 1) / TEAM;
+  const int lo = t * per, hi = min(lo + per, span);
+  uint32_t loc = 0;
+  for (int i = lo; i < hi; ++i) loc += cnt[i];
+  uint32_t base = team_excl_scan<TEAM>(loc, s_w, nullptr);
+  for (int i = lo; i < hi; ++i) {
+    const uint32_t c = cnt[i];
+    cnt[i] = base;
+    base += c;
+  }
+  team_sync<TEAM>();
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const int e = t + q * TEAM;
+    if (e < n) {
+      const uint32_t k = kin[e];
+      const uint32_t pos = cnt[k] + r[q];
+      kout[pos] = k;
+      vout[pos] = vin[e];
     }
   }
   team_sync<TEAM>();

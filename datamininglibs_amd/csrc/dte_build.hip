@@ -71,7 +71,8 @@ struct DataArgs {
   int min_node;
   int max_depth;
   int igF;           // fixed-point shift of the x ln x terms (per fit)
-  const int64_t* T;  // T[x] = xlnx_fixed(x, igF) for x < kTT
+  const int64_t* T;  // T[x] = xlnx_fixed(x, igF) for x <= Tmax (classification)
+  int64_t Tmax;
 };
 
 struct LevelArgs {
@@ -108,18 +109,24 @@ struct ItemArgs {
 };
 
 // ------------------------------------------------- per-node context -----
-// Fixed-point x ln x with the fit-wide shift F: table lookup for small
-// counts (LDS copy of T[x] = xlnx_fixed(x, F) for x < kTT), computed
-// otherwise -- the same bits either way.
-constexpr int kTT = 1024;
+// Fixed-point x ln x with the fit-wide shift F.  Counts never exceed the
+// root's total weight, so a per-fit table T[x] = xlnx_fixed(x, F) for
+// x <= Tmax (built by k_build_T with the same function) answers every
+// lookup with one cached load; counts above a capped table fall back to the
+// computation -- the same bits either way.
+__device__ __attribute__((noinline)) int64_t xlnx_fixed_slow(uint64_t x, int F, const double* inv,
+                                                           const double* logc) {
+  return xlnx_fixed(x, F, inv, logc);
+}
 
 struct LnCtx {
   const double* inv;
   const double* logc;
   const int64_t* T;
+  int64_t Tmax;
   int F;
   __device__ __forceinline__ int64_t t(uint64_t x) const {
-    return x < (uint64_t)kTT ? T[x] : xlnx_fixed(x, F, inv, logc);
+    return x <= (uint64_t)Tmax ? T[x] : xlnx_fixed_slow(x, F, inv, logc);
   }
 };
 
@@ -161,8 +168,9 @@ struct TeamLds {
   uint32_t ncls[MAXC];
   uint32_t ctot[MAXC];
   uint32_t last[TEAM];
-  uint32_t w[4 * kWaves + kWaves];
+  uint32_t w[8 * kWaves + kWaves];
   int64_t w64[kWaves];
+  int64_t w64v[4 * kWaves];
   double bg[kWaves];
   uint32_t bk[kWaves];
   uint32_t bp[kWaves];
@@ -174,13 +182,14 @@ struct TeamLds {
 template <int MAXC, int TASK, int TEAM, int IPT>
 __device__ void scan_class_columns(TeamLds<MAXC, TASK, TEAM, IPT>& L, int C) {
   const int t = team_tid<TEAM>();
-  for (int c0 = 0; c0 < C; c0 += 4) {
-    uint32_t v[4], tot[4];
+  constexpr int G = MAXC < 8 ? MAXC : 8;
+  for (int c0 = 0; c0 < C; c0 += G) {
+    uint32_t v[G], tot[G];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = (c0 + q < C) ? L.pre[(c0 + q) * TEAM + t] : 0u;
-    team_excl_scan_vec<TEAM, 4>(v, L.w, tot);
+    for (int q = 0; q < G; ++q) v[q] = (c0 + q < C) ? L.pre[(c0 + q) * TEAM + t] : 0u;
+    team_excl_scan_vec<TEAM, G>(v, L.w, tot);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < G; ++q)
       if (c0 + q < C) {
         L.pre[(c0 + q) * TEAM + t] = v[q] + L.cr.run[c0 + q];
         if (t == 0) L.ctot[c0 + q] = tot[q];
@@ -242,8 +251,9 @@ template <int MAXC, int TASK, int TEAM, int IPT>
 __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, int n, uint32_t kbase,
                                   const NodeCtx& nc, TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx) {
   const int t = team_tid<TEAM>();
-  const int lo = t * IPT;
-  const int hi = min(lo + IPT, n);
+  const int ipt = lane_span<TEAM>(n);
+  const int lo = t * ipt;
+  const int hi = min(lo + ipt, n);
   BestCand tb{0.0, kNone, kNone};
   if constexpr (TASK == 0) {
     for (int c = 0; c < nc.C; ++c) L.pre[c * TEAM + t] = 0;
@@ -252,18 +262,62 @@ __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, in
       L.pre[(a >> 16) * TEAM + t] += a & 0xFFFFu;
     }
     scan_class_columns<MAXC, TASK, TEAM, IPT>(L, nc.C);
+    // Three passes so that the x ln x lookups of all positions are
+    // independent: (1) counts before each position, (2) the lane's deltas of
+    // the left / right sums (plus one exclusive scan that gives every lane
+    // its starting sums), (3) running sums and the gain at each boundary.
+    uint32_t lb[IPT], tc[IPT], wv[IPT];
+    int64_t nlb[IPT];
     if (lo < hi) {
-      WalkState w = walk_init<MAXC, TASK, TEAM, IPT>(L, nc, lx);
-      uint32_t prevk = (lo > 0) ? keys[lo - 1] + kbase : L.cr.last_key;
-      for (int p = lo; p < hi; ++p) {
-        const uint32_t k = keys[p] + kbase;
-        if (w.nl > 0 && prevk < k) {
-          const double g = walk_gain(w, nc, lx);
-          if (g > tb.g) tb = BestCand{g, k, prevk};
+      int64_t nl = 0;
+      for (int c = 0; c < nc.C; ++c) nl += L.pre[c * TEAM + t];
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        const int p = lo + i;
+        if (p < hi) {
+          const uint32_t a = L.aux[vals[p]];
+          const int c = (int)(a >> 16);
+          const uint32_t wt = a & 0xFFFFu;
+          lb[i] = L.pre[c * TEAM + t];
+          L.pre[c * TEAM + t] = lb[i] + wt;
+          tc[i] = L.ncls[c];
+          wv[i] = wt;
+          nlb[i] = nl;
+          nl += wt;
         }
-        const uint32_t a = L.aux[vals[p]];
-        walk_add<MAXC, TASK, TEAM, IPT>(w, L, (int)(a >> 16), a & 0xFFFFu, lx);
-        prevk = k;
+      }
+    }
+    int64_t sc[4] = {0, 0, 0, 0};  // lane deltas of sl, sr; sums at the chunk start
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      if (lo + i < hi) {
+        sc[0] += lx.t(lb[i] + wv[i]) - lx.t(lb[i]);
+        sc[1] += lx.t(tc[i] - lb[i] - wv[i]) - lx.t(tc[i] - lb[i]);
+      }
+    }
+    if (t < nc.C) {  // one class per lane
+      sc[2] = lx.t(L.cr.run[t]);
+      sc[3] = lx.t(L.ncls[t] - L.cr.run[t]);
+    }
+    int64_t tot[4];
+    team_excl_scan_i64_vec<TEAM, 4>(sc, L.w64v, tot);
+    if (lo < hi) {
+      int64_t sl = tot[2] + sc[0], sr = tot[3] + sc[1];
+      uint32_t prevk = (lo > 0) ? keys[lo - 1] + kbase : L.cr.last_key;
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        const int p = lo + i;
+        if (p < hi) {
+          const uint32_t k = keys[p] + kbase;
+          if (nlb[i] > 0 && prevk < k) {
+            const int64_t q = (sl - lx.t((uint64_t)nlb[i])) + (sr - lx.t((uint64_t)(nc.n - nlb[i])));
+            const double g = ig_from_fixed(nc.P + q, lx.F, nc.inv);
+            if (g > tb.g) tb = BestCand{g, k, prevk};
+          }
+          sl += lx.t(lb[i] + wv[i]) - lx.t(lb[i]);
+          sr += lx.t(tc[i] - lb[i] - wv[i]) - lx.t(tc[i] - lb[i]);
+          prevk = k;
+        }
       }
     }
     const BestCand b = team_best<TEAM>(tb, L.bg, L.bk, L.bp);
@@ -441,6 +495,13 @@ __device__ void eval_hist_bucket(const uint32_t* skey, const uint32_t* saux, con
 template <int MAXC, int TASK, int TEAM, int IPT>
 __device__ void sort_and_eval(int n, int bits, uint32_t kbase, const NodeCtx& nc,
                               TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx) {
+  constexpr int kSpan = 8 * TEAM;  // u32 counters in the radix counter area
+  if (bits > 4 && (1 << bits) <= kSpan) {
+    lds_count_sort<IPT, TEAM>(L.u.srt.k[0], L.u.srt.v[0], L.u.srt.k[1], L.u.srt.v[1], n, 1 << bits,
+                              reinterpret_cast<uint32_t*>(L.u.srt.cnt), L.w);
+    eval_sorted_chunk<MAXC, TASK, TEAM, IPT>(L.u.srt.k[1], L.u.srt.v[1], n, kbase, nc, L, lx);
+    return;
+  }
   int cur = 0;
   for (int sh = 0; sh < bits; sh += 4) {
     lds_sort_pass<IPT, TEAM>(L.u.srt.k[cur], L.u.srt.v[cur], L.u.srt.k[cur ^ 1], L.u.srt.v[cur ^ 1], n, sh,
@@ -452,13 +513,11 @@ __device__ void sort_and_eval(int n, int bits, uint32_t kbase, const NodeCtx& nc
 
 __device__ __forceinline__ int bit_length(uint32_t v) { return v ? 32 - __clz((int)v) : 0; }
 
-__device__ __forceinline__ void load_tables(double* s_inv, double* s_logc, int64_t* s_T, const int64_t* T) {
+__device__ __forceinline__ void load_tables(double* s_inv, double* s_logc) {
   for (int i = threadIdx.x; i < 128; i += blockDim.x) {
     s_inv[i] = g_ln_table.inv[i];
     s_logc[i] = g_ln_table.logc[i];
   }
-  if (s_T)
-    for (int i = threadIdx.x; i < kTT; i += blockDim.x) s_T[i] = T[i];
 }
 
 // Node statistics of one item, computed by every lane of the team.
@@ -673,12 +732,11 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
   constexpr int IPT = kIpt;
   constexpr int NT = kBlock / TEAM;
   __shared__ double s_inv[128], s_logc[128];
-  __shared__ int64_t s_T[kTT];
   __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[NT];
   __shared__ uint32_t s_boff[TEAM == 256 ? kMaxBuckets + 1 : 1];
-  load_tables(s_inv, s_logc, TASK == 0 ? s_T : nullptr, d.T);
+  load_tables(s_inv, s_logc);
   __syncthreads();
-  const LnCtx lx{s_inv, s_logc, s_T, d.igF};
+  const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
   const int team = TEAM == 256 ? 0 : (int)(threadIdx.x / kWave);
   TeamLds<MAXC, TASK, TEAM, IPT>& L = s_team[team];
   const int t = team_tid<TEAM>();
@@ -715,383 +773,663 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
 }
 
 // ------------------------------------- chunk-parallel big-node search ---
-// Nodes above the wave-team capacity are split into key chunks that are
-// evaluated independently, so every chunk of every big item of a level runs
-// in parallel:
-//   1. k_big_hist    per (item, tile): bucket histogram of the dense keys
-//   2. k_big_plan    per item: bucket offsets, greedy packing of consecutive
-//                    buckets into chunks of <= 512 entries (an overfull
-//                    bucket -- heavy ties -- becomes a histogram chunk)
-//   3. k_big_scatter per (item, tile): entries into bucket order
-//   4. k_big_ctot    per chunk: class totals / target sums, last key
-//   5. k_big_carry   per item: running totals before each chunk
-//   6. k_big_eval    per chunk: LDS sort + scored walk from the carry
-//   7. k_big_reduce  per item: best chunk -> (score, threshold)
-struct BigItem {
+// Items (node, feature draw) above the workgroup's LDS capacity.  The keys of
+// an item (dense ranks of the drawn feature) are split into nb coarse buckets
+// of consecutive keys, bucket = key >> shift, sized so that a bucket holds
+// about kV2Target entries: one workgroup-sized chunk.  Every bucket of every
+// item of a level is then evaluated in parallel:
+//   1. k_v2_hist     per (item, tile): bucket counts, max key and class
+//                    totals (regression: weight and target sums)
+//   2. k_v2_scan     per item: bucket offsets, carries (totals and last key
+//                    before each bucket), task lists
+//   3. k_v2_scatter  per (item, tile): entries into bucket order, staged in
+//                    LDS so that each bucket's run is written contiguously
+//   4. k_v2_eval     per bucket: LDS radix sort + scored walk from the carry
+//   5. k_v2_spill    per overfull bucket (ties, clustered keys): fine
+//                    sub-buckets walked in order by one workgroup
+//   6. k_v2_reduce   per item: best bucket -> (score, threshold)
+// Entry order inside a bucket does not matter: the walk sorts by key and
+// scores only boundaries between distinct keys from integer prefix sums.
+constexpr int kV2Tile = 8192;       // entries per histogram tile
+constexpr int kV2MaxB = 1024;       // buckets per item
+constexpr int kV2Target = 1024;     // entries per bucket aimed at
+constexpr int kV2ClsWords = 16384;  // LDS words for per-bucket class totals
+
+struct V2Item {
   int32_t f;
   int32_t m;
-  int64_t start;  // entry start of the node
-  int32_t wbits;  // bucket width 2^wbits keys
+  int64_t start;  // first entry of the node
+  int64_t soff;   // first scratch slot of the item
+  int32_t boff;   // first bucket slot of the item
   int32_t nb;     // buckets
-  int64_t boff;   // offset in the bucket arrays
-  int64_t soff;   // offset in the scratch element arrays
-  int64_t ridx;   // result slot
+  int32_t shift;  // bucket = key >> shift
   uint32_t node;  // level node index
-  uint32_t cbeg;  // first chunk (set by k_big_plan)
-  uint32_t ncnk;  // chunks (set by k_big_plan)
-  uint32_t pad;
+  int64_t ridx;   // result slot (open position * D + draw)
+  // node context (as node_context computes it)
+  int64_t n_w;    // sum of weights
+  int64_t P;      // classification: T(n) - sum_c T(n_c)
+  int64_t ssum;   // regression: fixed-point target sum
+  double inv;     // 1 / (n ln 2)
 };
 
-struct BigTile {
+struct V2Tile {
   uint32_t item;
   uint32_t e0;
 };
 
-struct BigChunk {
+struct V2Task {
   uint32_t item;
-  uint32_t e0;     // first element (relative to the item's scratch base)
-  uint32_t n;      // elements
-  uint32_t kbase;  // first key of the chunk's first bucket
-  uint32_t bits;   // key span bits
-  uint32_t hist;   // 1: single overfull bucket, evaluated by histogram
+  uint32_t b;
 };
 
-struct BigArgs {
-  BigItem* items;
+struct V2Args {
+  const V2Item* items;
   int64_t n_items;
-  const BigTile* tiles;
-  uint32_t* bcount;  // per bucket: count, then offset, then cursor
+  const V2Tile* tiles;
+  // per bucket
+  uint32_t* bcount;
+  uint32_t* bmax;
+  uint32_t* bcls;   // [bucket * C + c] (classification)
+  uint32_t* bwn;    // weight sum (regression)
+  int64_t* bsum;    // target sum (regression)
+  uint32_t* bbase;  // scratch offset within the item
   uint32_t* bcur;
-  BigChunk* chunks;
-  unsigned int* chunk_ctr;
-  uint32_t* skey;
-  uint32_t* saux;
-  int64_t* sy;
-  // per chunk
-  uint32_t* ctot_cls;  // [chunk * C]
-  int64_t* ctot_n;
-  int64_t* ctot_s;
-  uint32_t* clast;
   uint32_t* carry_cls;
   int64_t* carry_n;
   int64_t* carry_s;
   uint32_t* carry_last;
-  double* cb_g;
-  uint32_t* cb_key;
-  uint32_t* cb_prev;
+  double* res_g;
+  uint32_t* res_key;
+  uint32_t* res_prev;
+  // scratch, item-major, bucket order
+  uint32_t* skey;
+  uint32_t* saux;
+  int64_t* sy;
+  uint32_t* skey2;  // overfull buckets: fine sub-bucket order
+  uint32_t* saux2;
+  int64_t* sy2;
+  // task lists
+  V2Task* tasks;
+  V2Task* spill;
+  unsigned int* counters;  // [0] tasks, [1] spill
 };
 
-constexpr int kBigTile = 4096;
-constexpr int kBigCap = 64 * kIpt;  // chunk capacity of a wave team (512)
-
-template <int TASK>
-__global__ void __launch_bounds__(kBlock) k_big_hist(DataArgs d, LevelArgs lv, BigArgs ba) {
-  __shared__ uint32_t s_h[kMaxBuckets];
-  const BigTile tl = ba.tiles[blockIdx.x];
-  const BigItem it = ba.items[tl.item];
-  const int t = threadIdx.x;
-  for (int b = t; b < it.nb; b += kBlock) s_h[b] = 0;
-  __syncthreads();
-  const uint32_t* Kf = d.K + (int64_t)it.f * d.N;
-  constexpr int U = kBigTile / kBlock;
-  uint32_t rows[U];
+// Exclusive max-scan over a 256-thread workgroup; s_w holds kWaves words.
+__device__ __forceinline__ uint32_t block_excl_max(uint32_t v, uint32_t* s_w) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  uint32_t inc = v;
 #pragma unroll
-  for (int q = 0; q < U; ++q) {
-    const int e = (int)tl.e0 + t + q * kBlock;
-    rows[q] = e < it.m ? lv.row[it.start + e] : 0u;
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off, kWave);
+    if (lane >= off) inc = max(inc, y);
   }
-#pragma unroll
-  for (int q = 0; q < U; ++q) {
-    const int e = (int)tl.e0 + t + q * kBlock;
-    if (e < it.m) atomicAdd(&s_h[Kf[rows[q]] >> it.wbits], 1u);
-  }
+  uint32_t exc = __shfl_up(inc, 1, kWave);
+  if (lane == 0) exc = 0;
+  if (lane == kWave - 1) s_w[w] = inc;
   __syncthreads();
-  for (int b = t; b < it.nb; b += kBlock)
-    if (s_h[b]) atomicAdd(&ba.bcount[it.boff + b], s_h[b]);
+  uint32_t pre = 0;
+#pragma unroll
+  for (int i = 0; i < kWaves; ++i) pre = (i < w) ? max(pre, s_w[i]) : pre;
+  __syncthreads();
+  return max(pre, exc);
 }
 
-// One workgroup per item: exclusive scan of the bucket counts, cursor copy,
-// then thread 0 packs buckets into chunks (contiguous in the chunk list).
-__global__ void __launch_bounds__(kBlock) k_big_plan(BigArgs ba) {
-  __shared__ uint32_t s_c[kMaxBuckets + 1];
-  __shared__ uint32_t s_w[kWaves];
-  BigItem& it = ba.items[blockIdx.x];
+template <int TASK>
+__global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2Args a) {
+  extern __shared__ uint32_t s_dyn[];  // classification: nb * C class totals
+  __shared__ uint32_t s_cnt[kV2MaxB];
+  __shared__ uint32_t s_max[kV2MaxB];
+  __shared__ uint32_t s_wn[TASK == 1 ? kV2MaxB : 1];
+  __shared__ unsigned long long s_sum[TASK == 1 ? kV2MaxB : 1];
+  const V2Tile tl = a.tiles[blockIdx.x];
+  const V2Item it = a.items[tl.item];
   const int t = threadIdx.x;
-  const int nb = it.nb;
-  for (int b = t; b < nb; b += kBlock) s_c[b] = ba.bcount[it.boff + b];
-  __syncthreads();
-  const int per = (nb + kBlock - 1) / kBlock;
-  const int b0 = t * per, b1 = min(b0 + per, nb);
-  uint32_t loc = 0;
-  for (int b = b0; b < b1; ++b) loc += s_c[b];
-  uint32_t base = team_excl_scan<256>(loc, s_w, nullptr);
-  __syncthreads();
-  for (int b = b0; b < b1; ++b) {
-    const uint32_t c = s_c[b];
-    s_c[b] = base;
-    base += c;
+  const int C = d.C;
+  for (int b = t; b < it.nb; b += kBlock) {
+    s_cnt[b] = 0;
+    s_max[b] = 0;
+    if (TASK == 1) {
+      s_wn[b] = 0;
+      s_sum[b] = 0;
+    }
   }
-  if (t == 0) s_c[nb] = (uint32_t)it.m;
+  if (TASK == 0)
+    for (int i = t; i < it.nb * C; i += kBlock) s_dyn[i] = 0;
   __syncthreads();
-  for (int b = t; b < nb; b += kBlock) ba.bcur[it.boff + b] = s_c[b];
-  if (t == 0) {
-    // count chunks, reserve, write
-    uint32_t n = 0;
-    for (int pass = 0; pass < 2; ++pass) {
-      uint32_t cbeg = 0;
-      if (pass == 1) cbeg = atomicAdd(ba.chunk_ctr, n);
-      uint32_t k = 0;
-      int b = 0;
-      while (b < nb) {
-        const uint32_t cb = s_c[b + 1] - s_c[b];
-        if (cb == 0) {
-          ++b;
-          continue;
-        }
-        int be = b;
-        uint32_t tot = 0;
-        uint32_t hist = 0;
-        if (cb > (uint32_t)kBigCap) {
-          be = b + 1;
-          tot = cb;
-          hist = 1;
+  const uint32_t* Kf = d.K + (int64_t)it.f * d.N;
+  const int n = min(kV2Tile, it.m - (int)tl.e0);
+  const int64_t e0 = it.start + tl.e0;
+  constexpr int U = 8;
+  for (int base = 0; base < n; base += kBlock * U) {
+    uint32_t rows[U], auxv[U];
+    int64_t yv[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int e = base + t + q * kBlock;
+      if (e < n) {
+        rows[q] = lv.row[e0 + e];
+        auxv[q] = lv.aux[e0 + e];
+        if (TASK == 1) yv[q] = lv.ey[e0 + e];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int e = base + t + q * kBlock;
+      if (e < n) {
+        const uint32_t key = Kf[rows[q]];
+        const uint32_t b = key >> it.shift;
+        const uint32_t w = auxv[q] & 0xFFFFu;
+        atomicAdd(&s_cnt[b], 1u);
+        atomicMax(&s_max[b], key);
+        if (TASK == 0) {
+          atomicAdd(&s_dyn[b * C + (auxv[q] >> 16)], w);
         } else {
-          while (be < nb) {
-            const uint32_t c = s_c[be + 1] - s_c[be];
-            if (c > (uint32_t)kBigCap || tot + c > (uint32_t)kBigCap) break;
-            tot += c;
-            ++be;
-          }
+          atomicAdd(&s_wn[b], w);
+          atomicAdd(&s_sum[b], (unsigned long long)((int64_t)w * yv[q]));
         }
-        if (pass == 1) {
-          BigChunk ch;
-          ch.item = blockIdx.x;
-          ch.e0 = s_c[b];
-          ch.n = tot;
-          ch.kbase = (uint32_t)b << it.wbits;
-          const uint32_t span = (uint32_t)(be - b) << it.wbits;
-          ch.bits = (uint32_t)bit_length(span - 1u);
-          ch.hist = hist;
-          ba.chunks[cbeg + k] = ch;
-        }
-        ++k;
-        b = be;
       }
-      if (pass == 0) n = k;
-      else {
-        it.cbeg = cbeg;
-        it.ncnk = k;
-      }
-    }
-  }
-}
-
-template <int TASK>
-__global__ void __launch_bounds__(kBlock) k_big_scatter(DataArgs d, LevelArgs lv, BigArgs ba) {
-  __shared__ uint32_t s_h[kMaxBuckets];
-  __shared__ uint32_t s_base[kMaxBuckets];
-  const BigTile tl = ba.tiles[blockIdx.x];
-  const BigItem it = ba.items[tl.item];
-  const int t = threadIdx.x;
-  for (int b = t; b < it.nb; b += kBlock) s_h[b] = 0;
-  __syncthreads();
-  const uint32_t* Kf = d.K + (int64_t)it.f * d.N;
-  constexpr int U = kBigTile / kBlock;
-  uint32_t keys[U], auxv[U];
-  int64_t yv[U];
-#pragma unroll
-  for (int q = 0; q < U; ++q) {
-    const int e = (int)tl.e0 + t + q * kBlock;
-    keys[q] = e < it.m ? lv.row[it.start + e] : 0u;
-    if (e < it.m) {
-      auxv[q] = lv.aux[it.start + e];
-      if (TASK == 1) yv[q] = lv.ey[it.start + e];
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < U; ++q) {
-    const int e = (int)tl.e0 + t + q * kBlock;
-    if (e < it.m) {
-      keys[q] = Kf[keys[q]];
-      atomicAdd(&s_h[keys[q] >> it.wbits], 1u);
     }
   }
   __syncthreads();
   for (int b = t; b < it.nb; b += kBlock) {
-    const uint32_t c = s_h[b];
-    if (c) s_base[b] = atomicAdd(&ba.bcur[it.boff + b], c);
-    s_h[b] = 0;
+    const uint32_t c = s_cnt[b];
+    if (!c) continue;
+    const int64_t gb = (int64_t)it.boff + b;
+    atomicAdd(&a.bcount[gb], c);
+    atomicMax(&a.bmax[gb], s_max[b]);
+    if (TASK == 0) {
+      for (int q = 0; q < C; ++q) {
+        const uint32_t v = s_dyn[b * C + q];
+        if (v) atomicAdd(&a.bcls[gb * C + q], v);
+      }
+    } else {
+      atomicAdd(&a.bwn[gb], s_wn[b]);
+      atomicAdd((unsigned long long*)&a.bsum[gb], s_sum[b]);
+    }
+  }
+}
+
+// One workgroup per item.
+template <int TASK>
+__global__ void __launch_bounds__(kBlock) k_v2_scan(DataArgs d, V2Args a) {
+  __shared__ uint32_t s_w[4 * kWaves + kWaves];
+  __shared__ int64_t s_w64[kWaves];
+  const V2Item it = a.items[blockIdx.x];
+  const int t = threadIdx.x;
+  const int C = d.C;
+  constexpr int PER = kV2MaxB / kBlock;  // buckets per thread
+  const int b0 = t * PER;
+  const int64_t gb0 = (int64_t)it.boff + b0;
+  uint32_t cnt[PER], mx[PER];
+  uint32_t lcnt = 0, lmax = 0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const bool ok = b0 + q < it.nb;
+    cnt[q] = ok ? a.bcount[gb0 + q] : 0u;
+    mx[q] = ok ? a.bmax[gb0 + q] : 0u;
+    lcnt += cnt[q];
+    if (cnt[q]) lmax = max(lmax, mx[q]);
+  }
+  // offsets
+  uint32_t off = team_excl_scan<256>(lcnt, s_w, nullptr);
+  // last key before each bucket
+  uint32_t last = block_excl_max(lmax, s_w);
+  // weights / targets
+  int64_t run_n;
+  int64_t run_s = 0;
+  if (TASK == 0) {
+    int64_t lw = 0;
+    for (int q = 0; q < PER; ++q)
+      if (b0 + q < it.nb)
+        for (int c = 0; c < C; ++c) lw += a.bcls[(gb0 + q) * C + c];
+    run_n = team_excl_scan_i64<256>(lw, s_w64, nullptr);
+  } else {
+    int64_t lw = 0, ls = 0;
+    for (int q = 0; q < PER; ++q)
+      if (b0 + q < it.nb) {
+        lw += a.bwn[gb0 + q];
+        ls += a.bsum[gb0 + q];
+      }
+    run_n = team_excl_scan_i64<256>(lw, s_w64, nullptr);
+    run_s = team_excl_scan_i64<256>(ls, s_w64, nullptr);
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    if (b0 + q >= it.nb) break;
+    const int64_t gb = gb0 + q;
+    a.bbase[gb] = off;
+    a.bcur[gb] = off;
+    a.carry_n[gb] = run_n;
+    if (TASK == 1) a.carry_s[gb] = run_s;
+    a.carry_last[gb] = last;
+    off += cnt[q];
+    if (cnt[q]) last = max(last, mx[q]);
+    if (TASK == 0) {
+      int64_t w = 0;
+      for (int c = 0; c < C; ++c) w += a.bcls[gb * C + c];
+      run_n += w;
+    } else {
+      run_n += a.bwn[gb];
+      run_s += a.bsum[gb];
+    }
+  }
+  if (TASK == 0) {
+    // class carries, four classes per scan
+    for (int c0 = 0; c0 < C; c0 += 4) {
+      uint32_t v[4], tot[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        v[k] = 0;
+        if (c0 + k < C)
+          for (int q = 0; q < PER; ++q)
+            if (b0 + q < it.nb) v[k] += a.bcls[(gb0 + q) * C + c0 + k];
+      }
+      team_excl_scan_vec<256, 4>(v, s_w, tot);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (c0 + k >= C) continue;
+        uint32_t r = v[k];
+        for (int q = 0; q < PER; ++q) {
+          if (b0 + q >= it.nb) break;
+          const int64_t gb = gb0 + q;
+          a.carry_cls[gb * C + c0 + k] = r;
+          r += a.bcls[gb * C + c0 + k];
+        }
+      }
+    }
+  }
+  // task lists: non-empty buckets, overfull ones apart
+  uint32_t n_norm = 0, n_spill = 0;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    if (!cnt[q]) continue;
+    if (cnt[q] > (uint32_t)kCap)
+      ++n_spill;
+    else
+      ++n_norm;
+  }
+  uint32_t tot_norm = 0, tot_spill = 0;
+  const uint32_t pn = team_excl_scan<256>(n_norm, s_w, &tot_norm);
+  const uint32_t ps = team_excl_scan<256>(n_spill, s_w, &tot_spill);
+  __shared__ uint32_t s_base[2];
+  if (t == 0) {
+    s_base[0] = tot_norm ? atomicAdd(&a.counters[0], tot_norm) : 0u;
+    s_base[1] = tot_spill ? atomicAdd(&a.counters[1], tot_spill) : 0u;
+  }
+  __syncthreads();
+  uint32_t in = s_base[0] + pn, is = s_base[1] + ps;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    if (!cnt[q]) continue;
+    const V2Task tk{(uint32_t)blockIdx.x, (uint32_t)(b0 + q)};
+    if (cnt[q] > (uint32_t)kCap)
+      a.spill[is++] = tk;
+    else
+      a.tasks[in++] = tk;
+  }
+}
+
+// Entries of one tile into bucket order.  Each entry takes a rank inside its
+// bucket from an LDS counter, the tile reserves one run per bucket in the
+// item's scratch, and the entries are staged in LDS in (bucket, rank) order
+// so that consecutive threads write consecutive slots of a run.
+template <int TASK>
+__global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv, V2Args a) {
+  constexpr int TILE = TASK == 0 ? 4096 : 2048;
+  constexpr int U = TILE / kBlock;
+  __shared__ uint32_t s_cnt[kV2MaxB];
+  __shared__ uint32_t s_gpos[kV2MaxB];
+  __shared__ uint32_t s_key[TILE];
+  __shared__ uint32_t s_aux[TILE];
+  __shared__ uint16_t s_b[TILE];
+  __shared__ int64_t s_y[TASK == 1 ? TILE : 1];
+  __shared__ uint32_t s_w[kWaves];
+  const V2Tile tl = a.tiles[blockIdx.x];
+  const V2Item it = a.items[tl.item];
+  const int t = threadIdx.x;
+  for (int b = t; b < it.nb; b += kBlock) s_cnt[b] = 0;
+  __syncthreads();
+  const uint32_t* Kf = d.K + (int64_t)it.f * d.N;
+  const int n = min(TILE, it.m - (int)tl.e0);
+  const int64_t e0 = it.start + tl.e0;
+  uint32_t keys[U], auxv[U], rank[U];
+  int64_t yv[U];
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int e = t + q * kBlock;
+    if (e < n) {
+      keys[q] = lv.row[e0 + e];
+      auxv[q] = lv.aux[e0 + e];
+      if (TASK == 1) yv[q] = lv.ey[e0 + e];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int e = t + q * kBlock;
+    if (e < n) {
+      keys[q] = Kf[keys[q]];
+      rank[q] = atomicAdd(&s_cnt[keys[q] >> it.shift], 1u);
+    }
+  }
+  __syncthreads();
+  {
+    constexpr int PER = kV2MaxB / kBlock;
+    const int b0 = t * PER;
+    uint32_t c[PER], loc = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      c[q] = b0 + q < it.nb ? s_cnt[b0 + q] : 0u;
+      loc += c[q];
+    }
+    uint32_t off = team_excl_scan<256>(loc, s_w, nullptr);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      if (b0 + q >= it.nb) break;
+      s_cnt[b0 + q] = off;  // local offset of the bucket's run
+      s_gpos[b0 + q] = c[q] ? atomicAdd(&a.bcur[(int64_t)it.boff + b0 + q], c[q]) : 0u;
+      off += c[q];
+    }
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < U; ++q) {
-    const int e = (int)tl.e0 + t + q * kBlock;
-    if (e < it.m) {
-      const uint32_t b = keys[q] >> it.wbits;
-      const int64_t pos = it.soff + s_base[b] + atomicAdd(&s_h[b], 1u);
-      ba.skey[pos] = keys[q];
-      ba.saux[pos] = auxv[q];
-      if (TASK == 1) ba.sy[pos] = yv[q];
+    const int e = t + q * kBlock;
+    if (e < n) {
+      const uint32_t b = keys[q] >> it.shift;
+      const uint32_t li = s_cnt[b] + rank[q];
+      s_key[li] = keys[q];
+      s_aux[li] = auxv[q];
+      s_b[li] = (uint16_t)b;
+      if (TASK == 1) s_y[li] = yv[q];
     }
   }
-}
-
-// One wave per chunk: class totals (or count / target sum) and last key.
-template <int MAXC, int TASK>
-__global__ void __launch_bounds__(kBlock) k_big_ctot(DataArgs d, BigArgs ba, int64_t n_chunks) {
-  const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t c = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave; c < n_chunks;
-       c += (int64_t)gridDim.x * kWaves) {
-    const BigChunk ch = ba.chunks[c];
-    const BigItem& it = ba.items[ch.item];
-    const int64_t e0 = it.soff + ch.e0;
-    uint32_t loc[MAXC];
-#pragma unroll
-    for (int q = 0; q < MAXC; ++q) loc[q] = 0;
-    int64_t ln = 0, ls = 0;
-    uint32_t kmax = 0;
-    for (uint32_t e = lane; e < ch.n; e += kWave) {
-      const uint32_t a = ba.saux[e0 + e];
-      const uint32_t w = a & 0xFFFFu;
-      kmax = max(kmax, ba.skey[e0 + e]);
-      ln += w;
-      if (TASK == 0) {
-        const int lab = (int)(a >> 16);
-#pragma unroll
-        for (int q = 0; q < MAXC; ++q) loc[q] += (q == lab) ? w : 0u;
-      } else {
-        ls += (int64_t)w * ba.sy[e0 + e];
-      }
-    }
-    kmax = wave_max_u32(kmax);
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-      ln += __shfl_xor(ln, off, kWave);
-      ls += __shfl_xor(ls, off, kWave);
-    }
-    if (TASK == 0) {
-#pragma unroll
-      for (int q = 0; q < MAXC; ++q) {
-        uint32_t v = loc[q];
-#pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-        if (lane == q && q < d.C) ba.ctot_cls[c * d.C + q] = v;
-      }
-    }
-    if (lane == 0) {
-      ba.ctot_n[c] = ln;
-      if (TASK == 1) ba.ctot_s[c] = ls;
-      ba.clast[c] = kmax;
-    }
-  }
-}
-
-// One thread per item: totals of all earlier chunks of the item.
-__global__ void k_big_carry(DataArgs d, BigArgs ba) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ba.n_items;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    const BigItem it = ba.items[k];
-    uint32_t run[32];
-    for (int c = 0; c < d.C; ++c) run[c] = 0;
-    int64_t rn = 0, rs = 0;
-    uint32_t last = 0;
-    for (uint32_t q = 0; q < it.ncnk; ++q) {
-      const int64_t c = it.cbeg + q;
-      for (int j = 0; j < d.C; ++j) {
-        ba.carry_cls[c * d.C + j] = run[j];
-        run[j] += ba.ctot_cls[c * d.C + j];
-      }
-      ba.carry_n[c] = rn;
-      if (ba.carry_s) ba.carry_s[c] = rs;
-      ba.carry_last[c] = last;
-      rn += ba.ctot_n[c];
-      rs += ba.ctot_s ? ba.ctot_s[c] : 0;
-      last = ba.clast[c];
-    }
-  }
-}
-
-// Chunk evaluation.  TEAM = 64: sorted chunks (<= 512 entries), four per
-// workgroup.  TEAM = 256: histogram chunks (one overfull bucket).
-template <int MAXC, int TASK, int TEAM>
-__global__ void __launch_bounds__(kBlock) k_big_eval(DataArgs d, LevelArgs lv, BigArgs ba, const uint32_t* list,
-                                                    int64_t n_list) {
-  constexpr int IPT = kIpt;
-  constexpr int NT = kBlock / TEAM;
-  __shared__ double s_inv[128], s_logc[128];
-  __shared__ int64_t s_T[kTT];
-  __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[NT];
-  load_tables(s_inv, s_logc, TASK == 0 ? s_T : nullptr, d.T);
   __syncthreads();
-  const LnCtx lx{s_inv, s_logc, s_T, d.igF};
-  const int team = TEAM == 256 ? 0 : (int)(threadIdx.x / kWave);
-  TeamLds<MAXC, TASK, TEAM, IPT>& L = s_team[team];
-  const int t = team_tid<TEAM>();
-  for (int64_t q = (int64_t)blockIdx.x * NT + team; q < n_list; q += (int64_t)gridDim.x * NT) {
-    const uint32_t c = list[q];
-    const BigChunk ch = ba.chunks[c];
-    const BigItem it = ba.items[ch.item];
+  for (int i = t; i < n; i += kBlock) {
+    const uint32_t b = s_b[i];
+    const int64_t pos = it.soff + s_gpos[b] + (uint32_t)i - s_cnt[b];
+    a.skey[pos] = s_key[i];
+    a.saux[pos] = s_aux[i];
+    if (TASK == 1) a.sy[pos] = s_y[i];
+  }
+}
+
+template <int MAXC, int TASK, int IPT>
+__device__ __forceinline__ void v2_carry(const DataArgs& d, const V2Args& a, int64_t gb,
+                                         TeamLds<MAXC, TASK, 256, IPT>& L) {
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < MAXC; ++c) L.cr.run[c] = (TASK == 0 && c < d.C) ? a.carry_cls[gb * d.C + c] : 0u;
+    L.cr.run_n = a.carry_n[gb];
+    L.cr.run_s = TASK == 1 ? a.carry_s[gb] : 0;
+    L.cr.last_key = a.carry_last[gb];
+    L.cr.best_g = 0.0;
+    L.cr.best_key = kNone;
+    L.cr.best_prev = kNone;
+  }
+}
+
+// Node context of an item from its precomputed fields; class totals to LDS
+// (visible after the caller's next barrier).
+template <int MAXC, int TASK, int IPT>
+__device__ __forceinline__ NodeCtx v2_node(const DataArgs& d, const LevelArgs& lv, const V2Item& it,
+                                           TeamLds<MAXC, TASK, 256, IPT>& L) {
+  NodeCtx nc;
+  nc.C = d.C;
+  nc.n = it.n_w;
+  nc.P = it.P;
+  nc.ssum = it.ssum;
+  nc.inv = it.inv;
+  if (TASK == 0 && (int)threadIdx.x < d.C) L.ncls[threadIdx.x] = lv.cls[(int64_t)it.node * d.C + threadIdx.x];
+  return nc;
+}
+
+// One workgroup per bucket of at most kCap entries, grid-stride over the
+// task list.  The loads of the next bucket (task, item and bucket fields,
+// carries, entries) are issued before the current bucket is sorted and
+// walked, so their latency hides behind that work.
+template <int TASK, int IPT>
+struct V2Pre {
+  int64_t gb, e0, ridx_unused;
+  uint32_t node, kbase, bmax, shift;
+  int n;
+  int64_t n_w, P, ssum;
+  double inv;
+  uint32_t carry_c;  // class carry of class threadIdx.x (lanes < C)
+  int64_t carry_n, carry_s;
+  uint32_t carry_last;
+  uint32_t ncls;     // node class total of class threadIdx.x
+  uint32_t key[IPT], aux[IPT];
+  int64_t y[TASK == 1 ? IPT : 1];
+};
+
+template <int MAXC, int TASK, int IPT>
+__device__ __forceinline__ void v2_prefetch(const DataArgs& d, const LevelArgs& lv, const V2Args& a, int64_t q,
+                                            V2Pre<TASK, IPT>& P) {
+  const int t = threadIdx.x;
+  const V2Task tk = a.tasks[q];
+  const V2Item& it = a.items[tk.item];
+  P.gb = (int64_t)it.boff + tk.b;
+  P.shift = (uint32_t)it.shift;
+  P.kbase = tk.b << it.shift;
+  P.node = it.node;
+  P.n_w = it.n_w;
+  P.P = it.P;
+  P.ssum = it.ssum;
+  P.inv = it.inv;
+  P.n = (int)a.bcount[P.gb];
+  P.e0 = it.soff + a.bbase[P.gb];
+  P.bmax = a.bmax[P.gb];
+  P.carry_n = a.carry_n[P.gb];
+  P.carry_s = TASK == 1 ? a.carry_s[P.gb] : 0;
+  P.carry_last = a.carry_last[P.gb];
+  P.carry_c = (TASK == 0 && t < d.C) ? a.carry_cls[P.gb * d.C + t] : 0u;
+  P.ncls = (TASK == 0 && t < d.C) ? lv.cls[(int64_t)P.node * d.C + t] : 0u;
+#pragma unroll
+  for (int k = 0; k < IPT; ++k) {
+    const int e = t + k * 256;
+    if (e < P.n) {
+      P.key[k] = a.skey[P.e0 + e];
+      P.aux[k] = a.saux[P.e0 + e];
+      if (TASK == 1) P.y[k] = a.sy[P.e0 + e];
+    }
+  }
+}
+
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_v2_eval(DataArgs d, LevelArgs lv, V2Args a, int64_t n_tasks) {
+  constexpr int IPT = kIpt;
+  __shared__ double s_inv[128], s_logc[128];
+  __shared__ TeamLds<MAXC, TASK, 256, IPT> L;
+  load_tables(s_inv, s_logc);
+  __syncthreads();
+  const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
+  const int t = threadIdx.x;
+  int64_t q = blockIdx.x;
+  if (q >= n_tasks) return;
+  V2Pre<TASK, IPT> cur;
+  v2_prefetch<MAXC, TASK, IPT>(d, lv, a, q, cur);
+  while (q < n_tasks) {
+    // stage the current bucket
     NodeCtx nc;
-    (void)node_context<MAXC, TASK, TEAM, IPT>(d, lv, it.node, nc, L, lx);
+    nc.C = d.C;
+    nc.n = cur.n_w;
+    nc.P = cur.P;
+    nc.ssum = cur.ssum;
+    nc.inv = cur.inv;
+    if (TASK == 0 && t < d.C) {
+      L.ncls[t] = cur.ncls;
+      L.cr.run[t] = cur.carry_c;
+    }
     if (t == 0) {
-      for (int j = 0; j < MAXC; ++j) L.cr.run[j] = (TASK == 0 && j < d.C) ? ba.carry_cls[(int64_t)c * d.C + j] : 0u;
-      L.cr.run_n = ba.carry_n[c];
-      L.cr.run_s = ba.carry_s ? ba.carry_s[c] : 0;
-      L.cr.last_key = ba.carry_last[c];
+      for (int c = d.C; c < MAXC; ++c) L.cr.run[c] = 0u;
+      if (TASK == 1)
+        for (int c = 0; c < MAXC; ++c) L.cr.run[c] = 0u;
+      L.cr.run_n = cur.carry_n;
+      L.cr.run_s = cur.carry_s;
+      L.cr.last_key = cur.carry_last;
       L.cr.best_g = 0.0;
       L.cr.best_key = kNone;
       L.cr.best_prev = kNone;
     }
-    const int64_t e0 = it.soff + ch.e0;
-    if constexpr (TEAM == 256) {
-      team_sync<TEAM>();
-      eval_hist_bucket<MAXC, TASK, IPT>(ba.skey + e0, ba.saux + e0, TASK == 1 ? ba.sy + e0 : nullptr, ch.n,
-                                        ch.kbase, 1 << it.wbits, nc, L, lx);
-    } else {
+    const int n = cur.n;
 #pragma unroll
-      for (int k = 0; k < IPT; ++k) {
-        const int e = t + k * TEAM;
-        if (e < (int)ch.n) {
-          L.u.srt.k[0][e] = ba.skey[e0 + e] - ch.kbase;
-          L.u.srt.v[0][e] = (uint16_t)e;
-          L.aux[e] = ba.saux[e0 + e];
-          if (TASK == 1) L.y[e] = ba.sy[e0 + e];
-        }
+    for (int k = 0; k < IPT; ++k) {
+      const int e = t + k * 256;
+      if (e < n) {
+        L.u.srt.k[0][e] = cur.key[k] - cur.kbase;
+        L.u.srt.v[0][e] = (uint16_t)e;
+        L.aux[e] = cur.aux[k];
+        if (TASK == 1) L.y[e] = cur.y[k];
       }
-      team_sync<TEAM>();
-      sort_and_eval<MAXC, TASK, TEAM, IPT>((int)ch.n, (int)ch.bits, ch.kbase, nc, L, lx);
     }
-    team_sync<TEAM>();
+    const int64_t gb = cur.gb;
+    const uint32_t kbase = cur.kbase;
+    const int bits = bit_length(cur.bmax - kbase);
+    __syncthreads();
+    // issue the next bucket's loads
+    const int64_t qn = q + gridDim.x;
+    if (qn < n_tasks) v2_prefetch<MAXC, TASK, IPT>(d, lv, a, qn, cur);
+    sort_and_eval<MAXC, TASK, 256, IPT>(n, bits, kbase, nc, L, lx);
+    __syncthreads();
     if (t == 0) {
-      ba.cb_g[c] = L.cr.best_g;
-      ba.cb_key[c] = L.cr.best_key;
-      ba.cb_prev[c] = L.cr.best_prev;
+      a.res_g[gb] = L.cr.best_g;
+      a.res_key[gb] = L.cr.best_key;
+      a.res_prev[gb] = L.cr.best_prev;
     }
-    team_sync<TEAM>();
+    __syncthreads();
+    q = qn;
   }
 }
 
-template <int TASK>
-__global__ void k_big_reduce(DataArgs d, BigArgs ba, ItemArgs it, int f_unused) {
-  (void)f_unused;
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < ba.n_items;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    const BigItem bi = ba.items[k];
-    double g = 0.0;
-    uint32_t key = kNone, prev = kNone;
-    for (uint32_t q = 0; q < bi.ncnk; ++q) {
-      const int64_t c = bi.cbeg + q;
-      if (ba.cb_key[c] != kNone && cand_better(ba.cb_g[c], ba.cb_key[c], g, key)) {
-        g = ba.cb_g[c];
-        key = ba.cb_key[c];
-        prev = ba.cb_prev[c];
+// One workgroup per overfull bucket: fine sub-buckets of the bucket's key
+// range, packed into LDS-sized chunks or read through an LDS histogram
+// (a single sub-bucket above the capacity: heavy ties), walked in order.
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_v2_spill(DataArgs d, LevelArgs lv, V2Args a, int64_t n_spill) {
+  constexpr int IPT = kIpt;
+  constexpr int CAP = kCap;
+  __shared__ double s_inv[128], s_logc[128];
+  __shared__ TeamLds<MAXC, TASK, 256, IPT> L;
+  __shared__ uint32_t s_boff[kMaxBuckets + 1];
+  load_tables(s_inv, s_logc);
+  __syncthreads();
+  const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
+  const int t = threadIdx.x;
+  for (int64_t q = blockIdx.x; q < n_spill; q += gridDim.x) {
+    const V2Task tk = a.spill[q];
+    const V2Item it = a.items[tk.item];
+    const int64_t gb = (int64_t)it.boff + tk.b;
+    const int n = (int)a.bcount[gb];
+    const int64_t e0 = it.soff + a.bbase[gb];
+    const uint32_t klo = tk.b << it.shift;
+    const uint32_t span = a.bmax[gb] - klo + 1u;
+    const NodeCtx nc = v2_node(d, lv, it, L);
+    v2_carry<MAXC, TASK, IPT>(d, a, gb, L);
+    __syncthreads();
+    const int words = (TASK == 0) ? d.C : 3;
+    int wbits = 0;
+    while ((1 << (wbits + 1)) * words <= kHistWords) ++wbits;
+    while (((span + (1u << wbits) - 1) >> wbits) > (uint32_t)kMaxBuckets) ++wbits;
+    const int W = 1 << wbits;
+    const int NB = (int)((span + (uint32_t)W - 1) >> wbits);
+    const uint32_t* skey = a.skey + e0;
+    const uint32_t* saux = a.saux + e0;
+    const int64_t* sy = TASK == 1 ? a.sy + e0 : nullptr;
+    uint32_t* dkey = a.skey2 + e0;
+    uint32_t* daux = a.saux2 + e0;
+    int64_t* dy = TASK == 1 ? a.sy2 + e0 : nullptr;
+    for (int b = t; b <= NB; b += 256) s_boff[b] = 0;
+    __syncthreads();
+    for (int e = t; e < n; e += 256) atomicAdd(&s_boff[(skey[e] - klo) >> wbits], 1u);
+    __syncthreads();
+    {
+      const int per = (NB + 255) / 256;
+      const int b0 = t * per, b1 = min(b0 + per, NB);
+      uint32_t loc = 0;
+      for (int b = b0; b < b1; ++b) loc += s_boff[b];
+      uint32_t base = team_excl_scan<256>(loc, L.w, nullptr);
+      for (int b = b0; b < b1; ++b) {
+        const uint32_t c = s_boff[b];
+        s_boff[b] = base;
+        base += c;
       }
+      if (t == 0) s_boff[NB] = (uint32_t)n;
+      __syncthreads();
+      for (int b = t; b < NB; b += 256) L.u.bpos[b] = s_boff[b];
+      __syncthreads();
     }
+    for (int e = t; e < n; e += 256) {
+      const uint32_t k = skey[e];
+      const uint32_t pos = atomicAdd(&L.u.bpos[(k - klo) >> wbits], 1u);
+      dkey[pos] = k;
+      daux[pos] = saux[e];
+      if (TASK == 1) dy[pos] = sy[e];
+    }
+    __syncthreads();
+    int b = 0;
+    while (b < NB) {
+      const uint32_t cb = s_boff[b + 1] - s_boff[b];
+      if (cb == 0) {
+        ++b;
+        continue;
+      }
+      if (cb > (uint32_t)CAP) {
+        eval_hist_bucket<MAXC, TASK, IPT>(dkey + s_boff[b], daux + s_boff[b], TASK == 1 ? dy + s_boff[b] : nullptr,
+                                          cb, klo + ((uint32_t)b << wbits), W, nc, L, lx);
+        ++b;
+        continue;
+      }
+      int be = b;
+      uint32_t tot = 0;
+      while (be < NB) {
+        const uint32_t c = s_boff[be + 1] - s_boff[be];
+        if (c > (uint32_t)CAP || tot + c > (uint32_t)CAP) break;
+        tot += c;
+        ++be;
+      }
+      const uint32_t c0 = s_boff[b];
+      const uint32_t kbase = klo + ((uint32_t)b << wbits);
+      for (int e = t; e < (int)tot; e += 256) {
+        L.u.srt.k[0][e] = dkey[c0 + e] - kbase;
+        L.u.srt.v[0][e] = (uint16_t)e;
+        L.aux[e] = daux[c0 + e];
+        if (TASK == 1) L.y[e] = dy[c0 + e];
+      }
+      __syncthreads();
+      sort_and_eval<MAXC, TASK, 256, IPT>((int)tot, bit_length((uint32_t)(be - b) * (uint32_t)W - 1u), kbase, nc, L,
+                                          lx);
+      b = be;
+    }
+    __syncthreads();
+    if (t == 0) {
+      a.res_g[gb] = L.cr.best_g;
+      a.res_key[gb] = L.cr.best_key;
+      a.res_prev[gb] = L.cr.best_prev;
+    }
+    __syncthreads();
+  }
+}
+
+// One wave per item: the best bucket (ties to the lower key).
+template <int TASK>
+__global__ void __launch_bounds__(kBlock) k_v2_reduce(DataArgs d, V2Args a, ItemArgs it) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int64_t k = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
+  if (k >= a.n_items) return;
+  const V2Item bi = a.items[k];
+  BestCand b{0.0, kNone, kNone};
+  for (int q = lane; q < bi.nb; q += kWave) {
+    const int64_t gb = (int64_t)bi.boff + q;
+    if (!a.bcount[gb]) continue;
+    const uint32_t key = a.res_key[gb];
+    if (key != kNone && cand_better(a.res_g[gb], key, b.g, b.key)) b = BestCand{a.res_g[gb], key, a.res_prev[gb]};
+  }
+  b = wave_best(b);
+  if (lane == 0) {
     float score = 0.0f, thr = 0.0f;
-    if (key != kNone) {
+    if (b.key != kNone) {
       const float* sv = d.SV + d.SVoff[bi.f];
-      score = (float)(TASK == 0 ? g : ldexp_d(g, -2 * d.shift));
-      thr = midpoint(sv[prev], sv[key]);
+      score = (float)(TASK == 0 ? b.g : ldexp_d(b.g, -2 * d.shift));
+      thr = midpoint(sv[b.prev], sv[b.key]);
     }
     it.res_score[bi.ridx] = score;
     it.res_thr[bi.ridx] = thr;
@@ -1099,33 +1437,46 @@ __global__ void k_big_reduce(DataArgs d, BigArgs ba, ItemArgs it, int f_unused) 
   }
 }
 
-// Fill the item table of a batch of big groups on the device.
-__global__ void k_big_items(DataArgs d, LevelArgs lv, const uint32_t* groups, int64_t g0, int64_t n_groups,
-                            int per_group, int j0, int D, BigItem* items) {
+// Item table of a phase: one item per (big group, draw).
+__global__ void k_v2_items(DataArgs d, LevelArgs lv, const uint32_t* groups, int64_t n_groups, int per_group, int j0,
+                           int D, int nb_cap, V2Item* items) {
   const int64_t n = n_groups * per_group;
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t o = groups[g0 + k / per_group];
+    const uint32_t o = groups[k / per_group];
     const int j = j0 + (int)(k % per_group);
     const uint32_t i = lv.open_idx[o];
-    BigItem b;
+    V2Item b;
     b.f = (int)(rand_u32(d.seed, kStreamFeat, (uint32_t)lv.tree_global[lv.tree[i]], lv.id[i], (uint64_t)j) %
                 (uint32_t)d.F);
     b.m = lv.m[i];
     b.start = lv.start[i];
+    // nb: a power of two near m / kV2Target, then the shift that maps the
+    // feature's dense keys [0, S) onto at most nb buckets.
+    int nb = 1;
+    while (nb < nb_cap && nb * kV2Target < b.m) nb <<= 1;
     const uint32_t S = d.S[b.f];
-    const int words = d.C > 0 ? d.C : 3;
-    int wbits = 0;
-    while ((1 << (wbits + 1)) * words <= kHistWords) ++wbits;
-    while (((S + (1u << wbits) - 1) >> wbits) > (uint32_t)kMaxBuckets) ++wbits;
-    b.wbits = wbits;
-    b.nb = (int)((S + (1u << wbits) - 1) >> wbits);
-    b.boff = 0;
+    const int kb = bit_length(S > 1 ? S - 1u : 1u);
+    int lg = 0;
+    while ((1 << lg) < nb) ++lg;
+    b.shift = max(0, kb - lg);
+    b.nb = (int)(((S > 0 ? S - 1u : 0u) >> b.shift) + 1u);
     b.soff = 0;
-    b.ridx = (int64_t)o * D + j;
+    b.boff = 0;
     b.node = i;
-    b.cbeg = 0;
-    b.ncnk = 0;
-    b.pad = 0;
+    b.ridx = (int64_t)o * D + j;
+    b.n_w = lv.count[i];
+    b.P = 0;
+    b.ssum = 0;
+    b.inv = 0.0;
+    if (d.C > 0) {
+      const LnCtx lx{g_ln_table.inv, g_ln_table.logc, d.T, d.Tmax, d.igF};
+      int64_t tsum = 0;
+      for (int c = 0; c < d.C; ++c) tsum += lx.t(lv.cls[(int64_t)i * d.C + c]);
+      b.P = lx.t((uint64_t)b.n_w) - tsum;
+      b.inv = 1.0 / ((double)b.n_w * kLn2);
+    } else {
+      b.ssum = lv.sum[i];
+    }
     items[k] = b;
   }
 }
@@ -1277,10 +1628,9 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_search_rf_wave(DataArgs d, LevelArgs lv, ItemArgs it) {
   __shared__ double s_inv[128], s_logc[128];
-  __shared__ int64_t s_T[kTT];
-  load_tables(s_inv, s_logc, TASK == 0 ? s_T : nullptr, d.T);
+  load_tables(s_inv, s_logc);
   __syncthreads();
-  const LnCtx lx{s_inv, s_logc, s_T, d.igF};
+  const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
   const int lane = threadIdx.x & (kWave - 1);
   const int wv = threadIdx.x / kWave;
   unsigned long long my_bytes = 0;
@@ -1436,10 +1786,9 @@ template <int MAXC, int TASK, int SEG>
 __global__ void __launch_bounds__(kBlock) k_search_rf_seg(DataArgs d, LevelArgs lv, ItemArgs it) {
   constexpr int NPW = kWave / SEG;  // nodes per wave
   __shared__ double s_inv[128], s_logc[128];
-  __shared__ int64_t s_T[kTT];
-  load_tables(s_inv, s_logc, TASK == 0 ? s_T : nullptr, d.T);
+  load_tables(s_inv, s_logc);
   __syncthreads();
-  const LnCtx lx{s_inv, s_logc, s_T, d.igF};
+  const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
   const int sl = threadIdx.x & (SEG - 1);
   const int seg = (threadIdx.x & (kWave - 1)) / SEG;
   unsigned long long my_bytes = 0;
@@ -2483,145 +2832,151 @@ void launch_partition_wave(int grid, hipStream_t s, const DataArgs& d, const Lev
   hipLaunchKernelGGL((k_partition_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, pa, list, n);
 }
 
-__global__ void k_chunk_split(const BigChunk* chunks, int64_t n, uint32_t* sorted_list, uint32_t* hist_list,
-                              unsigned int* counts) {
-  const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t base = (blockIdx.x * (int64_t)blockDim.x) & ~(int64_t)(kWave - 1); base < n;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = base + (threadIdx.x & ~(kWave - 1)) + lane;
-    int k = -1;
-    if (c < n) k = chunks[c].hist ? 1 : 0;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const uint64_t mask = __ballot(k == q);
-      if (!mask) continue;
-      const int leader = __ffsll((unsigned long long)mask) - 1;
-      unsigned int bp = 0;
-      if (lane == leader) bp = atomicAdd(&counts[q], (unsigned int)__popcll(mask));
-      bp = __shfl(bp, leader, kWave);
-      if (k == q) (q == 0 ? sorted_list : hist_list)[bp + __popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)c;
-    }
-  }
-}
 
 // Scratch and bookkeeping of the big-node pipeline, reused across levels.
 struct BigState {
-  DevBuf<BigItem> items;
-  DevBuf<BigTile> tiles;
-  DevBuf<uint32_t> bcount, bcur;
-  DevBuf<BigChunk> chunks;
+  DevBuf<V2Item> items;
+  DevBuf<V2Tile> htiles, stiles;
+  DevBuf<uint32_t> bcount, bmax, bcls, bwn, bbase, bcur, carry_cls, carry_last, res_key, res_prev;
+  DevBuf<int64_t> bsum, carry_n, carry_s;
+  DevBuf<double> res_g;
+  DevBuf<uint32_t> skey, saux, skey2, saux2;
+  DevBuf<int64_t> sy, sy2;
+  DevBuf<V2Task> tasks, spill;
   DevBuf<unsigned int> ctr;
-  DevBuf<uint32_t> skey, saux;
-  DevBuf<int64_t> sy;
-  DevBuf<uint32_t> ctot_cls, clast, carry_cls, carry_last, cb_key, cb_prev, lists;
-  DevBuf<int64_t> ctot_n, ctot_s, carry_n, carry_s;
-  DevBuf<double> cb_g;
-  std::vector<BigItem> host_items;
+  std::vector<V2Item> host_items;
+  std::vector<V2Tile> host_htiles, host_stiles;
 };
+
+// Chunk-parallel search of the big items of one phase (see k_v2_*): items in
+// batches of at most kV2Batch scratch entries.
+constexpr int64_t kV2Batch = (int64_t)384 << 20;
 
 template <int MAXC, int TASK>
 void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const ItemArgs& it, const uint32_t* groups,
                   int64_t n_groups, hipStream_t s, int num_cu) {
   const int64_t n_all = n_groups * it.per_group;
   if (n_all == 0) return;
-  B.items.reserve(n_all);
-  hipLaunchKernelGGL(k_big_items, dim3(grid_for(n_all)), dim3(kBlock), 0, s, d, lv, groups, (int64_t)0, n_groups,
-                     it.per_group, it.j0, it.D, B.items.p);
-  B.host_items.resize(n_all);
-  DTE_HIP(hipMemcpyAsync(B.host_items.data(), B.items.p, sizeof(BigItem) * n_all, hipMemcpyDeviceToHost, s));
-  DTE_HIP(hipStreamSynchronize(s));
-  const int64_t budget = (int64_t)256 << 20;  // scratch elements per batch
   const int C = d.C;
+  int nb_cap = kV2MaxB;
+  if (TASK == 0)
+    while (nb_cap * C > kV2ClsWords) nb_cap >>= 1;
+  B.items.reserve(n_all);
+  hipLaunchKernelGGL(k_v2_items, dim3(grid_for(n_all)), dim3(kBlock), 0, s, d, lv, groups, n_groups, it.per_group,
+                     it.j0, it.D, nb_cap, B.items.p);
+  B.host_items.resize(n_all);
+  DTE_HIP(hipMemcpyAsync(B.host_items.data(), B.items.p, sizeof(V2Item) * n_all, hipMemcpyDeviceToHost, s));
+  DTE_HIP(hipStreamSynchronize(s));
+  constexpr int kSTile = TASK == 0 ? 4096 : 2048;
   int64_t k0 = 0;
-  std::vector<BigTile> tiles;
   while (k0 < n_all) {
     int64_t k1 = k0, el = 0, nb = 0;
-    tiles.clear();
-    while (k1 < n_all && (k1 == k0 || el + B.host_items[k1].m <= budget)) {
-      BigItem& bi = B.host_items[k1];
-      bi.boff = nb;
+    int nb_max = 1;
+    B.host_htiles.clear();
+    B.host_stiles.clear();
+    while (k1 < n_all && (k1 == k0 || el + B.host_items[k1].m <= kV2Batch)) {
+      V2Item& bi = B.host_items[k1];
+      bi.boff = (int32_t)nb;
       bi.soff = el;
-      for (int e0 = 0; e0 < bi.m; e0 += kBigTile) tiles.push_back(BigTile{(uint32_t)(k1 - k0), (uint32_t)e0});
+      const uint32_t li = (uint32_t)(k1 - k0);
+      for (int e0 = 0; e0 < bi.m; e0 += kV2Tile) B.host_htiles.push_back(V2Tile{li, (uint32_t)e0});
+      for (int e0 = 0; e0 < bi.m; e0 += kSTile) B.host_stiles.push_back(V2Tile{li, (uint32_t)e0});
       el += bi.m;
       nb += bi.nb;
+      nb_max = std::max(nb_max, bi.nb);
       ++k1;
     }
     const int64_t n_items = k1 - k0;
-    // Greedy packing: two consecutive sorted chunks hold more than kBigCap
-    // entries, a histogram chunk holds more than kBigCap on its own.
-    const int64_t max_chunks = std::min<int64_t>(nb + n_items, 3 * (el / kBigCap) + 2 * n_items + 2);
-    B.items.reserve(n_all);
-    DTE_HIP(hipMemcpyAsync(B.items.p, B.host_items.data() + k0, sizeof(BigItem) * n_items, hipMemcpyHostToDevice, s));
-    B.tiles.reserve(tiles.size());
-    DTE_HIP(hipMemcpyAsync(B.tiles.p, tiles.data(), sizeof(BigTile) * tiles.size(), hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(B.items.p, B.host_items.data() + k0, sizeof(V2Item) * n_items, hipMemcpyHostToDevice, s));
+    B.htiles.reserve(B.host_htiles.size());
+    B.stiles.reserve(B.host_stiles.size());
+    DTE_HIP(hipMemcpyAsync(B.htiles.p, B.host_htiles.data(), sizeof(V2Tile) * B.host_htiles.size(),
+                           hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemcpyAsync(B.stiles.p, B.host_stiles.data(), sizeof(V2Tile) * B.host_stiles.size(),
+                           hipMemcpyHostToDevice, s));
     B.bcount.reserve(nb);
+    B.bmax.reserve(nb);
+    B.bbase.reserve(nb);
     B.bcur.reserve(nb);
-    B.chunks.reserve(max_chunks);
-    B.ctr.reserve(3);
+    B.carry_n.reserve(nb);
+    B.carry_last.reserve(nb);
+    B.res_g.reserve(nb);
+    B.res_key.reserve(nb);
+    B.res_prev.reserve(nb);
+    B.tasks.reserve(nb);
+    B.spill.reserve(nb);
+    B.ctr.reserve(2);
+    if (TASK == 0) {
+      B.bcls.reserve((size_t)nb * C);
+      B.carry_cls.reserve((size_t)nb * C);
+    } else {
+      B.bwn.reserve(nb);
+      B.bsum.reserve(nb);
+      B.carry_s.reserve(nb);
+    }
     B.skey.reserve(el);
     B.saux.reserve(el);
-    if (TASK == 1) B.sy.reserve(el);
-    B.ctot_cls.reserve(TASK == 0 ? max_chunks * C : 1);
-    B.carry_cls.reserve(TASK == 0 ? max_chunks * C : 1);
-    B.ctot_n.reserve(max_chunks);
-    B.carry_n.reserve(max_chunks);
-    B.ctot_s.reserve(max_chunks);
-    B.carry_s.reserve(max_chunks);
-    B.clast.reserve(max_chunks);
-    B.carry_last.reserve(max_chunks);
-    B.cb_g.reserve(max_chunks);
-    B.cb_key.reserve(max_chunks);
-    B.cb_prev.reserve(max_chunks);
-    B.lists.reserve(2 * max_chunks);
+    B.skey2.reserve(el);
+    B.saux2.reserve(el);
+    if (TASK == 1) {
+      B.sy.reserve(el);
+      B.sy2.reserve(el);
+    }
     DTE_HIP(hipMemsetAsync(B.bcount.p, 0, 4 * nb, s));
-    DTE_HIP(hipMemsetAsync(B.ctr.p, 0, 12, s));
-    BigArgs ba{};
-    ba.items = B.items.p;
-    ba.n_items = n_items;
-    ba.tiles = B.tiles.p;
-    ba.bcount = B.bcount.p;
-    ba.bcur = B.bcur.p;
-    ba.chunks = B.chunks.p;
-    ba.chunk_ctr = B.ctr.p;
-    ba.skey = B.skey.p;
-    ba.saux = B.saux.p;
-    ba.sy = TASK == 1 ? B.sy.p : nullptr;
-    ba.ctot_cls = B.ctot_cls.p;
-    ba.ctot_n = B.ctot_n.p;
-    ba.ctot_s = TASK == 1 ? B.ctot_s.p : nullptr;
-    ba.clast = B.clast.p;
-    ba.carry_cls = B.carry_cls.p;
-    ba.carry_n = B.carry_n.p;
-    ba.carry_s = TASK == 1 ? B.carry_s.p : nullptr;
-    ba.carry_last = B.carry_last.p;
-    ba.cb_g = B.cb_g.p;
-    ba.cb_key = B.cb_key.p;
-    ba.cb_prev = B.cb_prev.p;
-    const unsigned n_tiles = (unsigned)tiles.size();
-    hipLaunchKernelGGL(k_big_hist<TASK>, dim3(n_tiles), dim3(kBlock), 0, s, d, lv, ba);
-    hipLaunchKernelGGL(k_big_plan, dim3((unsigned)n_items), dim3(kBlock), 0, s, ba);
-    hipLaunchKernelGGL(k_big_scatter<TASK>, dim3(n_tiles), dim3(kBlock), 0, s, d, lv, ba);
-    unsigned int n_chunks = 0;
-    DTE_HIP(hipMemcpyAsync(&n_chunks, B.ctr.p, 4, hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipMemsetAsync(B.bmax.p, 0, 4 * nb, s));
+    if (TASK == 0) {
+      DTE_HIP(hipMemsetAsync(B.bcls.p, 0, 4 * (size_t)nb * C, s));
+    } else {
+      DTE_HIP(hipMemsetAsync(B.bwn.p, 0, 4 * nb, s));
+      DTE_HIP(hipMemsetAsync(B.bsum.p, 0, 8 * nb, s));
+    }
+    DTE_HIP(hipMemsetAsync(B.ctr.p, 0, 8, s));
+    V2Args a{};
+    a.items = B.items.p;
+    a.n_items = n_items;
+    a.bcount = B.bcount.p;
+    a.bmax = B.bmax.p;
+    a.bcls = B.bcls.p;
+    a.bwn = B.bwn.p;
+    a.bsum = B.bsum.p;
+    a.bbase = B.bbase.p;
+    a.bcur = B.bcur.p;
+    a.carry_cls = B.carry_cls.p;
+    a.carry_n = B.carry_n.p;
+    a.carry_s = B.carry_s.p;
+    a.carry_last = B.carry_last.p;
+    a.res_g = B.res_g.p;
+    a.res_key = B.res_key.p;
+    a.res_prev = B.res_prev.p;
+    a.skey = B.skey.p;
+    a.saux = B.saux.p;
+    a.sy = B.sy.p;
+    a.skey2 = B.skey2.p;
+    a.saux2 = B.saux2.p;
+    a.sy2 = B.sy2.p;
+    a.tasks = B.tasks.p;
+    a.spill = B.spill.p;
+    a.counters = B.ctr.p;
+    a.tiles = B.htiles.p;
+    const size_t dyn = TASK == 0 ? sizeof(uint32_t) * (size_t)nb_max * C : 0;
+    hipLaunchKernelGGL(k_v2_hist<TASK>, dim3((unsigned)B.host_htiles.size()), dim3(kBlock), dyn, s, d, lv, a);
+    hipLaunchKernelGGL(k_v2_scan<TASK>, dim3((unsigned)n_items), dim3(kBlock), 0, s, d, a);
+    a.tiles = B.stiles.p;
+    hipLaunchKernelGGL(k_v2_scatter<TASK>, dim3((unsigned)B.host_stiles.size()), dim3(kBlock), 0, s, d, lv, a);
+    unsigned int nt[2];
+    DTE_HIP(hipMemcpyAsync(nt, B.ctr.p, 8, hipMemcpyDeviceToHost, s));
     DTE_HIP(hipStreamSynchronize(s));
-    hipLaunchKernelGGL((k_big_ctot<MAXC, TASK>), dim3(grid_for(n_chunks, kWaves, num_cu * 16)), dim3(kBlock), 0, s, d,
-                       ba, (int64_t)n_chunks);
-    hipLaunchKernelGGL(k_big_carry, dim3(grid_for(n_items)), dim3(kBlock), 0, s, d, ba);
-    hipLaunchKernelGGL(k_chunk_split, dim3(grid_for(n_chunks)), dim3(kBlock), 0, s, B.chunks.p, (int64_t)n_chunks,
-                       B.lists.p, B.lists.p + max_chunks, B.ctr.p + 1);
-    unsigned int nl[2];
-    DTE_HIP(hipMemcpyAsync(nl, B.ctr.p + 1, 8, hipMemcpyDeviceToHost, s));
-    DTE_HIP(hipStreamSynchronize(s));
-    if (nl[0])
-      hipLaunchKernelGGL((k_big_eval<MAXC, TASK, 64>), dim3(grid_for(nl[0], kWaves, num_cu * 16)), dim3(kBlock), 0,
-                         s, d, lv, ba, (const uint32_t*)B.lists.p, (int64_t)nl[0]);
-    if (nl[1])
-      hipLaunchKernelGGL((k_big_eval<MAXC, TASK, 256>), dim3(grid_for(nl[1], 1, num_cu * 4)), dim3(kBlock), 0, s,
-                         d, lv, ba, (const uint32_t*)(B.lists.p + max_chunks), (int64_t)nl[1]);
-    hipLaunchKernelGGL(k_big_reduce<TASK>, dim3(grid_for(n_items)), dim3(kBlock), 0, s, d, ba, it, 0);
+    if (nt[0])
+      hipLaunchKernelGGL((k_v2_eval<MAXC, TASK>), dim3((unsigned)std::min<int64_t>(nt[0], (int64_t)num_cu * 8)),
+                         dim3(kBlock), 0, s, d, lv, a, (int64_t)nt[0]);
+    if (nt[1])
+      hipLaunchKernelGGL((k_v2_spill<MAXC, TASK>), dim3((unsigned)std::min<int64_t>(nt[1], (int64_t)num_cu * 4)),
+                         dim3(kBlock), 0, s, d, lv, a, (int64_t)nt[1]);
+    hipLaunchKernelGGL(k_v2_reduce<TASK>, dim3((unsigned)((n_items + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d,
+                       a, it);
     if (it.bytes) {
       // algorithmic bytes of the batch: 12 (20) bytes per entry
-      unsigned long long by = (unsigned long long)el * (TASK == 0 ? 12ull : 20ull);
+      const unsigned long long by = (unsigned long long)el * (TASK == 0 ? 12ull : 20ull);
       hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(1), 0, s, it.bytes, by);
     }
     k0 = k1;
@@ -2703,6 +3058,14 @@ void upload_ln_table(int device) {
 constexpr int kPlainTrees = 100;
 constexpr int64_t kPlainChunk = 512;
 
+// Largest count with a precomputed x ln x term (512 MB of table).
+constexpr int64_t kTMaxCap = (int64_t)1 << 26;
+
+__global__ void k_build_T(int64_t* T, int64_t tmax, int F) {
+  for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x <= tmax; x += (int64_t)gridDim.x * blockDim.x)
+    T[x] = xlnx_fixed((uint64_t)x, F, g_ln_table.inv, g_ln_table.logc);
+}
+
 // Device-resident inputs of one fit, shared by every tree batch.
 struct FitData {
   DevBuf<float> X;         // canonical copy (-0 -> +0), feature-major
@@ -2713,7 +3076,7 @@ struct FitData {
   DevBuf<float> SV;
   DevBuf<int64_t> SVoff;
   DevBuf<uint32_t> Sdev;
-  DevBuf<int64_t> Ttab;    // fixed-point x ln x for small counts
+  DevBuf<int64_t> Ttab;    // fixed-point x ln x terms T[0..Tmax]
   DataArgs d{};
 };
 
@@ -2789,13 +3152,13 @@ void prepare_data(FitData& fd, const float* X_dev_in, const float* y_host, int64
   d.seed = p.seed;
   d.min_node = p.min_node_size;
   d.max_depth = p.max_depth;
-  Ttab.alloc(kTT);
   {
     const LnTable& lt = host_ln_table();
     d.igF = ig_shift((uint64_t)N, task == 0 ? C : 1, lt.inv, lt.logc);
-    std::vector<int64_t> th(kTT);
-    for (int x = 0; x < kTT; ++x) th[x] = xlnx_fixed((uint64_t)x, d.igF, lt.inv, lt.logc);
-    DTE_HIP(hipMemcpyAsync(Ttab.p, th.data(), 8 * kTT, hipMemcpyHostToDevice, s));
+    // Counts of a fit never exceed the root's total weight N.
+    d.Tmax = task == 0 ? std::min<int64_t>(N, kTMaxCap) : 0;
+    Ttab.alloc(d.Tmax + 1);
+    hipLaunchKernelGGL(k_build_T, dim3(grid_for(d.Tmax + 1)), dim3(kBlock), 0, s, Ttab.p, d.Tmax, d.igF);
     d.T = Ttab.p;
   }
 }
@@ -3012,8 +3375,10 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   // Size tiers of the split search: wave per item up to wave_cutoff entries,
   // block per item above (the block kernel itself switches to the bucketed
   // path above its LDS capacity).
-  int big_min = 0x7FFFFFFF;  // entries from which a node goes through the chunk pipeline (off by default)
-  if (const char* e = std::getenv("DTE_BIG_MIN")) big_min = std::atoi(e);
+  // Nodes above the workgroup's LDS capacity go through the chunk-parallel
+  // pipeline (the plain driver keeps one workgroup per node throughout).
+  int big_min = kCap + 1;
+  if (const char* e = std::getenv("DTE_BIG_MIN")) big_min = std::max(kCap + 1, std::atoi(e));
   Cuts cuts{};
   if (algorithm == DTE_RANDOM_FOREST) {
     const int c16 = (task == 1 || C <= 16) ? 16 : 0;
@@ -3050,6 +3415,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       const bool rf = algorithm == DTE_RANDOM_FOREST;
       trace_tier[rf ? (tier <= 2 ? 0 : (tier == 3 ? 1 : 2)) : (tier == 0 ? 0 : 2)] += cnt[tier];
       if (rf && tier == 5) {
+        Timer tk(search_family(algorithm, tier), 0.0, s);
         big(bigst, d, lv, a, tier_list[5].p, cnt[5], s, num_cu);
         continue;
       }
@@ -3148,7 +3514,9 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       retry_flag.reserve(n_open);
       retry_pos.reserve(n_open + 1);
       retry_list.reserve(n_open + 1);
-      if (algorithm == DTE_RANDOM_FOREST && max_m > kCap) {
+      // scratch slabs of the in-kernel bucketed path (nodes above kCap that
+      // do not go through the chunk pipeline)
+      if (algorithm == DTE_RANDOM_FOREST && max_m > kCap && (plain || big_min > kCap + 1)) {
         const int64_t need = (int64_t)search_grid_cap * max_m;
         if (scr_stride < max_m) {
           scr_key.alloc(need);
