@@ -546,8 +546,10 @@ __device__ __forceinline__ bool node_context(const DataArgs& d, const LevelArgs&
   }
 }
 
-// Evaluate one RF item on a team; returns through L.cr.
-template <int MAXC, int TASK, int TEAM, int IPT>
+// Evaluate one RF item on a team; returns through L.cr.  BIG: the item may
+// exceed the team's LDS capacity (whole-workgroup teams of the plain driver):
+// bucket it through the workgroup's scratch slab.
+template <int MAXC, int TASK, int TEAM, int IPT, bool BIG>
 __device__ void rf_item(const DataArgs& d, const LevelArgs& lv, const ItemArgs& it, int f, uint32_t i, const NodeCtx& nc,
                         TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx, uint32_t* s_boff, int team_index) {
   constexpr int CAP = TeamLds<MAXC, TASK, TEAM, IPT>::CAP;
@@ -595,7 +597,7 @@ __device__ void rf_item(const DataArgs& d, const LevelArgs& lv, const ItemArgs& 
     sort_and_eval<MAXC, TASK, TEAM, IPT>(m, bit_length(kmax - kmin), kmin, nc, L, lx);
     return;
   }
-  if constexpr (TEAM == 256) {
+  if constexpr (TEAM == 256 && BIG) {
     // Big node: bucket by dense key into this block's scratch slab, then
     // pack consecutive buckets into LDS-sized chunks.
     const uint32_t S = d.S[f];
@@ -727,13 +729,13 @@ __device__ __forceinline__ void rf_write_result(const DataArgs& d, const ItemArg
 // entries).  TEAM=256: whole workgroup; nodes up to 256*IPT entries are
 // sorted in LDS, larger ones are bucketed by dense key into a scratch slab
 // first.  Every boundary between distinct values is scored.
-template <int MAXC, int TASK, int TEAM>
+template <int MAXC, int TASK, int TEAM, bool BIG>
 __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, ItemArgs it) {
   constexpr int IPT = kIpt;
   constexpr int NT = kBlock / TEAM;
   __shared__ double s_inv[128], s_logc[128];
   __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[NT];
-  __shared__ uint32_t s_boff[TEAM == 256 ? kMaxBuckets + 1 : 1];
+  __shared__ uint32_t s_boff[BIG ? kMaxBuckets + 1 : 1];
   load_tables(s_inv, s_logc);
   __syncthreads();
   const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
@@ -764,7 +766,7 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
       continue;
     }
     if (t == 0) my_bytes += (unsigned long long)lv.m[i] * (TASK == 0 ? 12ull : 20ull);
-    rf_item<MAXC, TASK, TEAM, IPT>(d, lv, it, f, i, nc, L, lx, s_boff, (int)blockIdx.x);
+    rf_item<MAXC, TASK, TEAM, IPT, BIG>(d, lv, it, f, i, nc, L, lx, s_boff, (int)blockIdx.x);
     team_sync<TEAM>();
     if (t == 0) rf_write_result<MAXC, TASK, TEAM, IPT>(d, it, ridx, f, L.cr);
     team_sync<TEAM>();
@@ -2005,7 +2007,7 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArg
 // item writes its own result slot.
 // Wave-aggregated: one global atomic per wave and tier.  Tier k holds the
 // groups with cuts[k-1] < m <= cuts[k].
-constexpr int kMaxTiers = 6;
+constexpr int kMaxTiers = 7;
 struct Cuts {
   int c[kMaxTiers];  // ascending; the last tier takes everything above
   int n;             // number of tiers
@@ -2793,15 +2795,16 @@ T read_one(const T* p, hipStream_t s) {
 // Profile family of a split-search tier (one kernel per family).
 const char* search_family(int algorithm, int tier) {
   static const char* rf[] = {"search_seg16", "search_seg32", "search_seg64", "search_team64", "search_block",
-                             "search_chunked"};
+                             "search_chunked", "search_block"};
   static const char* ert[] = {"search_ert_wave", "search_ert_block"};
   return algorithm == DTE_RANDOM_FOREST ? rf[tier] : ert[tier];
 }
 
 // RF tiers: 0/1/2 = one 16/32/64-lane segment per node (one entry per lane,
 // draws looped in the kernel); 3 = wave teams (four items per workgroup);
-// 4 = whole workgroup per item.  ERT tiers: 0 = one wave per item,
-// 1 = workgroup per item.
+// 4 = whole workgroup per item within the LDS capacity; 5 = the chunk
+// pipeline (big_pipeline); 6 = whole workgroup per item of any size (the
+// plain driver).  ERT tiers: 0 = one wave per item, 1 = workgroup per item.
 template <int MAXC, int TASK>
 void launch_search(int algorithm, int tier, int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv,
                    const ItemArgs& it) {
@@ -2810,8 +2813,9 @@ void launch_search(int algorithm, int tier, int grid, hipStream_t s, const DataA
       case 0: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 16>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       case 1: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 32>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       case 2: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
-      case 3: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
-      default: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 3: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 64, false>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 4: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256, false>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      default: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256, true>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
     }
   } else {
     if (tier == 0)
@@ -3431,7 +3435,9 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         grid = (int)std::min<int64_t>((items + per_block - 1) / per_block, cap);
       }
       Timer tk(search_family(algorithm, tier), 0.0, s);
-      search(algorithm, tier, grid, s, d, lv, a);
+      // tier 4 holds nodes above the LDS capacity when the pipeline is
+      // raised past it (DTE_BIG_MIN): those need the bucketed kernel
+      search(algorithm, (rf && tier == 4 && big_min > kCap + 1) ? 6 : tier, grid, s, d, lv, a);
     }
   };
   // Plain launch structure: chunks of at most kPlainChunk open nodes, one
@@ -3444,7 +3450,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   };
   auto plain_phase = [&](ItemArgs it) {
     const uint32_t* lst = it.list ? it.list : identity_list(it.n_groups);
-    const int tier = algorithm == DTE_RANDOM_FOREST ? 4 : 1;
+    const int tier = algorithm == DTE_RANDOM_FOREST ? 6 : 1;  // RF: workgroup per node, any size
     Timer tm("split_search", 0.0, s);
     for (int64_t c0 = 0; c0 < it.n_groups; c0 += kPlainChunk) {
       ItemArgs a = it;
