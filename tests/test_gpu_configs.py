@@ -1,0 +1,131 @@
+"""The other workload shapes as GPU parity cases.
+
+- Regression forest, variance criterion: 4,000,000 x 64 U[0,1), y = sin(4 x0)
+  + x1 x2 + N(0, 0.1), 100 trees, depth 18, 21 features per node.  At full
+  size the forest is checked through properties that hold whatever the
+  draws (children partition their parent's weights, distinct rows, row
+  hashes and fixed-point target sums; the leaf rule; every split score within
+  1e-5 relative of an fp64 recompute); the oracle checks the same shape at
+  100,000 rows bit for bit.
+- 64 uneven extra-trees forests (2k..200k rows log-uniform, 16..256
+  features, 2..8 classes, 50 deep trees each): every forest is built and
+  checked by the same properties; the smallest eight are compared with the
+  oracle bit for bit.
+"""
+import numpy as np
+import pytest
+
+import pyoracle
+from datamininglibs_amd import capi, synth
+from parity import assert_same_forest
+
+pytestmark = pytest.mark.gpu
+
+MASK = (1 << 64) - 1
+
+
+def check_forest(c, *, n_rows, max_depth, min_node, task, n_classes=0, fixed_shift=0):
+    """Size-independent invariants of a canonical forest (numpy, vectorised)."""
+    ts = c["tree_start"]
+    feat = c["feature"]
+    left = c["left"]
+    internal = np.nonzero(feat >= 0)[0]
+    l = left[internal].astype(np.int64)
+    r = l + 1
+    assert (l > internal).all()
+    for k in ("count", "n_entries"):
+        a = c[k].astype(np.int64)
+        assert np.array_equal(a[l] + a[r], a[internal]), k
+    h = c["rowhash"]
+    assert np.array_equal(h[l] + h[r], h[internal])  # uint64 wrap-around sum
+    assert (c["depth"][l] == c["depth"][internal] + 1).all()
+    assert (c["depth"] <= max_depth).all()
+    assert (c["count"][internal] > min_node).all()
+    assert (c["count"][ts[:-1]] == n_rows).all()
+    leaves = feat < 0
+    tree_of = np.repeat(np.arange(ts.size - 1), np.diff(ts))
+    assert (np.bincount(tree_of[leaves], weights=c["count"][leaves]) == n_rows).all()
+    if task == capi.CLASSIFICATION:
+        cc = c["class_counts"].astype(np.int64)
+        assert np.array_equal(cc[l] + cc[r], cc[internal])
+        assert (cc.sum(1) == c["count"]).all()
+        # fp64 recompute of the information gain from the children's counts
+        def H(m):
+            tot = m.sum(1, keepdims=True).astype(np.float64)
+            q = np.where(m > 0, m / tot, 1.0)
+            return -(np.where(m > 0, q * np.log2(q), 0.0)).sum(1), tot[:, 0]
+        hn, tn = H(cc[internal])
+        hl, tl = H(cc[l])
+        hr, tr = H(cc[r])
+        rec = hn - tl / tn * hl - tr / tn * hr
+    else:
+        s = c["sum_fixed"]
+        assert np.array_equal(s[l] + s[r], s[internal])
+        nl = c["count"][l].astype(np.float64)
+        nr = c["count"][r].astype(np.float64)
+        ml = np.ldexp(s[l].astype(np.float64), -fixed_shift) / nl
+        mr = np.ldexp(s[r].astype(np.float64), -fixed_shift) / nr
+        rec = nl * nr / (nl + nr) * (ml - mr) ** 2
+        # leaf value = node mean
+        lv = leaves
+        mean = np.ldexp(s[lv].astype(np.float64), -fixed_shift) / c["count"][lv]
+        assert np.allclose(c["value"][lv], mean, rtol=1e-6, atol=1e-7)
+    sc = c["score"][internal].astype(np.float64)
+    tol = 1e-5 * np.maximum(np.abs(rec), 1e-30) + 1e-9
+    assert (np.abs(sc - rec) <= tol).all(), float(np.max(np.abs(sc - rec) / np.maximum(np.abs(rec), 1e-30)))
+
+
+def test_regression_4m_x_64_full_size():
+    X, y = synth.sine_regression(seed=0, n=4_000_000, f=64)
+    p = capi.default_params(capi.RANDOM_FOREST, task=capi.REGRESSION, n_trees=100, n_features_per_node=21,
+                            max_depth=18, min_node_size=5, n_classes=0, seed=7)
+    f = capi.fit(X, y, p)
+    c = f.canonical()
+    check_forest(c, n_rows=4_000_000, max_depth=18, min_node=5, task=capi.REGRESSION,
+                 fixed_shift=f.info.fixed_shift)
+    pred = f.predict(X[:, :200_000]) / 100.0
+    rmse = float(np.sqrt(np.mean((pred - y[:200_000]) ** 2)))
+    assert rmse < 0.12, rmse  # noise floor 0.1
+
+
+def test_regression_shape_vs_oracle():
+    X, y = synth.sine_regression(seed=1, n=100_000, f=64)
+    p = capi.default_params(capi.RANDOM_FOREST, task=capi.REGRESSION, n_trees=2, n_features_per_node=21,
+                            max_depth=18, min_node_size=5, n_classes=0, seed=7)
+    g = capi.fit(X, y, p)
+    o = pyoracle.fit(X, y, algorithm=0, task=1, n_trees=2, k=21, max_depth=18, min_node_size=5, seed=7)
+    assert_same_forest(g, o)
+
+
+def test_regression_plain_equals_batched_at_scale():
+    X, y = synth.sine_regression(seed=2, n=1_000_000, f=64)
+    kw = dict(task=capi.REGRESSION, n_trees=4, n_features_per_node=21, max_depth=18, min_node_size=5,
+              n_classes=0, seed=9)
+    a = capi.fit(X, y, capi.default_params(capi.RANDOM_FOREST, **kw)).canonical()
+    b = capi.fit(X, y, capi.default_params(capi.RANDOM_FOREST, build_mode=capi.BUILD_PLAIN, **kw)).canonical()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_64_uneven_extra_trees_forests():
+    specs = synth.uneven_forests(seed=0)
+    smallest = set(s["index"] for s in sorted(specs, key=lambda s: s["n"] * s["f"])[:8])
+    for spec in specs:
+        X, y = synth.uneven_forest_data(spec)
+        p = capi.default_params(capi.EXTRA_TREES, n_trees=50, n_classes=spec["classes"], seed=spec["seed"])
+        f = capi.fit(X, y, p)
+        c = f.canonical()
+        check_forest(c, n_rows=spec["n"], max_depth=1000, min_node=5, task=capi.CLASSIFICATION,
+                     n_classes=spec["classes"])
+        if spec["index"] in smallest:
+            o = pyoracle.fit(X, y, algorithm=1, n_trees=50, n_classes=spec["classes"], seed=spec["seed"])
+            assert_same_forest(f, o)
+        f.close()
+
+
+def test_rf_1m_x_128_full_size_invariants():
+    X, y = synth.gaussian_classes(seed=0)
+    p = capi.default_params(capi.RANDOM_FOREST, n_trees=20, n_features_per_node=11, max_depth=20, min_node_size=5,
+                            n_classes=10, seed=324123)
+    f = capi.fit(X, y, p)
+    check_forest(f.canonical(), n_rows=1_000_000, max_depth=20, min_node=5, task=capi.CLASSIFICATION, n_classes=10)
