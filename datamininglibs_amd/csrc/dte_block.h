@@ -29,24 +29,51 @@ __device__ __forceinline__ int team_tid() {
   return TEAM == kWave ? (int)(threadIdx.x & (kWave - 1)) : (int)threadIdx.x;
 }
 
+// Wave64 scans and reductions through DPP (GFX9 row shifts and row
+// broadcasts): plain VALU data movement, no LDS crossbar.  They assume all
+// 64 lanes are active, which every caller guarantees.
+// DPP controls: row_shr:n = 0x110 + n, row_bcast:15 = 0x142, row_bcast:31 =
+// 0x143.  A lane whose source is out of its row, or whose row is masked
+// off, gets `ident`.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v, uint32_t ident) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)ident, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    uint32_t y = __shfl_up(x, off, kWave);
-    if (lane >= off) x += y;
-  }
+  x += dpp_u32<0x111>(x, 0u);
+  x += dpp_u32<0x112>(x, 0u);
+  x += dpp_u32<0x114>(x, 0u);
+  x += dpp_u32<0x118>(x, 0u);
+  x += dpp_u32<0x142, 0xa>(x, 0u);
+  x += dpp_u32<0x143, 0xc>(x, 0u);
   return x;
 }
 
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ int64_t dpp_i64(int64_t v) {
+  const uint64_t u = (uint64_t)v;
+  const uint32_t lo = dpp_u32<CTRL, ROW_MASK>((uint32_t)u, 0u);
+  const uint32_t hi = dpp_u32<CTRL, ROW_MASK>((uint32_t)(u >> 32), 0u);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t x) {
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    int64_t y = __shfl_up(x, off, kWave);
-    if (lane >= off) x += y;
-  }
+  x += dpp_i64<0x111>(x);
+  x += dpp_i64<0x112>(x);
+  x += dpp_i64<0x114>(x);
+  x += dpp_i64<0x118>(x);
+  x += dpp_i64<0x142, 0xa>(x);
+  x += dpp_i64<0x143, 0xc>(x);
   return x;
+}
+
+__device__ __forceinline__ uint32_t wave_last(uint32_t v) { return (uint32_t)__builtin_amdgcn_readlane((int)v, 63); }
+__device__ __forceinline__ int64_t wave_last_i64(int64_t v) {
+  const uint64_t u = (uint64_t)v;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 63);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 63);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
 // Exclusive scan over the team; s_w holds kWaves words (unused for a wave
@@ -55,7 +82,7 @@ template <int TEAM>
 __device__ __forceinline__ uint32_t team_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
   const uint32_t inc = wave_incl_scan(v);
   if constexpr (TEAM == kWave) {
-    if (total) *total = __shfl(inc, kWave - 1, kWave);
+    if (total) *total = wave_last(inc);
     return inc - v;
   } else {
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
@@ -78,7 +105,7 @@ template <int TEAM>
 __device__ __forceinline__ int64_t team_excl_scan_i64(int64_t v, int64_t* s_w, int64_t* total) {
   const int64_t inc = wave_incl_scan_i64(v);
   if constexpr (TEAM == kWave) {
-    if (total) *total = __shfl(inc, kWave - 1, kWave);
+    if (total) *total = wave_last_i64(inc);
     return inc - v;
   } else {
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
@@ -106,7 +133,7 @@ __device__ __forceinline__ void team_excl_scan_vec(uint32_t (&v)[N], uint32_t* s
   if constexpr (TEAM == kWave) {
 #pragma unroll
     for (int c = 0; c < N; ++c) {
-      tot[c] = __shfl(inc[c], kWave - 1, kWave);
+      tot[c] = wave_last(inc[c]);
       v[c] = inc[c] - v[c];
     }
   } else {
@@ -141,7 +168,7 @@ __device__ __forceinline__ void team_excl_scan_i64_vec(int64_t (&v)[N], int64_t*
   if constexpr (TEAM == kWave) {
 #pragma unroll
     for (int c = 0; c < N; ++c) {
-      tot[c] = __shfl(inc[c], kWave - 1, kWave);
+      tot[c] = wave_last_i64(inc[c]);
       v[c] = inc[c] - v[c];
     }
   } else {
@@ -167,14 +194,22 @@ __device__ __forceinline__ void team_excl_scan_i64_vec(int64_t (&v)[N], int64_t*
 }
 
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, kWave));
-  return v;
+  v = max(v, dpp_u32<0x111>(v, 0u));
+  v = max(v, dpp_u32<0x112>(v, 0u));
+  v = max(v, dpp_u32<0x114>(v, 0u));
+  v = max(v, dpp_u32<0x118>(v, 0u));
+  v = max(v, dpp_u32<0x142, 0xa>(v, 0u));
+  v = max(v, dpp_u32<0x143, 0xc>(v, 0u));
+  return wave_last(v);
 }
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor(v, off, kWave));
-  return v;
+  v = min(v, dpp_u32<0x111>(v, ~0u));
+  v = min(v, dpp_u32<0x112>(v, ~0u));
+  v = min(v, dpp_u32<0x114>(v, ~0u));
+  v = min(v, dpp_u32<0x118>(v, ~0u));
+  v = min(v, dpp_u32<0x142, 0xa>(v, ~0u));
+  v = min(v, dpp_u32<0x143, 0xc>(v, ~0u));
+  return wave_last(v);
 }
 
 template <int TEAM>
@@ -223,19 +258,36 @@ __device__ __forceinline__ bool cand_better(double ga, uint32_t ka, double gb, u
   return ga > gb || (ga == gb && ka < kb);
 }
 
-__device__ __forceinline__ BestCand wave_best(BestCand b) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    const double g = __shfl_xor(b.g, off, kWave);
-    const uint32_t k = __shfl_xor(b.key, off, kWave);
-    const uint32_t p = __shfl_xor(b.prev, off, kWave);
-    if (cand_better(g, k, b.g, b.key)) {
-      b.g = g;
-      b.key = k;
-      b.prev = p;
-    }
+// One step of the best-candidate scan: combine with the DPP source lane
+// (lanes without a source see key = none, which never wins).
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ void best_step(BestCand& b) {
+  const uint64_t gb = __builtin_bit_cast(uint64_t, b.g);
+  const uint32_t glo = dpp_u32<CTRL, ROW_MASK>((uint32_t)gb, 0u);
+  const uint32_t ghi = dpp_u32<CTRL, ROW_MASK>((uint32_t)(gb >> 32), 0u);
+  const uint32_t k = dpp_u32<CTRL, ROW_MASK>(b.key, 0xFFFFFFFFu);
+  const uint32_t p = dpp_u32<CTRL, ROW_MASK>(b.prev, 0xFFFFFFFFu);
+  const double g = __builtin_bit_cast(double, ((uint64_t)ghi << 32) | glo);
+  if (k != 0xFFFFFFFFu && cand_better(g, k, b.g, b.key)) {
+    b.g = g;
+    b.key = k;
+    b.prev = p;
   }
-  return b;
+}
+
+// Best candidate of the wave (higher gain, then lower key); every lane
+// gets it.  The "better" relation is a total order, so a scan-style
+// combination in any association gives the maximum.
+__device__ __forceinline__ BestCand wave_best(BestCand b) {
+  best_step<0x111>(b);
+  best_step<0x112>(b);
+  best_step<0x114>(b);
+  best_step<0x118>(b);
+  best_step<0x142, 0xa>(b);
+  best_step<0x143, 0xc>(b);
+  const uint64_t gb = __builtin_bit_cast(uint64_t, b.g);
+  const uint32_t glo = wave_last((uint32_t)gb), ghi = wave_last((uint32_t)(gb >> 32));
+  return BestCand{__builtin_bit_cast(double, ((uint64_t)ghi << 32) | glo), wave_last(b.key), wave_last(b.prev)};
 }
 
 // Team reduction of the best candidate; every lane gets the result.

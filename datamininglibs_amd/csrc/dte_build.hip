@@ -778,24 +778,27 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
 // Items (node, feature draw) above the workgroup's LDS capacity.  The keys of
 // an item (dense ranks of the drawn feature) are split into nb coarse buckets
 // of consecutive keys, bucket = key >> shift, sized so that a bucket holds
-// about kV2Target entries: one workgroup-sized chunk.  Every bucket of every
-// item of a level is then evaluated in parallel:
-//   1. k_v2_hist     per (item, tile): bucket counts, max key and class
-//                    totals (regression: weight and target sums)
-//   2. k_v2_scan     per item: bucket offsets, carries (totals and last key
-//                    before each bucket), task lists
+// about kV2Target entries: one wave-sized chunk.  Every bucket of every item
+// of a level is then evaluated in parallel:
+//   1. k_v2_hist     per (item, tile): bucket counts and max key
+//   2. k_v2_scan     per item: bucket offsets, last key before each bucket,
+//                    task lists
 //   3. k_v2_scatter  per (item, tile): entries into bucket order, staged in
 //                    LDS so that each bucket's run is written contiguously
-//   4. k_v2_eval     per bucket: LDS radix sort + scored walk from the carry
-//   5. k_v2_spill    per overfull bucket (ties, clustered keys): fine
+//   4. k_v2_ctot     per bucket: class totals (regression: weight and target
+//                    sums) of the bucket's entries
+//   5. k_v2_carry    per item: totals before each bucket
+//   6. k_v2_eval     per bucket, one wave each: LDS sort + scored walk from
+//                    the carry
+//   7. k_v2_spill    per overfull bucket (ties, clustered keys): fine
 //                    sub-buckets walked in order by one workgroup
-//   6. k_v2_reduce   per item: best bucket -> (score, threshold)
+//   8. k_v2_reduce   per item: best bucket -> (score, threshold)
 // Entry order inside a bucket does not matter: the walk sorts by key and
 // scores only boundaries between distinct keys from integer prefix sums.
-constexpr int kV2Tile = 8192;       // entries per histogram tile
-constexpr int kV2MaxB = 1024;       // buckets per item
-constexpr int kV2Target = 1024;     // entries per bucket aimed at
-constexpr int kV2ClsWords = 16384;  // LDS words for per-bucket class totals
+constexpr int kV2Tile = 32768;   // entries per histogram tile
+constexpr int kV2MaxB = 4096;    // buckets per item
+constexpr int kV2Target = 256;   // entries per bucket aimed at
+constexpr int kV2WaveCap = kWave * kIpt;  // bucket capacity of a wave (512)
 
 struct V2Item {
   int32_t f;
@@ -878,41 +881,26 @@ __device__ __forceinline__ uint32_t block_excl_max(uint32_t v, uint32_t* s_w) {
 
 template <int TASK>
 __global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2Args a) {
-  extern __shared__ uint32_t s_dyn[];  // classification: nb * C class totals
   __shared__ uint32_t s_cnt[kV2MaxB];
   __shared__ uint32_t s_max[kV2MaxB];
-  __shared__ uint32_t s_wn[TASK == 1 ? kV2MaxB : 1];
-  __shared__ unsigned long long s_sum[TASK == 1 ? kV2MaxB : 1];
   const V2Tile tl = a.tiles[blockIdx.x];
   const V2Item it = a.items[tl.item];
   const int t = threadIdx.x;
-  const int C = d.C;
   for (int b = t; b < it.nb; b += kBlock) {
     s_cnt[b] = 0;
     s_max[b] = 0;
-    if (TASK == 1) {
-      s_wn[b] = 0;
-      s_sum[b] = 0;
-    }
   }
-  if (TASK == 0)
-    for (int i = t; i < it.nb * C; i += kBlock) s_dyn[i] = 0;
   __syncthreads();
   const uint32_t* Kf = d.K + (int64_t)it.f * d.N;
   const int n = min(kV2Tile, it.m - (int)tl.e0);
   const int64_t e0 = it.start + tl.e0;
   constexpr int U = 8;
   for (int base = 0; base < n; base += kBlock * U) {
-    uint32_t rows[U], auxv[U];
-    int64_t yv[U];
+    uint32_t rows[U];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const int e = base + t + q * kBlock;
-      if (e < n) {
-        rows[q] = lv.row[e0 + e];
-        auxv[q] = lv.aux[e0 + e];
-        if (TASK == 1) yv[q] = lv.ey[e0 + e];
-      }
+      if (e < n) rows[q] = lv.row[e0 + e];
     }
 #pragma unroll
     for (int q = 0; q < U; ++q) {
@@ -920,15 +908,8 @@ __global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2
       if (e < n) {
         const uint32_t key = Kf[rows[q]];
         const uint32_t b = key >> it.shift;
-        const uint32_t w = auxv[q] & 0xFFFFu;
         atomicAdd(&s_cnt[b], 1u);
         atomicMax(&s_max[b], key);
-        if (TASK == 0) {
-          atomicAdd(&s_dyn[b * C + (auxv[q] >> 16)], w);
-        } else {
-          atomicAdd(&s_wn[b], w);
-          atomicAdd(&s_sum[b], (unsigned long long)((int64_t)w * yv[q]));
-        }
       }
     }
   }
@@ -939,121 +920,37 @@ __global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2
     const int64_t gb = (int64_t)it.boff + b;
     atomicAdd(&a.bcount[gb], c);
     atomicMax(&a.bmax[gb], s_max[b]);
-    if (TASK == 0) {
-      for (int q = 0; q < C; ++q) {
-        const uint32_t v = s_dyn[b * C + q];
-        if (v) atomicAdd(&a.bcls[gb * C + q], v);
-      }
-    } else {
-      atomicAdd(&a.bwn[gb], s_wn[b]);
-      atomicAdd((unsigned long long*)&a.bsum[gb], s_sum[b]);
-    }
   }
 }
 
-// One workgroup per item.
-template <int TASK>
-__global__ void __launch_bounds__(kBlock) k_v2_scan(DataArgs d, V2Args a) {
-  __shared__ uint32_t s_w[4 * kWaves + kWaves];
-  __shared__ int64_t s_w64[kWaves];
+// One workgroup per item: offsets, last key before each bucket, task lists.
+__global__ void __launch_bounds__(kBlock) k_v2_scan(V2Args a) {
+  __shared__ uint32_t s_w[kWaves];
+  __shared__ uint32_t s_base[2];
   const V2Item it = a.items[blockIdx.x];
   const int t = threadIdx.x;
-  const int C = d.C;
   constexpr int PER = kV2MaxB / kBlock;  // buckets per thread
   const int b0 = t * PER;
   const int64_t gb0 = (int64_t)it.boff + b0;
   uint32_t cnt[PER], mx[PER];
-  uint32_t lcnt = 0, lmax = 0;
+  uint32_t lcnt = 0, lmax = 0, n_norm = 0, n_spill = 0;
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const bool ok = b0 + q < it.nb;
     cnt[q] = ok ? a.bcount[gb0 + q] : 0u;
     mx[q] = ok ? a.bmax[gb0 + q] : 0u;
     lcnt += cnt[q];
-    if (cnt[q]) lmax = max(lmax, mx[q]);
+    if (cnt[q]) {
+      lmax = max(lmax, mx[q]);
+      if (cnt[q] > (uint32_t)kV2WaveCap) ++n_spill;
+      else ++n_norm;
+    }
   }
-  // offsets
   uint32_t off = team_excl_scan<256>(lcnt, s_w, nullptr);
-  // last key before each bucket
   uint32_t last = block_excl_max(lmax, s_w);
-  // weights / targets
-  int64_t run_n;
-  int64_t run_s = 0;
-  if (TASK == 0) {
-    int64_t lw = 0;
-    for (int q = 0; q < PER; ++q)
-      if (b0 + q < it.nb)
-        for (int c = 0; c < C; ++c) lw += a.bcls[(gb0 + q) * C + c];
-    run_n = team_excl_scan_i64<256>(lw, s_w64, nullptr);
-  } else {
-    int64_t lw = 0, ls = 0;
-    for (int q = 0; q < PER; ++q)
-      if (b0 + q < it.nb) {
-        lw += a.bwn[gb0 + q];
-        ls += a.bsum[gb0 + q];
-      }
-    run_n = team_excl_scan_i64<256>(lw, s_w64, nullptr);
-    run_s = team_excl_scan_i64<256>(ls, s_w64, nullptr);
-  }
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    if (b0 + q >= it.nb) break;
-    const int64_t gb = gb0 + q;
-    a.bbase[gb] = off;
-    a.bcur[gb] = off;
-    a.carry_n[gb] = run_n;
-    if (TASK == 1) a.carry_s[gb] = run_s;
-    a.carry_last[gb] = last;
-    off += cnt[q];
-    if (cnt[q]) last = max(last, mx[q]);
-    if (TASK == 0) {
-      int64_t w = 0;
-      for (int c = 0; c < C; ++c) w += a.bcls[gb * C + c];
-      run_n += w;
-    } else {
-      run_n += a.bwn[gb];
-      run_s += a.bsum[gb];
-    }
-  }
-  if (TASK == 0) {
-    // class carries, four classes per scan
-    for (int c0 = 0; c0 < C; c0 += 4) {
-      uint32_t v[4], tot[4];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        v[k] = 0;
-        if (c0 + k < C)
-          for (int q = 0; q < PER; ++q)
-            if (b0 + q < it.nb) v[k] += a.bcls[(gb0 + q) * C + c0 + k];
-      }
-      team_excl_scan_vec<256, 4>(v, s_w, tot);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        if (c0 + k >= C) continue;
-        uint32_t r = v[k];
-        for (int q = 0; q < PER; ++q) {
-          if (b0 + q >= it.nb) break;
-          const int64_t gb = gb0 + q;
-          a.carry_cls[gb * C + c0 + k] = r;
-          r += a.bcls[gb * C + c0 + k];
-        }
-      }
-    }
-  }
-  // task lists: non-empty buckets, overfull ones apart
-  uint32_t n_norm = 0, n_spill = 0;
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    if (!cnt[q]) continue;
-    if (cnt[q] > (uint32_t)kCap)
-      ++n_spill;
-    else
-      ++n_norm;
-  }
   uint32_t tot_norm = 0, tot_spill = 0;
   const uint32_t pn = team_excl_scan<256>(n_norm, s_w, &tot_norm);
   const uint32_t ps = team_excl_scan<256>(n_spill, s_w, &tot_spill);
-  __shared__ uint32_t s_base[2];
   if (t == 0) {
     s_base[0] = tot_norm ? atomicAdd(&a.counters[0], tot_norm) : 0u;
     s_base[1] = tot_spill ? atomicAdd(&a.counters[1], tot_spill) : 0u;
@@ -1062,9 +959,16 @@ __global__ void __launch_bounds__(kBlock) k_v2_scan(DataArgs d, V2Args a) {
   uint32_t in = s_base[0] + pn, is = s_base[1] + ps;
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
+    if (b0 + q >= it.nb) break;
+    const int64_t gb = gb0 + q;
+    a.bbase[gb] = off;
+    a.bcur[gb] = off;
+    a.carry_last[gb] = last;
+    off += cnt[q];
     if (!cnt[q]) continue;
+    last = max(last, mx[q]);
     const V2Task tk{(uint32_t)blockIdx.x, (uint32_t)(b0 + q)};
-    if (cnt[q] > (uint32_t)kCap)
+    if (cnt[q] > (uint32_t)kV2WaveCap)
       a.spill[is++] = tk;
     else
       a.tasks[in++] = tk;
@@ -1155,6 +1059,111 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
   }
 }
 
+// One workgroup per item, one wave per bucket: class totals (regression:
+// weight and target sums) of each bucket's entries.
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_v2_ctot(DataArgs d, V2Args a) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const V2Item it = a.items[blockIdx.x];
+  for (int b = threadIdx.x / kWave; b < it.nb; b += kWaves) {
+    const int64_t gb = (int64_t)it.boff + b;
+    const uint32_t n = a.bcount[gb];
+    const int64_t e0 = it.soff + a.bbase[gb];
+    if constexpr (TASK == 0) {
+      uint32_t loc[MAXC];
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c) loc[c] = 0;
+      for (uint32_t e = lane; e < n; e += kWave) {
+        const uint32_t x = a.saux[e0 + e];
+        const uint32_t lab = x >> 16, w = x & 0xFFFFu;
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) loc[c] += (lab == (uint32_t)c) ? w : 0u;
+      }
+#pragma unroll
+      for (int c = 0; c < MAXC; ++c) {
+        uint32_t v = loc[c];
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+        if (lane == c && c < d.C) a.bcls[gb * d.C + c] = v;
+      }
+    } else {
+      int64_t wn = 0, ws = 0;
+      for (uint32_t e = lane; e < n; e += kWave) {
+        const int64_t w = (int64_t)(a.saux[e0 + e] & 0xFFFFu);
+        wn += w;
+        ws += w * a.sy[e0 + e];
+      }
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        wn += __shfl_xor(wn, off, kWave);
+        ws += __shfl_xor(ws, off, kWave);
+      }
+      if (lane == 0) {
+        a.bwn[gb] = (uint32_t)wn;
+        a.bsum[gb] = ws;
+      }
+    }
+  }
+}
+
+// One workgroup per item: totals before each bucket.
+template <int TASK>
+__global__ void __launch_bounds__(kBlock) k_v2_carry(DataArgs d, V2Args a) {
+  __shared__ uint32_t s_w[8 * kWaves + kWaves];
+  __shared__ int64_t s_w64[kWaves];
+  const V2Item it = a.items[blockIdx.x];
+  const int t = threadIdx.x;
+  const int C = d.C;
+  constexpr int PER = kV2MaxB / kBlock;
+  const int b0 = t * PER;
+  const int64_t gb0 = (int64_t)it.boff + b0;
+  if constexpr (TASK == 0) {
+    int64_t lw = 0;
+    for (int q = 0; q < PER && b0 + q < it.nb; ++q)
+      for (int c = 0; c < C; ++c) lw += a.bcls[(gb0 + q) * C + c];
+    int64_t run_n = team_excl_scan_i64<256>(lw, s_w64, nullptr);
+    for (int q = 0; q < PER && b0 + q < it.nb; ++q) {
+      a.carry_n[gb0 + q] = run_n;
+      for (int c = 0; c < C; ++c) run_n += a.bcls[(gb0 + q) * C + c];
+    }
+    for (int c0 = 0; c0 < C; c0 += 8) {
+      uint32_t v[8], tot[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        v[k] = 0;
+        if (c0 + k < C)
+          for (int q = 0; q < PER && b0 + q < it.nb; ++q) v[k] += a.bcls[(gb0 + q) * C + c0 + k];
+      }
+      team_excl_scan_vec<256, 8>(v, s_w, tot);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (c0 + k >= C) continue;
+        uint32_t r = v[k];
+        for (int q = 0; q < PER && b0 + q < it.nb; ++q) {
+          const int64_t gb = gb0 + q;
+          a.carry_cls[gb * C + c0 + k] = r;
+          r += a.bcls[gb * C + c0 + k];
+        }
+      }
+    }
+  } else {
+    int64_t lw = 0, ls = 0;
+    for (int q = 0; q < PER && b0 + q < it.nb; ++q) {
+      lw += a.bwn[gb0 + q];
+      ls += a.bsum[gb0 + q];
+    }
+    int64_t run_n = team_excl_scan_i64<256>(lw, s_w64, nullptr);
+    int64_t run_s = team_excl_scan_i64<256>(ls, s_w64, nullptr);
+    for (int q = 0; q < PER && b0 + q < it.nb; ++q) {
+      const int64_t gb = gb0 + q;
+      a.carry_n[gb] = run_n;
+      a.carry_s[gb] = run_s;
+      run_n += a.bwn[gb];
+      run_s += a.bsum[gb];
+    }
+  }
+}
+
 template <int MAXC, int TASK, int IPT>
 __device__ __forceinline__ void v2_carry(const DataArgs& d, const V2Args& a, int64_t gb,
                                          TeamLds<MAXC, TASK, 256, IPT>& L) {
@@ -1184,33 +1193,32 @@ __device__ __forceinline__ NodeCtx v2_node(const DataArgs& d, const LevelArgs& l
   return nc;
 }
 
-// One workgroup per bucket of at most kCap entries, grid-stride over the
-// task list.  The loads of the next bucket (task, item and bucket fields,
-// carries, entries) are issued before the current bucket is sorted and
-// walked, so their latency hides behind that work.
+// One wave per bucket of at most kV2WaveCap entries (four independent
+// buckets per workgroup), grid-stride over the task list.  The loads of a
+// wave's next bucket (task, item and bucket fields, carries, entries) are
+// issued before its current bucket is sorted and walked.
 template <int TASK, int IPT>
 struct V2Pre {
-  int64_t gb, e0, ridx_unused;
-  uint32_t node, kbase, bmax, shift;
+  int64_t gb, e0;
+  uint32_t node, kbase, bmax;
   int n;
   int64_t n_w, P, ssum;
   double inv;
-  uint32_t carry_c;  // class carry of class threadIdx.x (lanes < C)
+  uint32_t carry_c;  // class carry of class `lane` (lanes < C)
   int64_t carry_n, carry_s;
   uint32_t carry_last;
-  uint32_t ncls;     // node class total of class threadIdx.x
+  uint32_t ncls;     // node class total of class `lane`
   uint32_t key[IPT], aux[IPT];
   int64_t y[TASK == 1 ? IPT : 1];
 };
 
-template <int MAXC, int TASK, int IPT>
+template <int TASK, int IPT>
 __device__ __forceinline__ void v2_prefetch(const DataArgs& d, const LevelArgs& lv, const V2Args& a, int64_t q,
                                             V2Pre<TASK, IPT>& P) {
-  const int t = threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
   const V2Task tk = a.tasks[q];
   const V2Item& it = a.items[tk.item];
   P.gb = (int64_t)it.boff + tk.b;
-  P.shift = (uint32_t)it.shift;
   P.kbase = tk.b << it.shift;
   P.node = it.node;
   P.n_w = it.n_w;
@@ -1223,11 +1231,11 @@ __device__ __forceinline__ void v2_prefetch(const DataArgs& d, const LevelArgs& 
   P.carry_n = a.carry_n[P.gb];
   P.carry_s = TASK == 1 ? a.carry_s[P.gb] : 0;
   P.carry_last = a.carry_last[P.gb];
-  P.carry_c = (TASK == 0 && t < d.C) ? a.carry_cls[P.gb * d.C + t] : 0u;
-  P.ncls = (TASK == 0 && t < d.C) ? lv.cls[(int64_t)P.node * d.C + t] : 0u;
+  P.carry_c = (TASK == 0 && lane < d.C) ? a.carry_cls[P.gb * d.C + lane] : 0u;
+  P.ncls = (TASK == 0 && lane < d.C) ? lv.cls[(int64_t)P.node * d.C + lane] : 0u;
 #pragma unroll
   for (int k = 0; k < IPT; ++k) {
-    const int e = t + k * 256;
+    const int e = lane + k * kWave;
     if (e < P.n) {
       P.key[k] = a.skey[P.e0 + e];
       P.aux[k] = a.saux[P.e0 + e];
@@ -1240,17 +1248,18 @@ template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_v2_eval(DataArgs d, LevelArgs lv, V2Args a, int64_t n_tasks) {
   constexpr int IPT = kIpt;
   __shared__ double s_inv[128], s_logc[128];
-  __shared__ TeamLds<MAXC, TASK, 256, IPT> L;
+  __shared__ TeamLds<MAXC, TASK, kWave, IPT> s_team[kWaves];
   load_tables(s_inv, s_logc);
   __syncthreads();
   const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
-  const int t = threadIdx.x;
-  int64_t q = blockIdx.x;
-  if (q >= n_tasks) return;
+  const int t = threadIdx.x & (kWave - 1);
+  TeamLds<MAXC, TASK, kWave, IPT>& L = s_team[threadIdx.x / kWave];
+  const int64_t stride = (int64_t)gridDim.x * kWaves;
+  int64_t q = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
+  if (q >= n_tasks) return;  // wave-uniform; no workgroup barrier follows
   V2Pre<TASK, IPT> cur;
-  v2_prefetch<MAXC, TASK, IPT>(d, lv, a, q, cur);
+  v2_prefetch<TASK, IPT>(d, lv, a, q, cur);
   while (q < n_tasks) {
-    // stage the current bucket
     NodeCtx nc;
     nc.C = d.C;
     nc.n = cur.n_w;
@@ -1262,9 +1271,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_eval(DataArgs d, LevelArgs lv, V2
       L.cr.run[t] = cur.carry_c;
     }
     if (t == 0) {
-      for (int c = d.C; c < MAXC; ++c) L.cr.run[c] = 0u;
-      if (TASK == 1)
-        for (int c = 0; c < MAXC; ++c) L.cr.run[c] = 0u;
+      for (int c = TASK == 0 ? d.C : 0; c < MAXC; ++c) L.cr.run[c] = 0u;
       L.cr.run_n = cur.carry_n;
       L.cr.run_s = cur.carry_s;
       L.cr.last_key = cur.carry_last;
@@ -1275,7 +1282,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_eval(DataArgs d, LevelArgs lv, V2
     const int n = cur.n;
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
-      const int e = t + k * 256;
+      const int e = t + k * kWave;
       if (e < n) {
         L.u.srt.k[0][e] = cur.key[k] - cur.kbase;
         L.u.srt.v[0][e] = (uint16_t)e;
@@ -1286,18 +1293,17 @@ __global__ void __launch_bounds__(kBlock) k_v2_eval(DataArgs d, LevelArgs lv, V2
     const int64_t gb = cur.gb;
     const uint32_t kbase = cur.kbase;
     const int bits = bit_length(cur.bmax - kbase);
-    __syncthreads();
-    // issue the next bucket's loads
-    const int64_t qn = q + gridDim.x;
-    if (qn < n_tasks) v2_prefetch<MAXC, TASK, IPT>(d, lv, a, qn, cur);
-    sort_and_eval<MAXC, TASK, 256, IPT>(n, bits, kbase, nc, L, lx);
-    __syncthreads();
+    team_sync<kWave>();
+    const int64_t qn = q + stride;
+    if (qn < n_tasks) v2_prefetch<TASK, IPT>(d, lv, a, qn, cur);
+    sort_and_eval<MAXC, TASK, kWave, IPT>(n, bits, kbase, nc, L, lx);
+    team_sync<kWave>();
     if (t == 0) {
       a.res_g[gb] = L.cr.best_g;
       a.res_key[gb] = L.cr.best_key;
       a.res_prev[gb] = L.cr.best_prev;
     }
-    __syncthreads();
+    team_sync<kWave>();
     q = qn;
   }
 }
@@ -2861,10 +2867,8 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
                   int64_t n_groups, hipStream_t s, int num_cu) {
   const int64_t n_all = n_groups * it.per_group;
   if (n_all == 0) return;
+  const int nb_cap = kV2MaxB;
   const int C = d.C;
-  int nb_cap = kV2MaxB;
-  if (TASK == 0)
-    while (nb_cap * C > kV2ClsWords) nb_cap >>= 1;
   B.items.reserve(n_all);
   hipLaunchKernelGGL(k_v2_items, dim3(grid_for(n_all)), dim3(kBlock), 0, s, d, lv, groups, n_groups, it.per_group,
                      it.j0, it.D, nb_cap, B.items.p);
@@ -2875,7 +2879,6 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
   int64_t k0 = 0;
   while (k0 < n_all) {
     int64_t k1 = k0, el = 0, nb = 0;
-    int nb_max = 1;
     B.host_htiles.clear();
     B.host_stiles.clear();
     while (k1 < n_all && (k1 == k0 || el + B.host_items[k1].m <= kV2Batch)) {
@@ -2887,7 +2890,6 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
       for (int e0 = 0; e0 < bi.m; e0 += kSTile) B.host_stiles.push_back(V2Tile{li, (uint32_t)e0});
       el += bi.m;
       nb += bi.nb;
-      nb_max = std::max(nb_max, bi.nb);
       ++k1;
     }
     const int64_t n_items = k1 - k0;
@@ -2928,12 +2930,6 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     }
     DTE_HIP(hipMemsetAsync(B.bcount.p, 0, 4 * nb, s));
     DTE_HIP(hipMemsetAsync(B.bmax.p, 0, 4 * nb, s));
-    if (TASK == 0) {
-      DTE_HIP(hipMemsetAsync(B.bcls.p, 0, 4 * (size_t)nb * C, s));
-    } else {
-      DTE_HIP(hipMemsetAsync(B.bwn.p, 0, 4 * nb, s));
-      DTE_HIP(hipMemsetAsync(B.bsum.p, 0, 8 * nb, s));
-    }
     DTE_HIP(hipMemsetAsync(B.ctr.p, 0, 8, s));
     V2Args a{};
     a.items = B.items.p;
@@ -2962,16 +2958,18 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     a.spill = B.spill.p;
     a.counters = B.ctr.p;
     a.tiles = B.htiles.p;
-    const size_t dyn = TASK == 0 ? sizeof(uint32_t) * (size_t)nb_max * C : 0;
-    hipLaunchKernelGGL(k_v2_hist<TASK>, dim3((unsigned)B.host_htiles.size()), dim3(kBlock), dyn, s, d, lv, a);
-    hipLaunchKernelGGL(k_v2_scan<TASK>, dim3((unsigned)n_items), dim3(kBlock), 0, s, d, a);
+    hipLaunchKernelGGL(k_v2_hist<TASK>, dim3((unsigned)B.host_htiles.size()), dim3(kBlock), 0, s, d, lv, a);
+    hipLaunchKernelGGL(k_v2_scan, dim3((unsigned)n_items), dim3(kBlock), 0, s, a);
     a.tiles = B.stiles.p;
     hipLaunchKernelGGL(k_v2_scatter<TASK>, dim3((unsigned)B.host_stiles.size()), dim3(kBlock), 0, s, d, lv, a);
+    hipLaunchKernelGGL((k_v2_ctot<MAXC, TASK>), dim3((unsigned)n_items), dim3(kBlock), 0, s, d, a);
+    hipLaunchKernelGGL(k_v2_carry<TASK>, dim3((unsigned)n_items), dim3(kBlock), 0, s, d, a);
     unsigned int nt[2];
     DTE_HIP(hipMemcpyAsync(nt, B.ctr.p, 8, hipMemcpyDeviceToHost, s));
     DTE_HIP(hipStreamSynchronize(s));
     if (nt[0])
-      hipLaunchKernelGGL((k_v2_eval<MAXC, TASK>), dim3((unsigned)std::min<int64_t>(nt[0], (int64_t)num_cu * 8)),
+      hipLaunchKernelGGL((k_v2_eval<MAXC, TASK>),
+                         dim3((unsigned)std::min<int64_t>((nt[0] + kWaves - 1) / kWaves, (int64_t)num_cu * 8)),
                          dim3(kBlock), 0, s, d, lv, a, (int64_t)nt[0]);
     if (nt[1])
       hipLaunchKernelGGL((k_v2_spill<MAXC, TASK>), dim3((unsigned)std::min<int64_t>(nt[1], (int64_t)num_cu * 4)),
