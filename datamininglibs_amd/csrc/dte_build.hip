@@ -288,11 +288,14 @@ __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, in
       }
     }
     int64_t sc[4] = {0, 0, 0, 0};  // lane deltas of sl, sr; sums at the chunk start
+    int64_t dl[IPT], dr[IPT];
 #pragma unroll
     for (int i = 0; i < IPT; ++i) {
       if (lo + i < hi) {
-        sc[0] += lx.t(lb[i] + wv[i]) - lx.t(lb[i]);
-        sc[1] += lx.t(tc[i] - lb[i] - wv[i]) - lx.t(tc[i] - lb[i]);
+        dl[i] = lx.t(lb[i] + wv[i]) - lx.t(lb[i]);
+        dr[i] = lx.t(tc[i] - lb[i] - wv[i]) - lx.t(tc[i] - lb[i]);
+        sc[0] += dl[i];
+        sc[1] += dr[i];
       }
     }
     if (t < nc.C) {  // one class per lane
@@ -314,8 +317,8 @@ __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, in
             const double g = ig_from_fixed(nc.P + q, lx.F, nc.inv);
             if (g > tb.g) tb = BestCand{g, k, prevk};
           }
-          sl += lx.t(lb[i] + wv[i]) - lx.t(lb[i]);
-          sr += lx.t(tc[i] - lb[i] - wv[i]) - lx.t(tc[i] - lb[i]);
+          sl += dl[i];
+          sr += dr[i];
           prevk = k;
         }
       }
