@@ -1062,13 +1062,18 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
   }
 }
 
-// One workgroup per item, one wave per bucket: class totals (regression:
-// weight and target sums) of each bucket's entries.
+// One workgroup per tile of kV2CTile buckets of an item, one wave per
+// bucket: class totals (regression: weight and target sums) of each
+// bucket's entries.
+constexpr int kV2CTile = 64;
+
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_v2_ctot(DataArgs d, V2Args a) {
   const int lane = threadIdx.x & (kWave - 1);
-  const V2Item it = a.items[blockIdx.x];
-  for (int b = threadIdx.x / kWave; b < it.nb; b += kWaves) {
+  const V2Tile tl = a.tiles[blockIdx.x];
+  const V2Item it = a.items[tl.item];
+  const int b_end = min(it.nb, (int)tl.e0 + kV2CTile);
+  for (int b = (int)tl.e0 + threadIdx.x / kWave; b < b_end; b += kWaves) {
     const int64_t gb = (int64_t)it.boff + b;
     const uint32_t n = a.bcount[gb];
     const int64_t e0 = it.soff + a.bbase[gb];
@@ -2849,7 +2854,7 @@ void launch_partition_wave(int grid, hipStream_t s, const DataArgs& d, const Lev
 // Scratch and bookkeeping of the big-node pipeline, reused across levels.
 struct BigState {
   DevBuf<V2Item> items;
-  DevBuf<V2Tile> htiles, stiles;
+  DevBuf<V2Tile> htiles, stiles, ctiles;
   DevBuf<uint32_t> bcount, bmax, bcls, bwn, bbase, bcur, carry_cls, carry_last, res_key, res_prev;
   DevBuf<int64_t> bsum, carry_n, carry_s;
   DevBuf<double> res_g;
@@ -2858,7 +2863,7 @@ struct BigState {
   DevBuf<V2Task> tasks, spill;
   DevBuf<unsigned int> ctr;
   std::vector<V2Item> host_items;
-  std::vector<V2Tile> host_htiles, host_stiles;
+  std::vector<V2Tile> host_htiles, host_stiles, host_ctiles;
 };
 
 // Chunk-parallel search of the big items of one phase (see k_v2_*): items in
@@ -2884,6 +2889,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     int64_t k1 = k0, el = 0, nb = 0;
     B.host_htiles.clear();
     B.host_stiles.clear();
+    B.host_ctiles.clear();
     while (k1 < n_all && (k1 == k0 || el + B.host_items[k1].m <= kV2Batch)) {
       V2Item& bi = B.host_items[k1];
       bi.boff = (int32_t)nb;
@@ -2891,6 +2897,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
       const uint32_t li = (uint32_t)(k1 - k0);
       for (int e0 = 0; e0 < bi.m; e0 += kV2Tile) B.host_htiles.push_back(V2Tile{li, (uint32_t)e0});
       for (int e0 = 0; e0 < bi.m; e0 += kSTile) B.host_stiles.push_back(V2Tile{li, (uint32_t)e0});
+      for (int b0 = 0; b0 < bi.nb; b0 += kV2CTile) B.host_ctiles.push_back(V2Tile{li, (uint32_t)b0});
       el += bi.m;
       nb += bi.nb;
       ++k1;
@@ -2899,6 +2906,9 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     DTE_HIP(hipMemcpyAsync(B.items.p, B.host_items.data() + k0, sizeof(V2Item) * n_items, hipMemcpyHostToDevice, s));
     B.htiles.reserve(B.host_htiles.size());
     B.stiles.reserve(B.host_stiles.size());
+    B.ctiles.reserve(B.host_ctiles.size());
+    DTE_HIP(hipMemcpyAsync(B.ctiles.p, B.host_ctiles.data(), sizeof(V2Tile) * B.host_ctiles.size(),
+                           hipMemcpyHostToDevice, s));
     DTE_HIP(hipMemcpyAsync(B.htiles.p, B.host_htiles.data(), sizeof(V2Tile) * B.host_htiles.size(),
                            hipMemcpyHostToDevice, s));
     DTE_HIP(hipMemcpyAsync(B.stiles.p, B.host_stiles.data(), sizeof(V2Tile) * B.host_stiles.size(),
@@ -2965,7 +2975,8 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     hipLaunchKernelGGL(k_v2_scan, dim3((unsigned)n_items), dim3(kBlock), 0, s, a);
     a.tiles = B.stiles.p;
     hipLaunchKernelGGL(k_v2_scatter<TASK>, dim3((unsigned)B.host_stiles.size()), dim3(kBlock), 0, s, d, lv, a);
-    hipLaunchKernelGGL((k_v2_ctot<MAXC, TASK>), dim3((unsigned)n_items), dim3(kBlock), 0, s, d, a);
+    a.tiles = B.ctiles.p;
+    hipLaunchKernelGGL((k_v2_ctot<MAXC, TASK>), dim3((unsigned)B.host_ctiles.size()), dim3(kBlock), 0, s, d, a);
     hipLaunchKernelGGL(k_v2_carry<TASK>, dim3((unsigned)n_items), dim3(kBlock), 0, s, d, a);
     unsigned int nt[2];
     DTE_HIP(hipMemcpyAsync(nt, B.ctr.p, 8, hipMemcpyDeviceToHost, s));
