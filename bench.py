@@ -42,8 +42,9 @@ KERNEL_SYMBOL = {
     "search_seg16": "dte::(anonymous namespace)::k_search_rf_seg<16, 0, 16>",
     "search_seg32": "dte::(anonymous namespace)::k_search_rf_seg<16, 0, 32>",
     "search_seg64": "dte::(anonymous namespace)::k_search_rf_seg<16, 0, 64>",
-    "search_team64": "dte::(anonymous namespace)::k_search_rf<16, 0, 64>",
-    "search_block": "dte::(anonymous namespace)::k_search_rf<16, 0, 256>",
+    "search_team64": "dte::(anonymous namespace)::k_search_rf<16, 0, 64, false>",
+    "search_block": "dte::(anonymous namespace)::k_search_rf<16, 0, 256, false>",
+    "chunk_eval": "dte::(anonymous namespace)::k_v2_eval<16, 0>",
 }
 
 
@@ -183,6 +184,94 @@ def serve_bench(capi, synth, np, n_requests, seed=0):
             "batches": st["batches_launched"], "resident_nodes": nodes, "hbm_in_use_bytes": st["hbm_in_use"]}
 
 
+def other_workload(name, args):
+    """Measurement lines of the other workload shapes (not the headline)."""
+    import numpy as np
+    from datamininglibs_amd import capi, synth
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cores = args.cpu_cores or min(16, os.cpu_count() or 1)
+
+    def timed_fit(X, y, p, reps):
+        f = capi.fit(X, y, p)  # warm-up (host upload included in every call)
+        f.close()
+        Xd = capi.DeviceArray.from_numpy(X)
+        t = time.perf_counter()
+        for _ in range(reps):
+            f = capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p)
+            nodes = f.n_nodes
+            f.close()
+        dt = (time.perf_counter() - t) / reps
+        Xd.free()
+        log(f"{name}: {p.n_trees} trees ({'plain' if p.build_mode else 'batched'}) in {dt:.3f} s per fit")
+        return dt, nodes
+
+    if name == "c1":
+        X, y = synth.spam_like(seed=0)
+        kw = dict(n_trees=100, n_features_per_node=7, max_depth=25, min_node_size=10, n_classes=2, seed=0)
+        dt, nodes = timed_fit(X, y, capi.default_params(capi.EXTRA_TREES, **kw), 5)
+        dtp, _ = timed_fit(X, y, capi.default_params(capi.EXTRA_TREES, build_mode=capi.BUILD_PLAIN, **kw), 2)
+        t = time.perf_counter()
+        pyoracle.fit(X, y, algorithm=1, n_trees=100, k=7, max_depth=25, min_node_size=10, n_classes=2, seed=0,
+                     n_threads=cores)
+        dto = time.perf_counter() - t
+        line = {"workload": "extra-trees classifier 4,601 x 57 (spam-like), 100 trees, k 7, depth 25, min node 10",
+                "gpu_trees_per_s": round(100 / dt, 1), "gpu_plain_trees_per_s": round(100 / dtp, 1),
+                "cpu_trees_per_s": round(100 / dto, 1), "cpu_cores": cores, "nodes": nodes}
+    elif name == "c3":
+        X, y = synth.sine_regression(seed=0, n=4_000_000, f=64)
+        kw = dict(task=capi.REGRESSION, n_trees=100, n_features_per_node=21, max_depth=18, min_node_size=5,
+                  n_classes=0, seed=7)
+        capi.profile_reset()
+        capi.profiling(True)
+        dt, nodes = timed_fit(X, y, capi.default_params(capi.RANDOM_FOREST, **kw), 2)
+        capi.profiling(False)
+        st = capi.profile_stats()
+        kw10 = dict(kw, n_trees=10)  # the plain driver on a 10-tree slice
+        dtp, _ = timed_fit(X, y, capi.default_params(capi.RANDOM_FOREST, build_mode=capi.BUILD_PLAIN, **kw10), 1)
+        dtp *= 10.0
+        log(f"c3: cpu oracle, {cores} trees on {cores} threads")
+        t = time.perf_counter()
+        pyoracle.fit(X, y, algorithm=0, task=1, n_trees=cores, k=21, max_depth=18, min_node_size=5, seed=7,
+                     n_threads=cores)
+        dto = time.perf_counter() - t
+        ss = st.get("split_search", {"total_ms": 0, "bytes": 0})
+        line = {"workload": "random-forest regression 4,000,000 x 64, 100 trees, variance, k 21, depth 18",
+                "gpu_trees_per_s": round(100 / dt, 2), "gpu_plain_trees_per_s": round(100 / dtp, 2),
+                "plain_sample": "10 of the 100 trees",
+                "cpu_trees_per_s": round(cores / dto, 4), "cpu_cores": cores, "nodes": nodes,
+                "split_search_GBps": round(ss["bytes"] / (ss["total_ms"] * 1e-3) / 1e9, 1) if ss["total_ms"] else None,
+                "kernel_ms_per_fit": {k: round(v["total_ms"] / 3, 2) for k, v in st.items()}}
+    else:  # c4
+        specs = synth.uneven_forests(seed=0)
+        data = [synth.uneven_forest_data(sp) for sp in specs]
+        params = [capi.default_params(capi.EXTRA_TREES, n_trees=50, n_classes=sp["classes"], seed=sp["seed"])
+                  for sp in specs]
+        devs = [capi.DeviceArray.from_numpy(X) for X, _ in data]
+        for (X, y), Xd, p in zip(data, devs, params):
+            capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p).close()
+        t = time.perf_counter()
+        nodes = 0
+        for (X, y), Xd, p in zip(data, devs, params):
+            f = capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p)
+            nodes += f.n_nodes
+            f.close()
+        dt = time.perf_counter() - t
+        small = sorted(range(64), key=lambda i: specs[i]["n"] * specs[i]["f"])[:8]
+        t = time.perf_counter()
+        for i in small:
+            X, y = data[i]
+            pyoracle.fit(X, y, algorithm=1, n_trees=50, n_classes=specs[i]["classes"], seed=specs[i]["seed"],
+                         n_threads=cores)
+        dto = time.perf_counter() - t
+        ntr = sum(specs[i]["n"] for i in small)
+        line = {"workload": "64 extra-trees forests, 2k..200k rows, 16..256 features, 2..8 classes, 50 deep trees each",
+                "gpu_trees_per_s": round(64 * 50 / dt, 1), "gpu_seconds_all": round(dt, 2), "nodes": nodes,
+                "cpu_sample": f"8 smallest forests ({ntr} rows total), {dto:.1f} s on {cores} threads",
+                "cpu_trees_per_s_sample": round(8 * 50 / dto, 1)}
+    print(json.dumps({"workload_name": name, **line}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,11 +285,16 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true", help="also run a 1 GiB copy kernel")
     ap.add_argument("--serve-requests", type=int, default=1000, help="requests per serving QPS point (0: skip)")
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
+                    help="c2 (default): the headline random-forest line; c1/c3/c4: the other workload shapes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: this build measures the single-GPU share of the workload; "
                                    "multi-GPU tree sharding is a design note, not built", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload != "c2":
+        other_workload(args.workload, args)
+        return
 
     import ctypes
 
@@ -263,7 +357,10 @@ def main():
     pred_bytes = n_pred * args.features * 4 + forest.n_nodes * 16 + n_pred * 10 * 4
 
     # Roofline of the dominant split-search kernel.
-    fam = {k: v_ for k, v_ in stats.items() if k.startswith("search_")}
+    # single-kernel families that carry algorithmic bytes (the chunk
+    # pipeline's evaluation kernel carries its items' bytes)
+    fam = {k: v_ for k, v_ in stats.items()
+           if (k.startswith("search_") and k != "search_chunked") or k == "chunk_eval"}
     dom = max(fam, key=lambda k: fam[k]["total_ms"])
     ds = fam[dom]
     avg_ms = ds["total_ms"] / max(1, ds["launches"])

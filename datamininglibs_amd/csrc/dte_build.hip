@@ -2971,23 +2971,45 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     a.spill = B.spill.p;
     a.counters = B.ctr.p;
     a.tiles = B.htiles.p;
-    hipLaunchKernelGGL(k_v2_hist<TASK>, dim3((unsigned)B.host_htiles.size()), dim3(kBlock), 0, s, d, lv, a);
-    hipLaunchKernelGGL(k_v2_scan, dim3((unsigned)n_items), dim3(kBlock), 0, s, a);
+    {
+      Timer tm("chunk_hist", 0.0, s);
+      hipLaunchKernelGGL(k_v2_hist<TASK>, dim3((unsigned)B.host_htiles.size()), dim3(kBlock), 0, s, d, lv, a);
+    }
+    {
+      Timer tm("chunk_scan", 0.0, s);
+      hipLaunchKernelGGL(k_v2_scan, dim3((unsigned)n_items), dim3(kBlock), 0, s, a);
+    }
     a.tiles = B.stiles.p;
-    hipLaunchKernelGGL(k_v2_scatter<TASK>, dim3((unsigned)B.host_stiles.size()), dim3(kBlock), 0, s, d, lv, a);
+    {
+      Timer tm("chunk_scatter", 0.0, s);
+      hipLaunchKernelGGL(k_v2_scatter<TASK>, dim3((unsigned)B.host_stiles.size()), dim3(kBlock), 0, s, d, lv, a);
+    }
     a.tiles = B.ctiles.p;
-    hipLaunchKernelGGL((k_v2_ctot<MAXC, TASK>), dim3((unsigned)B.host_ctiles.size()), dim3(kBlock), 0, s, d, a);
-    hipLaunchKernelGGL(k_v2_carry<TASK>, dim3((unsigned)n_items), dim3(kBlock), 0, s, d, a);
+    {
+      Timer tm("chunk_ctot", 0.0, s);
+      hipLaunchKernelGGL((k_v2_ctot<MAXC, TASK>), dim3((unsigned)B.host_ctiles.size()), dim3(kBlock), 0, s, d, a);
+    }
+    {
+      Timer tm("chunk_carry", 0.0, s);
+      hipLaunchKernelGGL(k_v2_carry<TASK>, dim3((unsigned)n_items), dim3(kBlock), 0, s, d, a);
+    }
     unsigned int nt[2];
     DTE_HIP(hipMemcpyAsync(nt, B.ctr.p, 8, hipMemcpyDeviceToHost, s));
     DTE_HIP(hipStreamSynchronize(s));
-    if (nt[0])
+    // Algorithmic bytes of the batch's items (12 B per entry, 20 B for
+    // regression) belong to the evaluation kernel, where every entry is read
+    // and scored; the other pipeline kernels are the price of bucketing.
+    if (nt[0]) {
+      Timer tm("chunk_eval", profiling_enabled() ? (double)el * (TASK == 0 ? 12.0 : 20.0) : 0.0, s);
       hipLaunchKernelGGL((k_v2_eval<MAXC, TASK>),
                          dim3((unsigned)std::min<int64_t>((nt[0] + kWaves - 1) / kWaves, (int64_t)num_cu * 8)),
                          dim3(kBlock), 0, s, d, lv, a, (int64_t)nt[0]);
-    if (nt[1])
+    }
+    if (nt[1]) {
+      Timer tm("chunk_spill", 0.0, s);
       hipLaunchKernelGGL((k_v2_spill<MAXC, TASK>), dim3((unsigned)std::min<int64_t>(nt[1], (int64_t)num_cu * 4)),
                          dim3(kBlock), 0, s, d, lv, a, (int64_t)nt[1]);
+    }
     hipLaunchKernelGGL(k_v2_reduce<TASK>, dim3((unsigned)((n_items + kWaves - 1) / kWaves)), dim3(kBlock), 0, s, d,
                        a, it);
     if (it.bytes) {
