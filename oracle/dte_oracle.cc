@@ -26,6 +26,8 @@
 #include <atomic>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -421,7 +423,11 @@ void build_tree(const Dataset& d, const oracle_params& p, int k, uint32_t t, Tre
     level.push_back(std::move(root));
   }
   uint32_t next_id = 1;
+  static const bool progress = std::getenv("DTE_ORACLE_PROGRESS") != nullptr;
+  int depth_done = 0;
   while (!level.empty()) {
+    if (progress) std::fprintf(stderr, "[oracle] tree %u depth %d: %zu nodes\n", t, depth_done, level.size());
+    ++depth_done;
     std::vector<Node> next;
     for (Node& nd : level) {
       Stats st = node_stats(d, nd);
