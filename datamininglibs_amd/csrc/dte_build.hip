@@ -783,7 +783,8 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
 // of consecutive keys, bucket = key >> shift, sized so that a bucket holds
 // about kV2Target entries: one wave-sized chunk.  Every bucket of every item
 // of a level is then evaluated in parallel:
-//   1. k_v2_hist     per (item, tile): bucket counts and max key
+//   1. k_v2_hist     per (item, tile): bucket counts and max key; the
+//                    gathered keys are kept in entry order for the scatter
 //   2. k_v2_scan     per item: bucket offsets, last key before each bucket,
 //                    task lists
 //   3. k_v2_scatter  per (item, tile): entries into bucket order, staged in
@@ -854,6 +855,7 @@ struct V2Args {
   uint32_t* saux;
   int64_t* sy;
   uint32_t* skey2;  // overfull buckets: fine sub-bucket order
+  uint32_t* gkey;   // keys in entry order (histogram -> scatter; aliases skey2)
   uint32_t* saux2;
   int64_t* sy2;
   // task lists
@@ -862,9 +864,11 @@ struct V2Args {
   unsigned int* counters;  // [0] tasks, [1] spill
 };
 
-// Exclusive max-scan over a 256-thread workgroup; s_w holds kWaves words.
-__device__ __forceinline__ uint32_t block_excl_max(uint32_t v, uint32_t* s_w) {
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+// Exclusive max-scan over a team (one wave, or the 256-thread workgroup with
+// kWaves words in s_w).
+template <int TEAM>
+__device__ __forceinline__ uint32_t team_excl_max(uint32_t v, uint32_t* s_w) {
+  const int lane = threadIdx.x & (kWave - 1);
   uint32_t inc = v;
 #pragma unroll
   for (int off = 1; off < kWave; off <<= 1) {
@@ -873,13 +877,29 @@ __device__ __forceinline__ uint32_t block_excl_max(uint32_t v, uint32_t* s_w) {
   }
   uint32_t exc = __shfl_up(inc, 1, kWave);
   if (lane == 0) exc = 0;
-  if (lane == kWave - 1) s_w[w] = inc;
-  __syncthreads();
-  uint32_t pre = 0;
+  if constexpr (TEAM == kWave) {
+    return exc;
+  } else {
+    const int w = threadIdx.x / kWave;
+    if (lane == kWave - 1) s_w[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0;
 #pragma unroll
-  for (int i = 0; i < kWaves; ++i) pre = (i < w) ? max(pre, s_w[i]) : pre;
-  __syncthreads();
-  return max(pre, exc);
+    for (int i = 0; i < kWaves; ++i) pre = (i < w) ? max(pre, s_w[i]) : pre;
+    __syncthreads();
+    return max(pre, exc);
+  }
+}
+
+// Items with at most kV2SmallB buckets take one wave in the per-item passes
+// (offsets, carries); larger ones a workgroup.
+constexpr int kV2SmallB = 256;
+
+// The item of a per-item team: list[team], teams of TEAM threads.
+template <int TEAM>
+__device__ __forceinline__ int64_t v2_team_item(const uint32_t* list, int64_t n_list) {
+  const int64_t k = TEAM == kWave ? (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave : (int64_t)blockIdx.x;
+  return k < n_list ? (int64_t)list[k] : -1;
 }
 
 template <int TASK>
@@ -897,6 +917,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2
   const uint32_t* Kf = d.K + (int64_t)it.f * d.N;
   const int n = min(kV2Tile, it.m - (int)tl.e0);
   const int64_t e0 = it.start + tl.e0;
+  uint32_t* gkey = a.gkey + it.soff + tl.e0;
   constexpr int U = 8;
   for (int base = 0; base < n; base += kBlock * U) {
     uint32_t rows[U];
@@ -910,6 +931,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2
       const int e = base + t + q * kBlock;
       if (e < n) {
         const uint32_t key = Kf[rows[q]];
+        gkey[e] = key;
         const uint32_t b = key >> it.shift;
         atomicAdd(&s_cnt[b], 1u);
         atomicMax(&s_max[b], key);
@@ -926,13 +948,17 @@ __global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2
   }
 }
 
-// One workgroup per item: offsets, last key before each bucket, task lists.
-__global__ void __launch_bounds__(kBlock) k_v2_scan(V2Args a) {
+// One team per item: offsets, last key before each bucket, task lists.
+template <int TEAM>
+__global__ void __launch_bounds__(kBlock) k_v2_scan(V2Args a, const uint32_t* list, int64_t n_list) {
   __shared__ uint32_t s_w[kWaves];
-  __shared__ uint32_t s_base[2];
-  const V2Item it = a.items[blockIdx.x];
-  const int t = threadIdx.x;
-  constexpr int PER = kV2MaxB / kBlock;  // buckets per thread
+  __shared__ uint32_t s_base[kWaves][2];
+  const int64_t k = v2_team_item<TEAM>(list, n_list);
+  if (k < 0) return;  // team-uniform; wave teams have no workgroup barrier
+  const V2Item it = a.items[k];
+  const int t = team_tid<TEAM>();
+  uint32_t* sb = s_base[TEAM == kWave ? threadIdx.x / kWave : 0];
+  constexpr int PER = (TEAM == kWave ? kV2SmallB : kV2MaxB) / TEAM;  // buckets per thread
   const int b0 = t * PER;
   const int64_t gb0 = (int64_t)it.boff + b0;
   uint32_t cnt[PER], mx[PER];
@@ -949,17 +975,17 @@ __global__ void __launch_bounds__(kBlock) k_v2_scan(V2Args a) {
       else ++n_norm;
     }
   }
-  uint32_t off = team_excl_scan<256>(lcnt, s_w, nullptr);
-  uint32_t last = block_excl_max(lmax, s_w);
+  uint32_t off = team_excl_scan<TEAM>(lcnt, s_w, nullptr);
+  uint32_t last = team_excl_max<TEAM>(lmax, s_w);
   uint32_t tot_norm = 0, tot_spill = 0;
-  const uint32_t pn = team_excl_scan<256>(n_norm, s_w, &tot_norm);
-  const uint32_t ps = team_excl_scan<256>(n_spill, s_w, &tot_spill);
+  const uint32_t pn = team_excl_scan<TEAM>(n_norm, s_w, &tot_norm);
+  const uint32_t ps = team_excl_scan<TEAM>(n_spill, s_w, &tot_spill);
   if (t == 0) {
-    s_base[0] = tot_norm ? atomicAdd(&a.counters[0], tot_norm) : 0u;
-    s_base[1] = tot_spill ? atomicAdd(&a.counters[1], tot_spill) : 0u;
+    sb[0] = tot_norm ? atomicAdd(&a.counters[0], tot_norm) : 0u;
+    sb[1] = tot_spill ? atomicAdd(&a.counters[1], tot_spill) : 0u;
   }
-  __syncthreads();
-  uint32_t in = s_base[0] + pn, is = s_base[1] + ps;
+  team_sync<TEAM>();
+  uint32_t in = sb[0] + pn, is = sb[1] + ps;
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     if (b0 + q >= it.nb) break;
@@ -970,7 +996,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_scan(V2Args a) {
     off += cnt[q];
     if (!cnt[q]) continue;
     last = max(last, mx[q]);
-    const V2Task tk{(uint32_t)blockIdx.x, (uint32_t)(b0 + q)};
+    const V2Task tk{(uint32_t)k, (uint32_t)(b0 + q)};
     if (cnt[q] > (uint32_t)kV2WaveCap)
       a.spill[is++] = tk;
     else
@@ -998,16 +1024,16 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
   const int t = threadIdx.x;
   for (int b = t; b < it.nb; b += kBlock) s_cnt[b] = 0;
   __syncthreads();
-  const uint32_t* Kf = d.K + (int64_t)it.f * d.N;
   const int n = min(TILE, it.m - (int)tl.e0);
   const int64_t e0 = it.start + tl.e0;
+  const uint32_t* gkey = a.gkey + it.soff + tl.e0;
   uint32_t keys[U], auxv[U], rank[U];
   int64_t yv[U];
 #pragma unroll
   for (int q = 0; q < U; ++q) {
     const int e = t + q * kBlock;
     if (e < n) {
-      keys[q] = lv.row[e0 + e];
+      keys[q] = gkey[e];
       auxv[q] = lv.aux[e0 + e];
       if (TASK == 1) yv[q] = lv.ey[e0 + e];
     }
@@ -1015,10 +1041,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
 #pragma unroll
   for (int q = 0; q < U; ++q) {
     const int e = t + q * kBlock;
-    if (e < n) {
-      keys[q] = Kf[keys[q]];
-      rank[q] = atomicAdd(&s_cnt[keys[q] >> it.shift], 1u);
-    }
+    if (e < n) rank[q] = atomicAdd(&s_cnt[keys[q] >> it.shift], 1u);
   }
   __syncthreads();
   {
@@ -1069,6 +1092,7 @@ constexpr int kV2CTile = 64;
 
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_v2_ctot(DataArgs d, V2Args a) {
+  __shared__ uint32_t s_c[kWaves][MAXC];
   const int lane = threadIdx.x & (kWave - 1);
   const V2Tile tl = a.tiles[blockIdx.x];
   const V2Item it = a.items[tl.item];
@@ -1078,22 +1102,21 @@ __global__ void __launch_bounds__(kBlock) k_v2_ctot(DataArgs d, V2Args a) {
     const uint32_t n = a.bcount[gb];
     const int64_t e0 = it.soff + a.bbase[gb];
     if constexpr (TASK == 0) {
-      uint32_t loc[MAXC];
+      uint32_t* sc = s_c[threadIdx.x / kWave];
+      if (lane < MAXC) sc[lane] = 0u;
+      team_sync<kWave>();
+      constexpr int U = 4;
+      for (uint32_t e = lane; e < n; e += kWave * U) {
+        uint32_t x[U];
 #pragma unroll
-      for (int c = 0; c < MAXC; ++c) loc[c] = 0;
-      for (uint32_t e = lane; e < n; e += kWave) {
-        const uint32_t x = a.saux[e0 + e];
-        const uint32_t lab = x >> 16, w = x & 0xFFFFu;
+        for (int q = 0; q < U; ++q) x[q] = e + q * kWave < n ? a.saux[e0 + e + q * kWave] : 0u;
 #pragma unroll
-        for (int c = 0; c < MAXC; ++c) loc[c] += (lab == (uint32_t)c) ? w : 0u;
+        for (int q = 0; q < U; ++q)
+          if (e + q * kWave < n) atomicAdd(&sc[x[q] >> 16], x[q] & 0xFFFFu);
       }
-#pragma unroll
-      for (int c = 0; c < MAXC; ++c) {
-        uint32_t v = loc[c];
-#pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-        if (lane == c && c < d.C) a.bcls[gb * d.C + c] = v;
-      }
+      team_sync<kWave>();
+      if (lane < d.C) a.bcls[gb * d.C + lane] = sc[lane];
+      team_sync<kWave>();
     } else {
       int64_t wn = 0, ws = 0;
       for (uint32_t e = lane; e < n; e += kWave) {
@@ -1114,22 +1137,24 @@ __global__ void __launch_bounds__(kBlock) k_v2_ctot(DataArgs d, V2Args a) {
   }
 }
 
-// One workgroup per item: totals before each bucket.
-template <int TASK>
-__global__ void __launch_bounds__(kBlock) k_v2_carry(DataArgs d, V2Args a) {
+// One team per item: totals before each bucket.
+template <int TASK, int TEAM>
+__global__ void __launch_bounds__(kBlock) k_v2_carry(DataArgs d, V2Args a, const uint32_t* list, int64_t n_list) {
   __shared__ uint32_t s_w[8 * kWaves + kWaves];
   __shared__ int64_t s_w64[kWaves];
-  const V2Item it = a.items[blockIdx.x];
-  const int t = threadIdx.x;
+  const int64_t ki = v2_team_item<TEAM>(list, n_list);
+  if (ki < 0) return;  // team-uniform; wave teams have no workgroup barrier
+  const V2Item it = a.items[ki];
+  const int t = team_tid<TEAM>();
   const int C = d.C;
-  constexpr int PER = kV2MaxB / kBlock;
+  constexpr int PER = (TEAM == kWave ? kV2SmallB : kV2MaxB) / TEAM;
   const int b0 = t * PER;
   const int64_t gb0 = (int64_t)it.boff + b0;
   if constexpr (TASK == 0) {
     int64_t lw = 0;
     for (int q = 0; q < PER && b0 + q < it.nb; ++q)
       for (int c = 0; c < C; ++c) lw += a.bcls[(gb0 + q) * C + c];
-    int64_t run_n = team_excl_scan_i64<256>(lw, s_w64, nullptr);
+    int64_t run_n = team_excl_scan_i64<TEAM>(lw, s_w64, nullptr);
     for (int q = 0; q < PER && b0 + q < it.nb; ++q) {
       a.carry_n[gb0 + q] = run_n;
       for (int c = 0; c < C; ++c) run_n += a.bcls[(gb0 + q) * C + c];
@@ -1142,7 +1167,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_carry(DataArgs d, V2Args a) {
         if (c0 + k < C)
           for (int q = 0; q < PER && b0 + q < it.nb; ++q) v[k] += a.bcls[(gb0 + q) * C + c0 + k];
       }
-      team_excl_scan_vec<256, 8>(v, s_w, tot);
+      team_excl_scan_vec<TEAM, 8>(v, s_w, tot);
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         if (c0 + k >= C) continue;
@@ -1160,8 +1185,8 @@ __global__ void __launch_bounds__(kBlock) k_v2_carry(DataArgs d, V2Args a) {
       lw += a.bwn[gb0 + q];
       ls += a.bsum[gb0 + q];
     }
-    int64_t run_n = team_excl_scan_i64<256>(lw, s_w64, nullptr);
-    int64_t run_s = team_excl_scan_i64<256>(ls, s_w64, nullptr);
+    int64_t run_n = team_excl_scan_i64<TEAM>(lw, s_w64, nullptr);
+    int64_t run_s = team_excl_scan_i64<TEAM>(ls, s_w64, nullptr);
     for (int q = 0; q < PER && b0 + q < it.nb; ++q) {
       const int64_t gb = gb0 + q;
       a.carry_n[gb] = run_n;
@@ -2862,8 +2887,10 @@ struct BigState {
   DevBuf<int64_t> sy, sy2;
   DevBuf<V2Task> tasks, spill;
   DevBuf<unsigned int> ctr;
+  DevBuf<uint32_t> lists;  // item indices: small items (wave teams), then the rest
   std::vector<V2Item> host_items;
   std::vector<V2Tile> host_htiles, host_stiles, host_ctiles;
+  std::vector<uint32_t> host_lists;
 };
 
 // Chunk-parallel search of the big items of one phase (see k_v2_*): items in
@@ -2890,6 +2917,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     B.host_htiles.clear();
     B.host_stiles.clear();
     B.host_ctiles.clear();
+    B.host_lists.clear();
     while (k1 < n_all && (k1 == k0 || el + B.host_items[k1].m <= kV2Batch)) {
       V2Item& bi = B.host_items[k1];
       bi.boff = (int32_t)nb;
@@ -2903,6 +2931,14 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
       ++k1;
     }
     const int64_t n_items = k1 - k0;
+    for (int64_t k = k0; k < k1; ++k)
+      if (B.host_items[k].nb <= kV2SmallB) B.host_lists.push_back((uint32_t)(k - k0));
+    const int64_t n_small = (int64_t)B.host_lists.size();
+    for (int64_t k = k0; k < k1; ++k)
+      if (B.host_items[k].nb > kV2SmallB) B.host_lists.push_back((uint32_t)(k - k0));
+    const int64_t n_large = n_items - n_small;
+    B.lists.reserve(n_items);
+    DTE_HIP(hipMemcpyAsync(B.lists.p, B.host_lists.data(), 4 * n_items, hipMemcpyHostToDevice, s));
     DTE_HIP(hipMemcpyAsync(B.items.p, B.host_items.data() + k0, sizeof(V2Item) * n_items, hipMemcpyHostToDevice, s));
     B.htiles.reserve(B.host_htiles.size());
     B.stiles.reserve(B.host_stiles.size());
@@ -2965,6 +3001,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     a.saux = B.saux.p;
     a.sy = B.sy.p;
     a.skey2 = B.skey2.p;
+    a.gkey = B.skey2.p;
     a.saux2 = B.saux2.p;
     a.sy2 = B.sy2.p;
     a.tasks = B.tasks.p;
@@ -2977,7 +3014,12 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     }
     {
       Timer tm("chunk_scan", 0.0, s);
-      hipLaunchKernelGGL(k_v2_scan, dim3((unsigned)n_items), dim3(kBlock), 0, s, a);
+      if (n_small)
+        hipLaunchKernelGGL(k_v2_scan<kWave>, dim3((unsigned)((n_small + kWaves - 1) / kWaves)), dim3(kBlock), 0, s,
+                           a, B.lists.p, n_small);
+      if (n_large)
+        hipLaunchKernelGGL(k_v2_scan<256>, dim3((unsigned)n_large), dim3(kBlock), 0, s, a, B.lists.p + n_small,
+                           n_large);
     }
     a.tiles = B.stiles.p;
     {
@@ -2991,7 +3033,12 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     }
     {
       Timer tm("chunk_carry", 0.0, s);
-      hipLaunchKernelGGL(k_v2_carry<TASK>, dim3((unsigned)n_items), dim3(kBlock), 0, s, d, a);
+      if (n_small)
+        hipLaunchKernelGGL((k_v2_carry<TASK, kWave>), dim3((unsigned)((n_small + kWaves - 1) / kWaves)), dim3(kBlock),
+                           0, s, d, a, B.lists.p, n_small);
+      if (n_large)
+        hipLaunchKernelGGL((k_v2_carry<TASK, 256>), dim3((unsigned)n_large), dim3(kBlock), 0, s, d, a,
+                           B.lists.p + n_small, n_large);
     }
     unsigned int nt[2];
     DTE_HIP(hipMemcpyAsync(nt, B.ctr.p, 8, hipMemcpyDeviceToHost, s));
