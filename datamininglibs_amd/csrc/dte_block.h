@@ -376,31 +376,40 @@ __device__ __forceinline__ void lds_sort_pass(const uint32_t* kin, const uint16_
 }
 
 // Counting sort of n <= TEAM*IPT (key, val) pairs whose keys lie in
-// [0, span), span <= 2 * 16 * TEAM / 2 words of `cnt` (the radix counters
-// of lds_sort_pass, reused as span u32 counters).  Equal keys come out in
+// [0, span), with two 16-bit counters per word of `cnt` (span / 2 words;
+// n < 65536, so counts and offsets fit a half word).  Equal keys come out in
 // any order: callers only use the order of distinct keys.
 template <int IPT, int TEAM>
 __device__ __forceinline__ void lds_count_sort(const uint32_t* kin, const uint16_t* vin, uint32_t* kout,
                                                uint16_t* vout, int n, int span, uint32_t* cnt, uint32_t* s_w) {
   const int t = team_tid<TEAM>();
-  for (int i = t; i < span; i += TEAM) cnt[i] = 0;
+  const int words = (span + 1) >> 1;
+  for (int i = t; i < words; i += TEAM) cnt[i] = 0;
   team_sync<TEAM>();
   uint32_t r[IPT];
 #pragma unroll
   for (int q = 0; q < IPT; ++q) {
     const int e = t + q * TEAM;
-    if (e < n) r[q] = atomicAdd(&cnt[kin[e]], 1u);
+    if (e < n) {
+      const uint32_t k = kin[e];
+      const uint32_t sh = (k & 1u) << 4;
+      r[q] = (atomicAdd(&cnt[k >> 1], 1u << sh) >> sh) & 0xFFFFu;
+    }
   }
   team_sync<TEAM>();
-  const int per = (span + TEAM - 1) / TEAM;
-  const int lo = t * per, hi = min(lo + per, span);
+  const int per = (words + TEAM - 1) / TEAM;
+  const int lo = t * per, hi = min(lo + per, words);
   uint32_t loc = 0;
-  for (int i = lo; i < hi; ++i) loc += cnt[i];
+  for (int i = lo; i < hi; ++i) {
+    const uint32_t w = cnt[i];
+    loc += (w & 0xFFFFu) + (w >> 16);
+  }
   uint32_t base = team_excl_scan<TEAM>(loc, s_w, nullptr);
   for (int i = lo; i < hi; ++i) {
-    const uint32_t c = cnt[i];
-    cnt[i] = base;
-    base += c;
+    const uint32_t w = cnt[i];
+    const uint32_t c0 = w & 0xFFFFu;
+    cnt[i] = base | ((base + c0) << 16);
+    base += c0 + (w >> 16);
   }
   team_sync<TEAM>();
 #pragma unroll
@@ -408,7 +417,8 @@ __device__ __forceinline__ void lds_count_sort(const uint32_t* kin, const uint16
     const int e = t + q * TEAM;
     if (e < n) {
       const uint32_t k = kin[e];
-      const uint32_t pos = cnt[k] + r[q];
+      const uint32_t sh = (k & 1u) << 4;
+      const uint32_t pos = ((cnt[k >> 1] >> sh) & 0xFFFFu) + r[q];
       kout[pos] = k;
       vout[pos] = vin[e];
     }

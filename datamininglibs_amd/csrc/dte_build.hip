@@ -153,11 +153,15 @@ struct Carry {
 template <int MAXC, int TASK, int TEAM, int IPT>
 struct TeamLds {
   static constexpr int CAP = TEAM * IPT;
+  // 16-bit counters: 16 radix digits per thread, or the packed counters of a
+  // counting sort over a key span of up to SPAN.
+  static constexpr int SPAN = 16 * TEAM > (4 * CAP < 2048 ? 4 * CAP : 2048) ? 16 * TEAM
+                                                                           : (4 * CAP < 2048 ? 4 * CAP : 2048);
   union {
     struct {
       uint32_t k[2][CAP];
       uint16_t v[2][CAP];
-      uint16_t cnt[16 * TEAM];
+      uint16_t cnt[SPAN];
     } srt;
     uint32_t hist[TEAM == 256 ? kHistWords : 1];
     uint32_t bpos[TEAM == 256 ? kMaxBuckets : 1];
@@ -179,21 +183,40 @@ struct TeamLds {
 
 // Exclusive scan of the per-lane class columns pre[c][t] across the team,
 // plus the carry; chunk totals go to ctot[c].
+template <int G, int MAXC, int TASK, int TEAM, int IPT>
+__device__ __forceinline__ void scan_class_group(TeamLds<MAXC, TASK, TEAM, IPT>& L, int c0, int C) {
+  const int t = team_tid<TEAM>();
+  uint32_t v[G], tot[G];
+#pragma unroll
+  for (int q = 0; q < G; ++q) v[q] = (c0 + q < C) ? L.pre[(c0 + q) * TEAM + t] : 0u;
+  team_excl_scan_vec<TEAM, G>(v, L.w, tot);
+#pragma unroll
+  for (int q = 0; q < G; ++q)
+    if (c0 + q < C) {
+      L.pre[(c0 + q) * TEAM + t] = v[q] + L.cr.run[c0 + q];
+      if (t == 0) L.ctot[c0 + q] = tot[q];
+    }
+}
+
+// Groups of 8 columns, then one group of 4, 2 or 1 for the rest, so that C
+// classes take C scans.
 template <int MAXC, int TASK, int TEAM, int IPT>
 __device__ void scan_class_columns(TeamLds<MAXC, TASK, TEAM, IPT>& L, int C) {
-  const int t = team_tid<TEAM>();
-  constexpr int G = MAXC < 8 ? MAXC : 8;
-  for (int c0 = 0; c0 < C; c0 += G) {
-    uint32_t v[G], tot[G];
-#pragma unroll
-    for (int q = 0; q < G; ++q) v[q] = (c0 + q < C) ? L.pre[(c0 + q) * TEAM + t] : 0u;
-    team_excl_scan_vec<TEAM, G>(v, L.w, tot);
-#pragma unroll
-    for (int q = 0; q < G; ++q)
-      if (c0 + q < C) {
-        L.pre[(c0 + q) * TEAM + t] = v[q] + L.cr.run[c0 + q];
-        if (t == 0) L.ctot[c0 + q] = tot[q];
-      }
+  int c0 = 0;
+  if constexpr (MAXC >= 8)
+    for (; c0 + 8 <= C; c0 += 8) scan_class_group<8>(L, c0, C);
+  while (c0 < C) {
+    const int r = C - c0;
+    if (MAXC >= 4 && r > 2) {
+      scan_class_group<4>(L, c0, C);
+      c0 += 4;
+    } else if (r == 2) {
+      scan_class_group<2>(L, c0, C);
+      c0 += 2;
+    } else {
+      scan_class_group<1>(L, c0, C);
+      c0 += 1;
+    }
   }
 }
 
@@ -307,15 +330,24 @@ __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, in
     if (lo < hi) {
       int64_t sl = tot[2] + sc[0], sr = tot[3] + sc[1];
       uint32_t prevk = (lo > 0) ? keys[lo - 1] + kbase : L.cr.last_key;
+      // The gain is a non-decreasing function of the exact sum P + q, and a
+      // later boundary only wins with a strictly larger gain: a sum below
+      // the lane's best sum cannot win.
+      int64_t best_pq = 0;
 #pragma unroll
       for (int i = 0; i < IPT; ++i) {
         const int p = lo + i;
         if (p < hi) {
           const uint32_t k = keys[p] + kbase;
           if (nlb[i] > 0 && prevk < k) {
-            const int64_t q = (sl - lx.t((uint64_t)nlb[i])) + (sr - lx.t((uint64_t)(nc.n - nlb[i])));
-            const double g = ig_from_fixed(nc.P + q, lx.F, nc.inv);
-            if (g > tb.g) tb = BestCand{g, k, prevk};
+            const int64_t pq = nc.P + (sl - lx.t((uint64_t)nlb[i])) + (sr - lx.t((uint64_t)(nc.n - nlb[i])));
+            if (pq > best_pq) {
+              const double g = ig_from_fixed(pq, lx.F, nc.inv);
+              if (g > tb.g) {
+                tb = BestCand{g, k, prevk};
+                best_pq = pq;
+              }
+            }
           }
           sl += dl[i];
           sr += dr[i];
@@ -498,7 +530,7 @@ __device__ void eval_hist_bucket(const uint32_t* skey, const uint32_t* saux, con
 template <int MAXC, int TASK, int TEAM, int IPT>
 __device__ void sort_and_eval(int n, int bits, uint32_t kbase, const NodeCtx& nc,
                               TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx) {
-  constexpr int kSpan = 8 * TEAM;  // u32 counters in the radix counter area
+  constexpr int kSpan = TeamLds<MAXC, TASK, TEAM, IPT>::SPAN;
   if (bits > 4 && (1 << bits) <= kSpan) {
     lds_count_sort<IPT, TEAM>(L.u.srt.k[0], L.u.srt.v[0], L.u.srt.k[1], L.u.srt.v[1], n, 1 << bits,
                               reinterpret_cast<uint32_t*>(L.u.srt.cnt), L.w);
@@ -801,8 +833,9 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
 // scores only boundaries between distinct keys from integer prefix sums.
 constexpr int kV2Tile = 32768;   // entries per histogram tile
 constexpr int kV2MaxB = 4096;    // buckets per item
+constexpr int kV2Ipt = kIpt;     // entries per lane of the bucket evaluator
 constexpr int kV2Target = 256;   // entries per bucket aimed at
-constexpr int kV2WaveCap = kWave * kIpt;  // bucket capacity of a wave (512)
+constexpr int kV2WaveCap = kWave * kV2Ipt;  // bucket capacity of a wave (512)
 
 struct V2Item {
   int32_t f;
@@ -1279,7 +1312,7 @@ __device__ __forceinline__ void v2_prefetch(const DataArgs& d, const LevelArgs& 
 
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_v2_eval(DataArgs d, LevelArgs lv, V2Args a, int64_t n_tasks) {
-  constexpr int IPT = kIpt;
+  constexpr int IPT = kV2Ipt;
   __shared__ double s_inv[128], s_logc[128];
   __shared__ TeamLds<MAXC, TASK, kWave, IPT> s_team[kWaves];
   load_tables(s_inv, s_logc);
@@ -1872,19 +1905,40 @@ __global__ void __launch_bounds__(kBlock) k_search_rf_seg(DataArgs d, LevelArgs 
       if (TASK == 1) y0 = lv.ey[st + sl];
     }
     if (sl == 0 && node_ok && !pure) my_bytes += (unsigned long long)m * it.per_group * (TASK == 0 ? 12ull : 20ull);
+    // Draw pipeline: the key gather (and value offset) of draw jj + 1 and the
+    // threshold values of draw jj - 1 are in flight while draw jj is ranked
+    // and scored.
+    int f_nx = 0;
+    uint32_t key_nx = kNone;
+    int64_t svo_nx = 0;
+    if (!pure && it.per_group > 0) {
+      f_nx = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)it.j0) % (uint32_t)d.F);
+      key_nx = valid ? d.K[(int64_t)f_nx * d.N + row] : kNone;
+      svo_nx = d.SVoff[f_nx];
+    }
+    bool pend = false, p_has = false;
+    int64_t p_ridx = 0;
+    int p_f = 0;
+    float p_score = 0.0f, p_lo = 0.0f, p_hi = 0.0f;
     for (int jj = 0; jj < it.per_group; ++jj) {
       const int j = it.j0 + jj;
-      const int f = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)j) % (uint32_t)d.F);
       const int64_t ridx = (int64_t)o * it.D + j;
       if (pure) {
         if (sl == 0 && node_ok) {
           it.res_score[ridx] = 0.0f;
           it.res_thr[ridx] = 0.0f;
-          it.res_feat[ridx] = f;
+          it.res_feat[ridx] = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)j) % (uint32_t)d.F);
         }
         continue;
       }
-      uint32_t key = valid ? d.K[(int64_t)f * d.N + row] : kNone;
+      const int f = f_nx;
+      const int64_t svo = svo_nx;
+      uint32_t key = key_nx;
+      if (jj + 1 < it.per_group) {
+        f_nx = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)(j + 1)) % (uint32_t)d.F);
+        key_nx = valid ? d.K[(int64_t)f_nx * d.N + row] : kNone;
+        svo_nx = d.SVoff[f_nx];
+      }
       // rank inside the segment (ties by lane), then move entries into key order
       uint32_t cnt = 0;
 #pragma unroll
@@ -1934,17 +1988,30 @@ __global__ void __launch_bounds__(kBlock) k_search_rf_seg(DataArgs d, LevelArgs 
       }
       BestCand b = (cand && gval > 0.0) ? BestCand{gval, key, prevk} : BestCand{0.0, kNone, kNone};
       b = seg_best<SEG>(b);
-      if (sl == 0 && node_ok) {
-        float score = 0.0f, thr = 0.0f;
-        if (b.key != kNone) {
-          const float* sv = d.SV + d.SVoff[f];
-          score = (float)(TASK == 0 ? b.g : ldexp_d(b.g, -2 * d.shift));
-          thr = midpoint(sv[b.prev], sv[b.key]);
-        }
-        it.res_score[ridx] = score;
-        it.res_thr[ridx] = thr;
-        it.res_feat[ridx] = f;
+      if (pend) {
+        it.res_score[p_ridx] = p_score;
+        it.res_thr[p_ridx] = p_has ? midpoint(p_lo, p_hi) : 0.0f;
+        it.res_feat[p_ridx] = p_f;
+        pend = false;
       }
+      if (sl == 0 && node_ok) {
+        pend = true;
+        p_ridx = ridx;
+        p_f = f;
+        p_has = b.key != kNone;
+        p_score = 0.0f;
+        if (p_has) {
+          const float* sv = d.SV + svo;
+          p_score = (float)(TASK == 0 ? b.g : ldexp_d(b.g, -2 * d.shift));
+          p_lo = sv[b.prev];
+          p_hi = sv[b.key];
+        }
+      }
+    }
+    if (pend) {
+      it.res_score[p_ridx] = p_score;
+      it.res_thr[p_ridx] = p_has ? midpoint(p_lo, p_hi) : 0.0f;
+      it.res_feat[p_ridx] = p_f;
     }
   }
   if (sl == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
