@@ -265,6 +265,13 @@ def profile_reset():
     _check(lib().dte_profile_reset())
 
 
+def release_cached_memory():
+    """Frees the device scratch that fits keep cached; returns the bytes freed."""
+    b = ctypes.c_int64()
+    _check(lib().dte_release_cached_memory(ctypes.byref(b)))
+    return b.value
+
+
 def profile_stats():
     L = lib()
     n = L.dte_profile_query(-1, None, 0, None, None, None)

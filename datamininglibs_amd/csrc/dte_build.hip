@@ -3886,6 +3886,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   fo->fixed_shift = shift;
   fo->n_nodes = n_nodes;
   fo->has_stats = true;
+  PoolSuspend own;  // the forest outlives the fit
   fo->tree_start.alloc(T + 1);
   DTE_HIP(hipMemcpyAsync(fo->tree_start.p, ts.data(), 8 * (T + 1), hipMemcpyHostToDevice, s));
   fo->feature.alloc(n_nodes);
@@ -3956,6 +3957,7 @@ Forest* merge_forests(std::vector<std::unique_ptr<Forest>>& parts, hipStream_t s
   fo->n_trees = T;
   const int C = fo->n_classes;
   const size_t W = fo->task == 0 ? (size_t)C : 1;
+  PoolSuspend own;  // the forest outlives the fit
   fo->tree_start.alloc(T + 1);
   fo->feature.alloc(n);
   fo->left.alloc(n);
@@ -3999,6 +4001,7 @@ Forest* merge_forests(std::vector<std::unique_ptr<Forest>>& parts, hipStream_t s
 
 Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_t F, const dte_params& p,
                    hipStream_t s) {
+  PoolScope pool(s);  // the fit's temporaries come from the stream's scratch pool
   FitData fd;
   prepare_data(fd, X_dev_in, y_host, N, F, p, s);
   if (p.build_mode != DTE_BUILD_PLAIN) return fit_batch(fd, p.tree_offset, p.n_trees, N, F, p, s, false);
@@ -4011,6 +4014,7 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
 }
 
 void build_predict_nodes(Forest& f, hipStream_t s) {
+  PoolSuspend own;
   f.pnodes.alloc(f.n_nodes);
   hipLaunchKernelGGL(k_predict_nodes, dim3(grid_for(f.n_nodes)), dim3(kBlock), 0, s, f.n_nodes, f.n_classes,
                      f.task, f.feature.p, f.threshold.p, f.left.p, f.class_counts.p, f.value.p, f.pnodes.p);

@@ -122,6 +122,86 @@ void select_device(int device) { set_device(device); }
 
 void fail(int code, const std::string& msg) { throw DteException(code, msg); }
 
+// ------------------------------------------------------- scratch pool --
+namespace {
+std::mutex g_pool_mu;
+// (device, stream) -> cached blocks by size
+std::map<std::pair<int, hipStream_t>, std::multimap<size_t, void*>> g_pool;
+thread_local bool t_pool_on = false;
+thread_local hipStream_t t_pool_s = nullptr;
+constexpr size_t kPoolGrain = (size_t)64 << 10;
+}  // namespace
+
+bool pool_on() { return t_pool_on; }
+
+PoolScope::PoolScope(hipStream_t s) : prev_on(t_pool_on), prev_s(t_pool_s) {
+  t_pool_on = true;
+  t_pool_s = s;
+}
+PoolScope::~PoolScope() {
+  t_pool_on = prev_on;
+  t_pool_s = prev_s;
+}
+PoolSuspend::PoolSuspend() : prev_on(t_pool_on) { t_pool_on = false; }
+PoolSuspend::~PoolSuspend() { t_pool_on = prev_on; }
+
+void* pool_alloc(size_t bytes, size_t* got, int* dev, hipStream_t* owner) {
+  int d = 0;
+  DTE_HIP(hipGetDevice(&d));
+  const size_t want = (bytes + kPoolGrain - 1) / kPoolGrain * kPoolGrain;
+  *dev = d;
+  *owner = t_pool_s;
+  {
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    auto& m = g_pool[{d, t_pool_s}];
+    auto it = m.lower_bound(want);
+    // a cached block serves requests down to two thirds of its size
+    if (it != m.end() && it->first <= want + want / 2) {
+      void* p = it->second;
+      *got = it->first;
+      m.erase(it);
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    pool_trim();
+    e = hipMalloc(&p, want);
+  }
+  if (e != hipSuccess)
+    fail(e == hipErrorOutOfMemory ? DTE_ERR_OUT_OF_MEMORY : DTE_ERR_HIP,
+         std::string("hipMalloc: ") + hipGetErrorString(e));
+  *got = want;
+  return p;
+}
+
+void pool_free(void* p, size_t bytes, int dev, hipStream_t owner) {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  g_pool[{dev, owner}].emplace(bytes, p);
+}
+
+size_t pool_trim() {
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  size_t freed = 0;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (auto& kv : g_pool) {
+    if (kv.second.empty()) continue;
+    // blocks may still be read by work queued on their stream
+    (void)hipSetDevice(kv.first.first);
+    (void)hipDeviceSynchronize();
+    for (auto& b : kv.second) {
+      (void)hipFree(b.second);
+      freed += b.first;
+    }
+    kv.second.clear();
+  }
+  (void)hipSetDevice(cur);
+  return freed;
+}
+
 bool profiling_enabled() { return g_prof_on; }
 
 void profile_record(const char* name, hipEvent_t a, hipEvent_t b, double bytes) {
@@ -475,6 +555,13 @@ int dte_forest_canonical(const dte_forest* forest, int64_t* tree_start, int32_t*
     }
     if (f.task == 0) cp(class_counts, f.class_counts.p, 4 * (size_t)n * f.n_classes);
     DTE_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dte_release_cached_memory(int64_t* bytes_released) {
+  return guarded([&] {
+    const size_t b = pool_trim();
+    if (bytes_released) *bytes_released = (int64_t)b;
   });
 }
 

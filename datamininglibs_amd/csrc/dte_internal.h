@@ -33,36 +33,99 @@ void select_device(int device);
                   std::string(#expr) + ": " + hipGetErrorString(_e));                  \
   } while (0)
 
+// ------------------------------------------------------- scratch pool --
+// Device blocks of a fit's temporaries are cached per (device, stream) when
+// released and handed back to later requests on the same stream, so that the
+// growing per-level buffers and repeated fits do not call hipMalloc / hipFree
+// (which synchronise the device) inside the level loop.  Reuse on the same
+// stream is ordered by the stream.  Pooling is on for the allocations of a
+// thread inside a PoolScope; forests handed to the caller are allocated
+// outside the pool (PoolSuspend).
+struct PoolBlock {
+  void* p;
+  size_t bytes;
+};
+// Takes a block of at least `bytes` for the thread's pool stream (granted
+// size in *got, owning device and stream in *dev, *owner).
+void* pool_alloc(size_t bytes, size_t* got, int* dev, hipStream_t* owner);
+void pool_free(void* p, size_t bytes, int dev, hipStream_t owner);
+// Frees every cached block (all devices); returns the bytes released.
+size_t pool_trim();
+bool pool_on();  // inside a PoolScope (and not suspended) on this thread
+
+struct PoolScope {
+  bool prev_on;
+  hipStream_t prev_s;
+  explicit PoolScope(hipStream_t s);
+  ~PoolScope();
+};
+struct PoolSuspend {
+  bool prev_on;
+  PoolSuspend();
+  ~PoolSuspend();
+};
+
 // Device buffer owned by RAII (the reference frees by hand in
 // GpuParams::finalize, gpudte_algorithm_shared.cc:30-52, and leaks on throw).
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
-  size_t n = 0;
+  size_t n = 0;           // elements requested
+  size_t block = 0;       // bytes of the pool block (pooled buffers)
+  bool pooled = false;
+  int dev = 0;
+  hipStream_t owner = nullptr;
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), block(o.block), pooled(o.pooled), dev(o.dev), owner(o.owner) {
+    o.p = nullptr;
+    o.n = 0;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      block = o.block;
+      pooled = o.pooled;
+      dev = o.dev;
+      owner = o.owner;
+      o.p = nullptr;
+      o.n = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
   void alloc(size_t count) {
     release();
     if (count == 0) count = 1;
-    DTE_HIP(hipMalloc((void**)&p, count * sizeof(T)));
-    n = count;
+    if (pool_on()) {
+      p = (T*)pool_alloc(count * sizeof(T), &block, &dev, &owner);
+      pooled = true;
+      n = count;
+    } else {
+      DTE_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+      pooled = false;
+      n = count;
+    }
   }
   // Grow to at least `count` elements; contents are not preserved.
   void reserve(size_t count) {
     if (count > n) alloc(count + count / 4);
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pooled)
+        pool_free(p, block, dev, owner);
+      else
+        (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    block = 0;
+    pooled = false;
   }
   size_t bytes() const { return n * sizeof(T); }
 };

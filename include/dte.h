@@ -114,7 +114,8 @@ int dte_params_default(int32_t algorithm, dte_params* out);
  * (gpudte_algorithm.cc:14-267: AllocateFit, kSetupKernel, kInitTreeBatch,
  * kFindSplit, kPerformSplit, node streaming).  samples: host, feature-major
  * n_features x n_rows; targets: host, n_rows (class ids as float, or
- * regression targets). */
+ * regression targets).  The fit's temporary device buffers stay cached for
+ * later fits on the same stream (dte_release_cached_memory frees them). */
 int dte_fit(const float* samples, const float* targets, int64_t n_rows, int32_t n_features,
             const dte_params* params, dte_forest** out);
 
@@ -203,6 +204,14 @@ int dte_device_alloc(int32_t device_id, int64_t bytes, void** out);
 int dte_device_free(int32_t device_id, void* ptr);
 int dte_memcpy_to_device(int32_t device_id, void* dst, const void* src, int64_t bytes);
 int dte_memcpy_to_host(int32_t device_id, void* dst, const void* src, int64_t bytes);
+
+/* Frees the device memory a fit keeps cached for later fits on the same
+ * stream (its per-level scratch; see dte_fit).  The reference allocates and
+ * frees its buffers around every fit (gpudte_algorithm.cc:486-596,
+ * gpudte_algorithm_shared.cc:30-52), so it has no counterpart; callers that
+ * size other allocations from the free memory call this first.
+ * bytes_released may be NULL. */
+int dte_release_cached_memory(int64_t* bytes_released);
 
 int dte_device_count(int32_t* count);
 int dte_last_error(char* buf, int64_t cap);

@@ -92,3 +92,21 @@ def test_plain_regression(algorithm):
     g, o = fit_both(X, y, algorithm=algorithm, task=capi.REGRESSION, n_trees=3, k=2, max_depth=10,
                     min_node_size=5, build_mode=capi.BUILD_PLAIN)
     assert_same_forest(g, o)
+
+
+def test_cached_scratch_reuse_and_release():
+    """Fits reuse the scratch the previous fit left cached (a larger fit,
+    then a smaller one of another shape, then a release of the cache and the
+    first fit again): every forest equals its oracle forest."""
+    Xa, ya = small_classes(21, 20000, 12, 4)
+    Xb, yb = small_classes(22, 3000, 7, 3)
+    kwa = dict(algorithm=capi.RANDOM_FOREST, n_trees=2, k=3, max_depth=12, min_node_size=5, n_classes=4)
+    kwb = dict(algorithm=capi.RANDOM_FOREST, n_trees=3, k=2, max_depth=10, min_node_size=5, n_classes=3)
+    g, o = fit_both(Xa, ya, **kwa)
+    assert_same_forest(g, o)
+    g, o = fit_both(Xb, yb, **kwb)
+    assert_same_forest(g, o)
+    assert capi.release_cached_memory() > 0
+    assert capi.release_cached_memory() == 0
+    g, o = fit_both(Xa, ya, **kwa)
+    assert_same_forest(g, o)
