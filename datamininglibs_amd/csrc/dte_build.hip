@@ -2122,32 +2122,59 @@ struct TierLists {
   uint32_t* l[kMaxTiers];
 };
 
+__device__ __forceinline__ int tier_of(int m, const Cuts& cuts) {
+  int tier = cuts.n - 1;
+#pragma unroll
+  for (int k = 0; k < kMaxTiers - 1; ++k)
+    if (k < cuts.n - 1 && m <= cuts.c[k]) {
+      tier = k;
+      break;
+    }
+  return tier;
+}
+
+// Each wave owns a contiguous range of the list: it counts its tiers, takes
+// one slot range per tier with a single atomic, then writes its entries.
 __global__ void k_tier_split(const uint32_t* list, int64_t n, const uint32_t* open_idx, const int32_t* node_m, Cuts cuts,
                              TierLists out, unsigned int* counts) {
   const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t base = (blockIdx.x * (int64_t)blockDim.x) & ~(int64_t)(kWave - 1); base < n;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t g = base + (threadIdx.x & ~(kWave - 1)) + lane;
+  const int64_t wave = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
+  const int64_t n_waves = (int64_t)gridDim.x * kWaves;
+  const int64_t per = (((n + n_waves - 1) / n_waves) + kWave - 1) & ~(int64_t)(kWave - 1);
+  const int64_t lo = wave * per, hi = min(lo + per, n);
+  if (lo >= hi) return;  // wave-uniform
+  unsigned int cnt[kMaxTiers];
+#pragma unroll
+  for (int k = 0; k < kMaxTiers; ++k) cnt[k] = 0;
+  for (int64_t b = lo; b < hi; b += kWave) {
+    const int64_t g = b + lane;
+    const int tier = g < hi ? tier_of(node_m[open_idx[list ? list[g] : (uint32_t)g]], cuts) : -1;
+#pragma unroll
+    for (int k = 0; k < kMaxTiers; ++k)
+      if (k < cuts.n) cnt[k] += (unsigned int)__popcll(__ballot(tier == k));
+  }
+  unsigned int base[kMaxTiers];
+#pragma unroll
+  for (int k = 0; k < kMaxTiers; ++k) {
+    unsigned int b0 = 0;
+    if (lane == 0 && k < cuts.n && cnt[k]) b0 = atomicAdd(&counts[k], cnt[k]);
+    base[k] = __shfl(b0, 0, kWave);
+  }
+  const uint64_t below = (1ull << lane) - 1ull;
+  for (int64_t b = lo; b < hi; b += kWave) {
+    const int64_t g = b + lane;
     uint32_t o = 0;
     int tier = -1;
-    if (g < n) {
+    if (g < hi) {
       o = list ? list[g] : (uint32_t)g;
-      const int m = node_m[open_idx[o]];
-      tier = cuts.n - 1;
-      for (int k = 0; k < cuts.n - 1; ++k)
-        if (m <= cuts.c[k]) {
-          tier = k;
-          break;
-        }
+      tier = tier_of(node_m[open_idx[o]], cuts);
     }
-    for (int k = 0; k < cuts.n; ++k) {
+#pragma unroll
+    for (int k = 0; k < kMaxTiers; ++k) {
+      if (k >= cuts.n) break;
       const uint64_t mask = __ballot(tier == k);
-      if (!mask) continue;
-      const int leader = __ffsll((unsigned long long)mask) - 1;
-      unsigned int basepos = 0;
-      if (lane == leader) basepos = atomicAdd(&counts[k], (unsigned int)__popcll(mask));
-      basepos = __shfl(basepos, leader, kWave);
-      if (tier == k) out.l[k][basepos + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull))] = o;
+      if (tier == k) out.l[k][base[k] + (unsigned int)__popcll(mask & below)] = o;
+      base[k] += (unsigned int)__popcll(mask);
     }
   }
 }
@@ -3552,7 +3579,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   auto run_phase = [&](ItemArgs it) {
     for (auto& tl : tier_list) tl.reserve(it.n_groups);
     DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
-    hipLaunchKernelGGL(k_tier_split, dim3(grid_for(it.n_groups)), dim3(kBlock), 0, s, it.list, it.n_groups,
+    hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(it.n_groups), 1024)), dim3(kBlock), 0, s, it.list, it.n_groups,
                        lv.open_idx, lv.m, cuts, tier_lists(), tier_cnt.p);
     unsigned int cnt[kMaxTiers];
     DTE_HIP(hipMemcpyAsync(cnt, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
@@ -3756,7 +3783,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       } else {
         for (auto& tl : tier_list) tl.reserve(n_open);
         DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
-        hipLaunchKernelGGL(k_tier_split, dim3(grid_for(n_open)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
+        hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(n_open), 1024)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
                            n_open, lv.open_idx, lv.m, Cuts{{512, 0x7FFFFFFF, 0, 0, 0, 0}, 2}, tier_lists(),
                            tier_cnt.p);
         unsigned int pc[kMaxTiers];
@@ -3846,7 +3873,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     if (n_level > 0 && algorithm == DTE_RANDOM_FOREST) {
       DevBuf<uint32_t> mx(1);
       DTE_HIP(hipMemsetAsync(mx.p, 0, 4, s));
-      hipLaunchKernelGGL(k_max_m, dim3(grid_for(n_level)), dim3(kBlock), 0, s,
+      hipLaunchKernelGGL(k_max_m, dim3(std::min(grid_for(n_level), 1024)), dim3(kBlock), 0, s,
                          o_m.p + off_level, n_level, mx.p);
       max_m = read_one(mx.p, s);
     }
