@@ -312,11 +312,13 @@ __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, in
     }
     int64_t sc[4] = {0, 0, 0, 0};  // lane deltas of sl, sr; sums at the chunk start
     int64_t dl[IPT], dr[IPT];
+    int64_t tq[IPT];  // T(nl) + T(n - nl) before each position, read with the deltas
 #pragma unroll
     for (int i = 0; i < IPT; ++i) {
       if (lo + i < hi) {
         dl[i] = lx.t(lb[i] + wv[i]) - lx.t(lb[i]);
         dr[i] = lx.t(tc[i] - lb[i] - wv[i]) - lx.t(tc[i] - lb[i]);
+        tq[i] = nlb[i] > 0 ? lx.t((uint64_t)nlb[i]) + lx.t((uint64_t)(nc.n - nlb[i])) : 0;
         sc[0] += dl[i];
         sc[1] += dr[i];
       }
@@ -340,7 +342,7 @@ __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, in
         if (p < hi) {
           const uint32_t k = keys[p] + kbase;
           if (nlb[i] > 0 && prevk < k) {
-            const int64_t pq = nc.P + (sl - lx.t((uint64_t)nlb[i])) + (sr - lx.t((uint64_t)(nc.n - nlb[i])));
+            const int64_t pq = nc.P + sl + sr - tq[i];
             if (pq > best_pq) {
               const double g = ig_from_fixed(pq, lx.F, nc.inv);
               if (g > tb.g) {
