@@ -248,6 +248,20 @@ def fit_device(X_dev_ptr, n_rows, n_features, y, params):
     return Forest(h)
 
 
+def fit_many(X_dev_ptrs, n_rows, n_features, ys, params, max_concurrent=0):
+    """Independent fits run concurrently on separate streams (dte_fit_many)."""
+    n = len(X_dev_ptrs)
+    ys = [np.ascontiguousarray(y, np.float32) for y in ys]
+    xs = (P * n)(*[P(x) for x in X_dev_ptrs])
+    ts = (P * n)(*[P(y.ctypes.data) for y in ys])
+    rows = (ctypes.c_int64 * n)(*[int(r) for r in n_rows])
+    feats = (ctypes.c_int32 * n)(*[int(f) for f in n_features])
+    ps = (Params * n)(*params)
+    out = (P * n)()
+    _check(lib().dte_fit_many(n, xs, ts, rows, feats, ps, int(max_concurrent), out))
+    return [Forest(P(out[i])) for i in range(n)]
+
+
 def import_forest(nodes, probs, n_trees, n_classes, n_features, task, device_id=0):
     nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
     probs = np.ascontiguousarray(probs, np.float32)

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <functional>
@@ -31,6 +33,7 @@ struct DteException : std::exception {
 
 std::mutex g_dev_mu;
 std::map<int, hipStream_t> g_streams;
+std::map<int, std::vector<hipStream_t>> g_workers;  // streams of concurrent fits
 
 std::mutex g_prof_mu;
 bool g_prof_on = false;
@@ -220,6 +223,20 @@ void device_init(int device) {
   g_streams[device] = s;
 }
 
+// k-th stream of the concurrent-fit workers of a device (created on first
+// use, kept for the life of the process so that their scratch pools are
+// reused).  The device must be current.
+hipStream_t worker_stream(int device, int k) {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  auto& v = g_workers[device];
+  while ((int)v.size() <= k) {
+    hipStream_t s;
+    DTE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    v.push_back(s);
+  }
+  return v[k];
+}
+
 hipStream_t library_stream(int device) {
   std::lock_guard<std::mutex> lk(g_dev_mu);
   auto it = g_streams.find(device);
@@ -349,6 +366,70 @@ int dte_fit(const float* samples, const float* targets, int64_t n_rows, int32_t 
 int dte_fit_device(const float* samples_dev, const float* targets, int64_t n_rows, int32_t n_features,
                    const dte_params* params, dte_forest** out) {
   return guarded([&] { fit_common(samples_dev, true, targets, n_rows, n_features, params, out); });
+}
+
+int dte_fit_many(int32_t n, const float* const* samples_dev, const float* const* targets, const int64_t* n_rows,
+                 const int32_t* n_features, const dte_params* params, int32_t max_concurrent, dte_forest** out) {
+  return guarded([&] {
+    if (n < 0) fail(DTE_ERR_INVALID_ARGUMENT, "n < 0");
+    if (n == 0) return;
+    if (!samples_dev || !targets || !n_rows || !n_features || !params || !out)
+      fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    for (int32_t i = 0; i < n; ++i) {
+      check_params(&params[i]);
+      if (!samples_dev[i] || !targets[i]) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+      if (n_rows[i] <= 0 || n_features[i] <= 0) fail(DTE_ERR_INVALID_ARGUMENT, "empty data set");
+      if (n_rows[i] > 0x7FFFFFFF) fail(DTE_ERR_INVALID_ARGUMENT, "n_rows above 2^31-1");
+      if (params[i].device_id != params[0].device_id)
+        fail(DTE_ERR_INVALID_ARGUMENT, "all fits of one call run on one device");
+      out[i] = nullptr;
+    }
+    const int dev = params[0].device_id;
+    set_device(dev);
+    const int W = std::max(1, std::min<int>(max_concurrent > 0 ? max_concurrent : 8, n));
+    std::vector<hipStream_t> streams(W);
+    for (int k = 0; k < W; ++k) streams[k] = worker_stream(dev, k);
+    std::vector<std::unique_ptr<Forest>> res(n);
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    std::mutex emu;
+    int ecode = 0;
+    std::string emsg;
+    auto work = [&](int k) {
+      try {
+        DTE_HIP(hipSetDevice(dev));
+        for (;;) {
+          const int i = next.fetch_add(1);
+          if (i >= n || failed.load()) break;
+          res[i].reset(fit_forest(samples_dev[i], targets[i], n_rows[i], n_features[i], params[i], streams[k]));
+        }
+      } catch (const DteException& e) {
+        std::lock_guard<std::mutex> lk(emu);
+        if (!ecode) {
+          ecode = e.code;
+          emsg = e.msg;
+        }
+        failed = true;
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(emu);
+        if (!ecode) {
+          ecode = DTE_ERR_INTERNAL;
+          emsg = e.what();
+        }
+        failed = true;
+      }
+    };
+    std::vector<std::thread> th;
+    for (int k = 1; k < W; ++k) th.emplace_back(work, k);
+    work(0);
+    for (auto& t : th) t.join();
+    if (ecode) fail(ecode, emsg);
+    for (int32_t i = 0; i < n; ++i) {
+      auto* h = new dte_forest();
+      h->f = std::move(*res[i]);
+      out[i] = h;
+    }
+  });
 }
 
 int dte_forest_free(dte_forest* forest) {

@@ -124,6 +124,16 @@ int dte_fit(const float* samples, const float* targets, int64_t n_rows, int32_t 
 int dte_fit_device(const float* samples_dev, const float* targets, int64_t n_rows,
                    int32_t n_features, const dte_params* params, dte_forest** out);
 
+/* n independent fits, samples resident in HBM (as dte_fit_device), run
+ * concurrently: up to max_concurrent (0: 8) host threads, each with its own
+ * HIP stream, take the fits in order; every forest is bit-identical to its
+ * fit alone.  The many-forests workload of the reference, which fits one
+ * model per device thread (lib_algorithms/source/gpu_algorithm.cc:18-31) and
+ * otherwise one after another.  All params[i].device_id must be equal.  On
+ * failure no forest is returned. */
+int dte_fit_many(int32_t n, const float* const* samples_dev, const float* const* targets, const int64_t* n_rows,
+                 const int32_t* n_features, const dte_params* params, int32_t max_concurrent, dte_forest** out);
+
 int dte_forest_info_get(const dte_forest* forest, dte_forest_info* out);
 
 /* Export in the reference model layout: MlModel kNodeArray / kProbArray

@@ -129,3 +129,27 @@ def test_rf_1m_x_128_full_size_invariants():
                             n_classes=10, seed=324123)
     f = capi.fit(X, y, p)
     check_forest(f.canonical(), n_rows=1_000_000, max_depth=20, min_node=5, task=capi.CLASSIFICATION, n_classes=10)
+
+
+def test_fit_many_concurrent_equals_alone():
+    """Twelve uneven extra-trees forests fitted together on 4 streams equal
+    the same fits one at a time, and the three smallest equal the oracle."""
+    specs = synth.uneven_forests(seed=0)[:12]
+    data = [synth.uneven_forest_data(sp) for sp in specs]
+    params = [capi.default_params(capi.EXTRA_TREES, n_trees=10, n_classes=sp["classes"], seed=sp["seed"])
+              for sp in specs]
+    devs = [capi.DeviceArray.from_numpy(X) for X, _ in data]
+    many = capi.fit_many([d.ptr for d in devs], [X.shape[1] for X, _ in data], [X.shape[0] for X, _ in data],
+                         [y for _, y in data], params, max_concurrent=4)
+    smallest = sorted(range(len(specs)), key=lambda i: specs[i]["n"] * specs[i]["f"])[:3]
+    for i, ((X, y), d, p, fm) in enumerate(zip(data, devs, params, many)):
+        alone = capi.fit_device(d.ptr, X.shape[1], X.shape[0], y, p)
+        a, b = fm.canonical(), alone.canonical()
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (i, k)
+        if i in smallest:
+            o = pyoracle.fit(X, y, algorithm=1, n_trees=10, n_classes=specs[i]["classes"], seed=specs[i]["seed"])
+            assert_same_forest(fm, o)
+        alone.close()
+        fm.close()
+        d.free()
