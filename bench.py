@@ -248,6 +248,13 @@ def other_workload(name, args):
         params = [capi.default_params(capi.EXTRA_TREES, n_trees=50, n_classes=sp["classes"], seed=sp["seed"])
                   for sp in specs]
         devs = [capi.DeviceArray.from_numpy(X) for X, _ in data]
+        # one autotuner run on the largest forest; its extra-trees tier cut
+        # serves every forest
+        big = max(range(len(specs)), key=lambda i: specs[i]["n"] * specs[i]["f"])
+        tuned, tune_s = capi.autotune(devs[big].ptr, data[big][0].shape[1], data[big][0].shape[0], data[big][1],
+                                      params[big], trial_trees=10)
+        for p in params:
+            p.cut_ert = tuned.cut_ert
         for (X, y), Xd, p in zip(data, devs, params):
             capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p).close()
         t = time.perf_counter()
@@ -279,6 +286,7 @@ def other_workload(name, args):
         line = {"workload": "64 extra-trees forests, 2k..200k rows, 16..256 features, 2..8 classes, 50 deep trees each",
                 "gpu_trees_per_s": round(64 * 50 / dt, 1), "gpu_seconds_all": round(dt, 2), "nodes": nodes,
                 "gpu_concurrency": "8 fits at a time on 8 streams (dte_fit_many)",
+                "tuning": {"cut_ert": tuned.cut_ert, "autotune_s": round(tune_s, 2)},
                 "gpu_trees_per_s_one_after_another": round(64 * 50 / dt_seq, 1),
                 "cpu_sample": f"8 smallest forests ({ntr} rows total), {dto:.1f} s on {cores} threads",
                 "cpu_trees_per_s_sample": round(8 * 50 / dto, 1)}
@@ -298,6 +306,7 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true", help="also run a 1 GiB copy kernel")
     ap.add_argument("--serve-requests", type=int, default=1000, help="requests per serving QPS point (0: skip)")
+    ap.add_argument("--no-autotune", action="store_true", help="default tier cuts instead of dte_autotune's")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
                     help="c2 (default): the headline random-forest line; c1/c3/c4: the other workload shapes")
     args = ap.parse_args()
@@ -324,8 +333,16 @@ def main():
     common = dict(n_trees=args.trees, n_features_per_node=11, max_depth=20, min_node_size=5, n_classes=10,
                   seed=324123)
 
-    times, stats, forest = fit_steps(capi, Xd, args.rows, args.features, y,
-                                     capi.default_params(capi.RANDOM_FOREST, **common), args.steps, args.warmup,
+    params = capi.default_params(capi.RANDOM_FOREST, **common)
+    tuning = None
+    if not args.no_autotune:
+        # split-search tier cuts and bucket size for this data set, from
+        # trial fits of 16 trees (outside the timed steps)
+        params, tune_s = capi.autotune(Xd.ptr, args.rows, args.features, y, params, trial_trees=16)
+        tuning = {"cut_team": params.cut_team, "cut_block": params.cut_block,
+                  "bucket_target": params.bucket_target, "autotune_s": round(tune_s, 2)}
+        log(f"autotune: {tuning}")
+    times, stats, forest = fit_steps(capi, Xd, args.rows, args.features, y, params, args.steps, args.warmup,
                                      "batched")
     total = sum(times)
     trees_per_s = args.trees * args.steps / total
@@ -419,7 +436,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "random forest classifier, 1,000,000 x 128 fp32 N(0,1), 16 informative, 10 classes, "
                                "200 bootstrap trees, entropy, 11 features/node, max depth 20, min node 5",
-                   "rows": args.rows, "features": args.features, "trees": args.trees, "parallelism": "1 GPU"},
+                   "rows": args.rows, "features": args.features, "trees": args.trees, "parallelism": "1 GPU",
+                   "tuning": tuning},
         "plain_path": plain,
         "predict": {"rows_per_s": round(n_pred / pred_dt, 1), "kernel_ms": round(pred_kernel_ms, 3),
                     "compulsory_GBps": round(pred_bytes / (pred_kernel_ms * 1e-3) / 1e9, 1) if pred_kernel_ms else None,

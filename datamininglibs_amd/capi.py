@@ -49,7 +49,11 @@ class Params(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("build_mode", ctypes.c_int32),
         ("tree_offset", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 6),
+        ("cut_team", ctypes.c_int32),
+        ("cut_block", ctypes.c_int32),
+        ("bucket_target", ctypes.c_int32),
+        ("cut_ert", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 
@@ -246,6 +250,16 @@ def fit_device(X_dev_ptr, n_rows, n_features, y, params):
     _check(lib().dte_fit_device(P(X_dev_ptr), _ptr(y), n_rows, n_features, ctypes.byref(params),
                                 ctypes.byref(h)))
     return Forest(h)
+
+
+def autotune(X_dev_ptr, n_rows, n_features, y, params, trial_trees=0):
+    """Split-search tuning for this data set (dte_autotune): (tuned params, seconds spent)."""
+    y = np.ascontiguousarray(y, np.float32)
+    tuned = Params()
+    secs = ctypes.c_double()
+    _check(lib().dte_autotune(P(X_dev_ptr), _ptr(y), n_rows, n_features, ctypes.byref(params), int(trial_trees),
+                              ctypes.byref(tuned), ctypes.byref(secs)))
+    return tuned, secs.value
 
 
 def fit_many(X_dev_ptrs, n_rows, n_features, ys, params, max_concurrent=0):

@@ -836,7 +836,7 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
 constexpr int kV2Tile = 32768;   // entries per histogram tile
 constexpr int kV2MaxB = 4096;    // buckets per item
 constexpr int kV2Ipt = kIpt;     // entries per lane of the bucket evaluator
-constexpr int kV2Target = 256;   // entries per bucket aimed at
+constexpr int kV2Target = kDefBucketTarget;  // entries per bucket aimed at (default)
 constexpr int kV2WaveCap = kWave * kV2Ipt;  // bucket capacity of a wave (512)
 
 struct V2Item {
@@ -1515,7 +1515,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_reduce(DataArgs d, V2Args a, Item
 
 // Item table of a phase: one item per (big group, draw).
 __global__ void k_v2_items(DataArgs d, LevelArgs lv, const uint32_t* groups, int64_t n_groups, int per_group, int j0,
-                           int D, int nb_cap, V2Item* items) {
+                           int D, int nb_cap, int target, V2Item* items) {
   const int64_t n = n_groups * per_group;
   for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t o = groups[k / per_group];
@@ -1526,10 +1526,10 @@ __global__ void k_v2_items(DataArgs d, LevelArgs lv, const uint32_t* groups, int
                 (uint32_t)d.F);
     b.m = lv.m[i];
     b.start = lv.start[i];
-    // nb: a power of two near m / kV2Target, then the shift that maps the
+    // nb: a power of two near m / target, then the shift that maps the
     // feature's dense keys [0, S) onto at most nb buckets.
     int nb = 1;
-    while (nb < nb_cap && nb * kV2Target < b.m) nb <<= 1;
+    while (nb < nb_cap && nb * target < b.m) nb <<= 1;
     const uint32_t S = d.S[b.f];
     const int kb = bit_length(S > 1 ? S - 1u : 1u);
     int lg = 0;
@@ -2987,6 +2987,7 @@ struct BigState {
   std::vector<V2Item> host_items;
   std::vector<V2Tile> host_htiles, host_stiles, host_ctiles;
   std::vector<uint32_t> host_lists;
+  int target = kV2Target;  // entries per bucket aimed at
 };
 
 // Chunk-parallel search of the big items of one phase (see k_v2_*): items in
@@ -3002,7 +3003,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
   const int C = d.C;
   B.items.reserve(n_all);
   hipLaunchKernelGGL(k_v2_items, dim3(grid_for(n_all)), dim3(kBlock), 0, s, d, lv, groups, n_groups, it.per_group,
-                     it.j0, it.D, nb_cap, B.items.p);
+                     it.j0, it.D, nb_cap, B.target, B.items.p);
   B.host_items.resize(n_all);
   DTE_HIP(hipMemcpyAsync(B.host_items.data(), B.items.p, sizeof(V2Item) * n_all, hipMemcpyDeviceToHost, s));
   DTE_HIP(hipStreamSynchronize(s));
@@ -3558,15 +3559,16 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   // path above its LDS capacity).
   // Nodes above the workgroup's LDS capacity go through the chunk-parallel
   // pipeline (the plain driver keeps one workgroup per node throughout).
-  int big_min = kCap + 1;
-  if (const char* e = std::getenv("DTE_BIG_MIN")) big_min = std::max(kCap + 1, std::atoi(e));
+  const int cut_team = p.cut_team ? p.cut_team : kDefCutTeam;
+  const int cut_block = std::max(cut_team, std::min(kCap, p.cut_block ? p.cut_block : kDefCutBlock));
+  bigst.target = p.bucket_target ? p.bucket_target : kDefBucketTarget;
   Cuts cuts{};
   if (algorithm == DTE_RANDOM_FOREST) {
     const int c16 = (task == 1 || C <= 16) ? 16 : 0;
     const int c32 = (task == 1 || C <= 32) ? 32 : 0;
-    cuts = Cuts{{c16, std::max(c16, c32), 64, 64 * kIpt, std::max(64 * kIpt, big_min - 1), 0x7FFFFFFF}, 6};
+    cuts = Cuts{{c16, std::max(c16, c32), 64, cut_team, cut_block, 0x7FFFFFFF}, 6};
   } else {
-    cuts = Cuts{{4096, 0x7FFFFFFF, 0, 0, 0, 0}, 2};
+    cuts = Cuts{{p.cut_ert ? p.cut_ert : kDefCutErt, 0x7FFFFFFF, 0, 0, 0, 0}, 2};
   }
   DevBuf<uint32_t> tier_list[kMaxTiers];
   DevBuf<unsigned int> tier_cnt(kMaxTiers);
@@ -3612,9 +3614,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         grid = (int)std::min<int64_t>((items + per_block - 1) / per_block, cap);
       }
       Timer tk(search_family(algorithm, tier), 0.0, s);
-      // tier 4 holds nodes above the LDS capacity when the pipeline is
-      // raised past it (DTE_BIG_MIN): those need the bucketed kernel
-      search(algorithm, (rf && tier == 4 && big_min > kCap + 1) ? 6 : tier, grid, s, d, lv, a);
+      search(algorithm, tier, grid, s, d, lv, a);
     }
   };
   // Plain launch structure: chunks of at most kPlainChunk open nodes, one
@@ -3699,7 +3699,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       retry_list.reserve(n_open + 1);
       // scratch slabs of the in-kernel bucketed path (nodes above kCap that
       // do not go through the chunk pipeline)
-      if (algorithm == DTE_RANDOM_FOREST && max_m > kCap && (plain || big_min > kCap + 1)) {
+      if (algorithm == DTE_RANDOM_FOREST && max_m > kCap && plain) {
         const int64_t need = (int64_t)search_grid_cap * max_m;
         if (scr_stride < max_m) {
           scr_key.alloc(need);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <thread>
 #include <atomic>
 #include <cstring>
@@ -100,7 +101,14 @@ void check_params(const dte_params* p) {
     fail(DTE_ERR_INVALID_ARGUMENT, "unknown build_mode");
   if (p->tree_offset < 0 || (int64_t)p->tree_offset + p->n_trees > 0x7FFFFFFF)
     fail(DTE_ERR_INVALID_ARGUMENT, "tree_offset out of range");
-  for (int i = 0; i < 6; ++i)
+  auto in = [](int32_t v, int32_t lo, int32_t hi) { return v == 0 || (v >= lo && v <= hi); };
+  if (!in(p->cut_team, 64, 512)) fail(DTE_ERR_INVALID_ARGUMENT, "cut_team must be 0 or in [64, 512]");
+  if (!in(p->cut_block, 64, 2048)) fail(DTE_ERR_INVALID_ARGUMENT, "cut_block must be 0 or in [64, 2048]");
+  if (p->cut_team && p->cut_block && p->cut_block < p->cut_team)
+    fail(DTE_ERR_INVALID_ARGUMENT, "cut_block must not be below cut_team");
+  if (!in(p->bucket_target, 64, 512)) fail(DTE_ERR_INVALID_ARGUMENT, "bucket_target must be 0 or in [64, 512]");
+  if (!in(p->cut_ert, 64, 1 << 20)) fail(DTE_ERR_INVALID_ARGUMENT, "cut_ert must be 0 or in [64, 2^20]");
+  for (int i = 0; i < 2; ++i)
     if (p->reserved[i] != 0) fail(DTE_ERR_INVALID_ARGUMENT, "reserved fields must be zero");
 }
 
@@ -366,6 +374,68 @@ int dte_fit(const float* samples, const float* targets, int64_t n_rows, int32_t 
 int dte_fit_device(const float* samples_dev, const float* targets, int64_t n_rows, int32_t n_features,
                    const dte_params* params, dte_forest** out) {
   return guarded([&] { fit_common(samples_dev, true, targets, n_rows, n_features, params, out); });
+}
+
+int dte_autotune(const float* samples_dev, const float* targets, int64_t n_rows, int32_t n_features,
+                 const dte_params* params, int32_t trial_trees, dte_params* tuned, double* trial_seconds) {
+  return guarded([&] {
+    check_params(params);
+    if (!samples_dev || !targets || !tuned) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    if (n_rows <= 0 || n_features <= 0) fail(DTE_ERR_INVALID_ARGUMENT, "empty data set");
+    if (n_rows > 0x7FFFFFFF) fail(DTE_ERR_INVALID_ARGUMENT, "n_rows above 2^31-1");
+    if (trial_trees < 0) fail(DTE_ERR_INVALID_ARGUMENT, "trial_trees < 0");
+    set_device(params->device_id);
+    hipStream_t s = library_stream(params->device_id);
+    const auto t_start = std::chrono::steady_clock::now();
+    dte_params best = *params;
+    best.build_mode = DTE_BUILD_BATCHED;
+    if (!best.cut_team) best.cut_team = kDefCutTeam;
+    if (!best.cut_block) best.cut_block = std::max(kDefCutBlock, best.cut_team);
+    if (!best.bucket_target) best.bucket_target = kDefBucketTarget;
+    if (!best.cut_ert) best.cut_ert = kDefCutErt;
+    best.n_trees = std::min(params->n_trees, trial_trees > 0 ? trial_trees : 16);
+    auto timed = [&](const dte_params& q) {
+      const auto t0 = std::chrono::steady_clock::now();
+      std::unique_ptr<Forest> f(fit_forest(samples_dev, targets, n_rows, n_features, q, s));
+      DTE_HIP(hipStreamSynchronize(s));
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    (void)timed(best);  // warm-up: device scratch, code objects
+    double bt = timed(best);
+    struct Knob {
+      int32_t dte_params::*field;
+      std::vector<int32_t> values;
+    };
+    std::vector<Knob> knobs;
+    if (params->algorithm == DTE_RANDOM_FOREST) {
+      knobs.push_back({&dte_params::cut_team, {128, 256, 512}});
+      knobs.push_back({&dte_params::cut_block, {1024, 1536, 2048}});
+      knobs.push_back({&dte_params::bucket_target, {128, 192, 256, 384}});
+    } else {
+      knobs.push_back({&dte_params::cut_ert, {1024, 4096, 16384, 65536}});
+    }
+    for (const Knob& k : knobs) {
+      const int32_t cur = best.*k.field;
+      for (int32_t v : k.values) {
+        if (v == cur) continue;
+        dte_params q = best;
+        q.*k.field = v;
+        if (q.cut_block < q.cut_team) continue;
+        const double t = timed(q);
+        if (t < 0.98 * bt) {
+          best = q;
+          bt = t;
+        }
+      }
+    }
+    *tuned = *params;
+    tuned->cut_team = best.cut_team;
+    tuned->cut_block = best.cut_block;
+    tuned->bucket_target = best.bucket_target;
+    tuned->cut_ert = best.cut_ert;
+    if (trial_seconds)
+      *trial_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+  });
 }
 
 int dte_fit_many(int32_t n, const float* const* samples_dev, const float* const* targets, const int64_t* n_rows,

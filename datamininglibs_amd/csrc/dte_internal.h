@@ -216,6 +216,12 @@ const LnTable& host_ln_table();
 
 constexpr int kBlock = 256;
 
+// Defaults of the split-search tuning fields of dte_params (0 selects them).
+constexpr int kDefCutTeam = 512;
+constexpr int kDefCutBlock = 2048;
+constexpr int kDefBucketTarget = 256;
+constexpr int kDefCutErt = 4096;
+
 }  // namespace dte
 
 // Opaque handle of the C ABI.

@@ -76,7 +76,15 @@ typedef struct dte_params {
                                   built with offsets 0, n0, n0+n1, ... concatenate to the forest of one
                                   build (what SplitParameterPack + AggregateModels need, gpu_algorithm.cc:
                                   17-31; the reference restarts its RNG per batch instead, D7) */
-  int32_t reserved[6];         /* must be zero */
+  /* Split-search tuning; 0 keeps the default, dte_autotune picks them for a
+   * data set.  Results do not depend on them, only the time does. */
+  int32_t cut_team;            /* RF: largest node (distinct rows) searched by one wave per draw,
+                                  64..512 (default 512); smaller nodes use 16/32/64-lane segments */
+  int32_t cut_block;           /* RF: largest node searched by one workgroup per draw, cut_team..2048
+                                  (default 2048); larger nodes take the bucketed chunk pipeline */
+  int32_t bucket_target;       /* RF: rows per key bucket aimed at in that pipeline, 64..512 (256) */
+  int32_t cut_ert;             /* ERT: largest node searched by one wave per draw, 64..2^20 (4096) */
+  int32_t reserved[2];         /* must be zero */
 } dte_params;
 
 /* One node of the exported model: gpuDTE_NodeHeader_Classify<float>
@@ -131,6 +139,17 @@ int dte_fit_device(const float* samples_dev, const float* targets, int64_t n_row
  * model per device thread (lib_algorithms/source/gpu_algorithm.cc:18-31) and
  * otherwise one after another.  All params[i].device_id must be equal.  On
  * failure no forest is returned. */
+/* Split-search tuning for one data set (the autotuner of the batched
+ * build): trial fits of trial_trees trees (0: 16) of *params on these
+ * samples, one candidate setting of cut_team, cut_block, bucket_target (RF)
+ * or cut_ert (ERT) at a time, each kept when it is at least 2 % faster.
+ * *tuned = *params with the chosen tuning fields; *trial_seconds (may be
+ * NULL) = the time spent.  No counterpart in the reference, whose launch
+ * shape is fixed (one block of 64 threads per node, gpudte_algorithm.cc:
+ * 149-151). */
+int dte_autotune(const float* samples_dev, const float* targets, int64_t n_rows, int32_t n_features,
+                 const dte_params* params, int32_t trial_trees, dte_params* tuned, double* trial_seconds);
+
 int dte_fit_many(int32_t n, const float* const* samples_dev, const float* const* targets, const int64_t* n_rows,
                  const int32_t* n_features, const dte_params* params, int32_t max_concurrent, dte_forest** out);
 

@@ -66,7 +66,9 @@ def _last_error():
 
 @pytest.mark.parametrize("field,value", [("algorithm", 7), ("task", 3), ("n_trees", 0), ("max_depth", -1),
                                          ("min_node_size", -2), ("n_classes", 0), ("n_classes", 33),
-                                         ("build_mode", 9), ("n_features_per_node", -1), ("tree_offset", -1)])
+                                         ("build_mode", 9), ("n_features_per_node", -1), ("tree_offset", -1),
+                                         ("cut_team", 32), ("cut_team", 1024), ("cut_block", 4096),
+                                         ("bucket_target", 16), ("bucket_target", 1024), ("cut_ert", 8)])
 def test_invalid_params_rejected_before_device(field, value):
     p = capi.default_params(capi.RANDOM_FOREST)
     setattr(p, field, value)
@@ -103,3 +105,20 @@ def test_error_state_is_per_call():
     n = ctypes.c_int32(-1)
     assert capi.lib().dte_device_count(ctypes.byref(n)) == capi.OK
     assert n.value >= 0
+
+
+def test_cut_block_below_cut_team_rejected():
+    p = capi.default_params(capi.RANDOM_FOREST, cut_team=512, cut_block=256)
+    assert _fit_rc(p) == capi.ERR_INVALID_ARGUMENT
+    assert "cut_block" in _last_error()
+
+
+def test_autotune_validates_before_device():
+    L = capi.lib()
+    p = capi.default_params(capi.RANDOM_FOREST)
+    out = capi.Params()
+    X = np.zeros((2, 8), np.float32)
+    y = np.zeros(8, np.float32)
+    assert L.dte_autotune(X.ctypes.data_as(ctypes.c_void_p), y.ctypes.data_as(ctypes.c_void_p), 8, 2,
+                          ctypes.byref(p), -1, ctypes.byref(out), None) == capi.ERR_INVALID_ARGUMENT
+    assert L.dte_autotune(None, None, 8, 2, ctypes.byref(p), 0, ctypes.byref(out), None) == capi.ERR_INVALID_ARGUMENT

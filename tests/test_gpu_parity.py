@@ -110,3 +110,49 @@ def test_cached_scratch_reuse_and_release():
     assert capi.release_cached_memory() == 0
     g, o = fit_both(Xa, ya, **kwa)
     assert_same_forest(g, o)
+
+
+@pytest.mark.parametrize("tuning", [dict(cut_team=128), dict(cut_team=64, cut_block=512), dict(cut_block=1024),
+                                    dict(bucket_target=64), dict(bucket_target=512),
+                                    dict(cut_team=256, cut_block=1536, bucket_target=192)])
+def test_rf_tuning_does_not_change_the_forest(tuning):
+    """Every tier split and bucket size gives the oracle's forest (20k rows:
+    nodes in every tier, the chunk pipeline at the top levels)."""
+    X, y = small_classes(5, 20000, 12, 4)
+    kw = dict(n_trees=2, n_features_per_node=3, max_depth=12, min_node_size=5, n_classes=4, seed=7)
+    g = capi.fit(X, y, capi.default_params(capi.RANDOM_FOREST, **kw, **tuning))
+    o = pyoracle.fit(X, y, algorithm=capi.RANDOM_FOREST, n_trees=2, k=3, max_depth=12, min_node_size=5,
+                     n_classes=4, seed=7)
+    assert_same_forest(g, o)
+
+
+@pytest.mark.parametrize("cut_ert", [64, 1024, 65536])
+def test_ert_tuning_does_not_change_the_forest(cut_ert):
+    X, y = small_classes(6, 12000, 10, 3)
+    kw = dict(n_trees=3, n_features_per_node=3, max_depth=14, min_node_size=5, n_classes=3, seed=5)
+    g = capi.fit(X, y, capi.default_params(capi.EXTRA_TREES, cut_ert=cut_ert, **kw))
+    o = pyoracle.fit(X, y, algorithm=capi.EXTRA_TREES, n_trees=3, k=3, max_depth=14, min_node_size=5,
+                     n_classes=3, seed=5)
+    assert_same_forest(g, o)
+
+
+@pytest.mark.parametrize("algorithm", [capi.EXTRA_TREES, capi.RANDOM_FOREST])
+def test_autotune_picks_a_valid_setting(algorithm):
+    X, y = small_classes(8, 30000, 16, 5)
+    p = capi.default_params(algorithm, n_trees=40, n_features_per_node=4, max_depth=16, min_node_size=5,
+                            n_classes=5, seed=3)
+    Xd = capi.DeviceArray.from_numpy(X)
+    tuned, secs = capi.autotune(Xd.ptr, X.shape[1], X.shape[0], y, p, trial_trees=4)
+    assert secs > 0
+    if algorithm == capi.RANDOM_FOREST:
+        assert tuned.cut_team in (128, 256, 512)
+        assert tuned.cut_block in (1024, 1536, 2048) and tuned.cut_block >= tuned.cut_team
+        assert tuned.bucket_target in (128, 192, 256, 384)
+    else:
+        assert tuned.cut_ert in (1024, 4096, 16384, 65536)
+    assert tuned.n_trees == 40 and tuned.seed == 3
+    a = capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, tuned).canonical()
+    b = capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p).canonical()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    Xd.free()
