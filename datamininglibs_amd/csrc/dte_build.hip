@@ -45,6 +45,8 @@ const LnTable& host_ln_table();
 namespace {
 
 constexpr int kIpt = 8;                 // elements per thread in the LDS sort
+constexpr int kTeamIpt = 8;             // entries per lane of the wave-per-item tier
+static_assert(kWave * kTeamIpt >= kMaxCutTeam, "wave tier capacity");
 constexpr int kCap = kBlock * kIpt;     // LDS chunk capacity (2048)
 constexpr int kHistWords = 8192;        // LDS histogram words (32 KB)
 constexpr int kMaxBuckets = 2048;       // bucket table of the big-item path
@@ -768,7 +770,7 @@ __device__ __forceinline__ void rf_write_result(const DataArgs& d, const ItemArg
 // first.  Every boundary between distinct values is scored.
 template <int MAXC, int TASK, int TEAM, bool BIG>
 __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, ItemArgs it) {
-  constexpr int IPT = kIpt;
+  constexpr int IPT = TEAM == kWave ? kTeamIpt : kIpt;
   constexpr int NT = kBlock / TEAM;
   __shared__ double s_inv[128], s_logc[128];
   __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[NT];

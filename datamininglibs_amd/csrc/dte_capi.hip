@@ -102,7 +102,7 @@ void check_params(const dte_params* p) {
   if (p->tree_offset < 0 || (int64_t)p->tree_offset + p->n_trees > 0x7FFFFFFF)
     fail(DTE_ERR_INVALID_ARGUMENT, "tree_offset out of range");
   auto in = [](int32_t v, int32_t lo, int32_t hi) { return v == 0 || (v >= lo && v <= hi); };
-  if (!in(p->cut_team, 64, 512)) fail(DTE_ERR_INVALID_ARGUMENT, "cut_team must be 0 or in [64, 512]");
+  if (!in(p->cut_team, 64, kMaxCutTeam)) fail(DTE_ERR_INVALID_ARGUMENT, "cut_team must be 0 or in [64, 512]");
   if (!in(p->cut_block, 64, 2048)) fail(DTE_ERR_INVALID_ARGUMENT, "cut_block must be 0 or in [64, 2048]");
   if (p->cut_team && p->cut_block && p->cut_block < p->cut_team)
     fail(DTE_ERR_INVALID_ARGUMENT, "cut_block must not be below cut_team");
@@ -408,9 +408,9 @@ int dte_autotune(const float* samples_dev, const float* targets, int64_t n_rows,
     };
     std::vector<Knob> knobs;
     if (params->algorithm == DTE_RANDOM_FOREST) {
-      knobs.push_back({&dte_params::cut_team, {128, 256, 512}});
+      knobs.push_back({&dte_params::bucket_target, {192, 256, 320, 384, 448}});
+      knobs.push_back({&dte_params::cut_team, {256, 384, 512}});
       knobs.push_back({&dte_params::cut_block, {1024, 1536, 2048}});
-      knobs.push_back({&dte_params::bucket_target, {128, 192, 256, 384}});
     } else {
       knobs.push_back({&dte_params::cut_ert, {1024, 4096, 16384, 65536}});
     }

@@ -218,6 +218,7 @@ constexpr int kBlock = 256;
 
 // Defaults of the split-search tuning fields of dte_params (0 selects them).
 constexpr int kDefCutTeam = 512;
+constexpr int kMaxCutTeam = 512;  // capacity of the wave-per-item tier (64 lanes x 8)
 constexpr int kDefCutBlock = 2048;
 constexpr int kDefBucketTarget = 256;
 constexpr int kDefCutErt = 4096;

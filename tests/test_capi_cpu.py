@@ -67,7 +67,7 @@ def _last_error():
 @pytest.mark.parametrize("field,value", [("algorithm", 7), ("task", 3), ("n_trees", 0), ("max_depth", -1),
                                          ("min_node_size", -2), ("n_classes", 0), ("n_classes", 33),
                                          ("build_mode", 9), ("n_features_per_node", -1), ("tree_offset", -1),
-                                         ("cut_team", 32), ("cut_team", 1024), ("cut_block", 4096),
+                                         ("cut_team", 32), ("cut_team", 513), ("cut_block", 4096),
                                          ("bucket_target", 16), ("bucket_target", 1024), ("cut_ert", 8)])
 def test_invalid_params_rejected_before_device(field, value):
     p = capi.default_params(capi.RANDOM_FOREST)

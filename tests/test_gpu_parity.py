@@ -113,6 +113,7 @@ def test_cached_scratch_reuse_and_release():
 
 
 @pytest.mark.parametrize("tuning", [dict(cut_team=128), dict(cut_team=64, cut_block=512), dict(cut_block=1024),
+                                    dict(cut_team=384), dict(bucket_target=448),
                                     dict(bucket_target=64), dict(bucket_target=512),
                                     dict(cut_team=256, cut_block=1536, bucket_target=192)])
 def test_rf_tuning_does_not_change_the_forest(tuning):
@@ -145,9 +146,9 @@ def test_autotune_picks_a_valid_setting(algorithm):
     tuned, secs = capi.autotune(Xd.ptr, X.shape[1], X.shape[0], y, p, trial_trees=4)
     assert secs > 0
     if algorithm == capi.RANDOM_FOREST:
-        assert tuned.cut_team in (128, 256, 512)
+        assert tuned.cut_team in (256, 384, 512)
         assert tuned.cut_block in (1024, 1536, 2048) and tuned.cut_block >= tuned.cut_team
-        assert tuned.bucket_target in (128, 192, 256, 384)
+        assert tuned.bucket_target in (192, 256, 320, 384, 448)
     else:
         assert tuned.cut_ert in (1024, 4096, 16384, 65536)
     assert tuned.n_trees == 40 and tuned.seed == 3
