@@ -848,7 +848,8 @@ struct V2Item {
   int64_t soff;   // first scratch slot of the item
   int32_t boff;   // first bucket slot of the item
   int32_t nb;     // buckets
-  int32_t shift;  // bucket = key >> shift
+  uint32_t width;  // bucket = key / width (v2_bucket)
+  uint32_t magic;  // floor(2^32 / width)
   uint32_t node;  // level node index
   int64_t ridx;   // result slot (open position * D + draw)
   // node context (as node_context computes it)
@@ -857,6 +858,14 @@ struct V2Item {
   int64_t ssum;   // regression: fixed-point target sum
   double inv;     // 1 / (n ln 2)
 };
+
+// key / width from the high word of key * floor(2^32 / width), which is the
+// quotient or one below it, and one exact correction.
+__device__ __forceinline__ uint32_t v2_bucket(uint32_t key, const V2Item& it) {
+  uint32_t q = __umulhi(key, it.magic);
+  if ((uint64_t)(q + 1u) * it.width <= key) ++q;
+  return q;
+}
 
 struct V2Tile {
   uint32_t item;
@@ -969,7 +978,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2
       if (e < n) {
         const uint32_t key = Kf[rows[q]];
         gkey[e] = key;
-        const uint32_t b = key >> it.shift;
+        const uint32_t b = v2_bucket(key, it);
         atomicAdd(&s_cnt[b], 1u);
         atomicMax(&s_max[b], key);
       }
@@ -1078,7 +1087,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
 #pragma unroll
   for (int q = 0; q < U; ++q) {
     const int e = t + q * kBlock;
-    if (e < n) rank[q] = atomicAdd(&s_cnt[keys[q] >> it.shift], 1u);
+    if (e < n) rank[q] = atomicAdd(&s_cnt[v2_bucket(keys[q], it)], 1u);
   }
   __syncthreads();
   {
@@ -1104,7 +1113,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
   for (int q = 0; q < U; ++q) {
     const int e = t + q * kBlock;
     if (e < n) {
-      const uint32_t b = keys[q] >> it.shift;
+      const uint32_t b = v2_bucket(keys[q], it);
       const uint32_t li = s_cnt[b] + rank[q];
       s_key[li] = keys[q];
       s_aux[li] = auxv[q];
@@ -1289,7 +1298,7 @@ __device__ __forceinline__ void v2_prefetch(const DataArgs& d, const LevelArgs& 
   const V2Task tk = a.tasks[q];
   const V2Item& it = a.items[tk.item];
   P.gb = (int64_t)it.boff + tk.b;
-  P.kbase = tk.b << it.shift;
+  P.kbase = tk.b * it.width;
   P.node = it.node;
   P.n_w = it.n_w;
   P.P = it.P;
@@ -1398,11 +1407,30 @@ __global__ void __launch_bounds__(kBlock) k_v2_spill(DataArgs d, LevelArgs lv, V
     const int64_t gb = (int64_t)it.boff + tk.b;
     const int n = (int)a.bcount[gb];
     const int64_t e0 = it.soff + a.bbase[gb];
-    const uint32_t klo = tk.b << it.shift;
+    const uint32_t klo = tk.b * it.width;
     const uint32_t span = a.bmax[gb] - klo + 1u;
     const NodeCtx nc = v2_node(d, lv, it, L);
     v2_carry<MAXC, TASK, IPT>(d, a, gb, L);
     __syncthreads();
+    if (n <= CAP) {
+      // within the workgroup's LDS chunk: one sort and walk
+      for (int e = t; e < n; e += 256) {
+        L.u.srt.k[0][e] = a.skey[e0 + e] - klo;
+        L.u.srt.v[0][e] = (uint16_t)e;
+        L.aux[e] = a.saux[e0 + e];
+        if (TASK == 1) L.y[e] = a.sy[e0 + e];
+      }
+      __syncthreads();
+      sort_and_eval<MAXC, TASK, 256, IPT>(n, bit_length(span - 1u), klo, nc, L, lx);
+      __syncthreads();
+      if (t == 0) {
+        a.res_g[gb] = L.cr.best_g;
+        a.res_key[gb] = L.cr.best_key;
+        a.res_prev[gb] = L.cr.best_prev;
+      }
+      __syncthreads();
+      continue;
+    }
     const int words = (TASK == 0) ? d.C : 3;
     int wbits = 0;
     while ((1 << (wbits + 1)) * words <= kHistWords) ++wbits;
@@ -1528,16 +1556,15 @@ __global__ void k_v2_items(DataArgs d, LevelArgs lv, const uint32_t* groups, int
                 (uint32_t)d.F);
     b.m = lv.m[i];
     b.start = lv.start[i];
-    // nb: a power of two near m / target, then the shift that maps the
-    // feature's dense keys [0, S) onto at most nb buckets.
-    int nb = 1;
-    while (nb < nb_cap && nb * target < b.m) nb <<= 1;
-    const uint32_t S = d.S[b.f];
-    const int kb = bit_length(S > 1 ? S - 1u : 1u);
-    int lg = 0;
-    while ((1 << lg) < nb) ++lg;
-    b.shift = max(0, kb - lg);
-    b.nb = (int)(((S > 0 ? S - 1u : 0u) >> b.shift) + 1u);
+    // Buckets of `width` consecutive keys of the feature's dense range
+    // [0, S), so that a bucket holds about `target` of the node's m entries
+    // (m * width / S), and at most nb_cap buckets.
+    const uint32_t S = max(d.S[b.f], 1u);
+    const uint64_t w_target = (uint64_t)S * (uint64_t)target / (uint64_t)max(b.m, 1);
+    const uint32_t w_min = (uint32_t)(((uint64_t)S + nb_cap - 1) / nb_cap);
+    b.width = (uint32_t)max<uint64_t>(max<uint64_t>(w_target, (uint64_t)w_min), 1);
+    b.magic = b.width == 1 ? 0xFFFFFFFFu : (uint32_t)(0x100000000ull / b.width);
+    b.nb = (int)((S + b.width - 1) / b.width);
     b.soff = 0;
     b.boff = 0;
     b.node = i;

@@ -408,7 +408,7 @@ int dte_autotune(const float* samples_dev, const float* targets, int64_t n_rows,
     };
     std::vector<Knob> knobs;
     if (params->algorithm == DTE_RANDOM_FOREST) {
-      knobs.push_back({&dte_params::bucket_target, {192, 256, 320, 384, 448}});
+      knobs.push_back({&dte_params::bucket_target, {192, 256, 320, 352, 384, 416, 448}});
       knobs.push_back({&dte_params::cut_team, {256, 384, 512}});
       knobs.push_back({&dte_params::cut_block, {1024, 1536, 2048}});
     } else {
