@@ -307,6 +307,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="also run a 1 GiB copy kernel")
     ap.add_argument("--serve-requests", type=int, default=1000, help="requests per serving QPS point (0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="default tier cuts instead of dte_autotune's")
+    ap.add_argument("--tuning", default="", help="explicit cut_team,cut_block,bucket_target (no autotuner run)")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
                     help="c2 (default): the headline random-forest line; c1/c3/c4: the other workload shapes")
     args = ap.parse_args()
@@ -335,7 +336,11 @@ def main():
 
     params = capi.default_params(capi.RANDOM_FOREST, **common)
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        params.cut_team, params.cut_block, params.bucket_target = (int(v) for v in args.tuning.split(","))
+        tuning = {"cut_team": params.cut_team, "cut_block": params.cut_block,
+                  "bucket_target": params.bucket_target, "autotune_s": None}
+    elif not args.no_autotune:
         # split-search tier cuts and bucket size for this data set, from
         # trial fits of 16 trees (outside the timed steps)
         params, tune_s = capi.autotune(Xd.ptr, args.rows, args.features, y, params, trial_trees=16)

@@ -394,13 +394,14 @@ int dte_autotune(const float* samples_dev, const float* targets, int64_t n_rows,
     if (!best.bucket_target) best.bucket_target = kDefBucketTarget;
     if (!best.cut_ert) best.cut_ert = kDefCutErt;
     best.n_trees = std::min(params->n_trees, trial_trees > 0 ? trial_trees : 16);
-    auto timed = [&](const dte_params& q) {
+    auto once = [&](const dte_params& q) {
       const auto t0 = std::chrono::steady_clock::now();
       std::unique_ptr<Forest> f(fit_forest(samples_dev, targets, n_rows, n_features, q, s));
       DTE_HIP(hipStreamSynchronize(s));
       return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     };
-    (void)timed(best);  // warm-up: device scratch, code objects
+    auto timed = [&](const dte_params& q) { return std::min(once(q), once(q)); };  // best of two
+    (void)once(best);  // warm-up: device scratch, code objects
     double bt = timed(best);
     struct Knob {
       int32_t dte_params::*field;
@@ -408,9 +409,9 @@ int dte_autotune(const float* samples_dev, const float* targets, int64_t n_rows,
     };
     std::vector<Knob> knobs;
     if (params->algorithm == DTE_RANDOM_FOREST) {
-      knobs.push_back({&dte_params::bucket_target, {192, 256, 320, 352, 384, 416, 448}});
+      knobs.push_back({&dte_params::bucket_target, {256, 320, 384, 448}});
       knobs.push_back({&dte_params::cut_team, {256, 384, 512}});
-      knobs.push_back({&dte_params::cut_block, {1024, 1536, 2048}});
+      knobs.push_back({&dte_params::cut_block, {1536, 2048}});
     } else {
       knobs.push_back({&dte_params::cut_ert, {1024, 4096, 16384, 65536}});
     }
@@ -422,7 +423,7 @@ int dte_autotune(const float* samples_dev, const float* targets, int64_t n_rows,
         q.*k.field = v;
         if (q.cut_block < q.cut_team) continue;
         const double t = timed(q);
-        if (t < 0.98 * bt) {
+        if (t < 0.97 * bt) {
           best = q;
           bt = t;
         }

@@ -220,7 +220,7 @@ constexpr int kBlock = 256;
 constexpr int kDefCutTeam = 512;
 constexpr int kMaxCutTeam = 512;  // capacity of the wave-per-item tier (64 lanes x 8)
 constexpr int kDefCutBlock = 2048;
-constexpr int kDefBucketTarget = 256;
+constexpr int kDefBucketTarget = 384;
 constexpr int kDefCutErt = 4096;
 
 }  // namespace dte

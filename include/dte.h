@@ -82,7 +82,7 @@ typedef struct dte_params {
                                   64..512 (default 512); smaller nodes use 16/32/64-lane segments */
   int32_t cut_block;           /* RF: largest node searched by one workgroup per draw, cut_team..2048
                                   (default 2048); larger nodes take the bucketed chunk pipeline */
-  int32_t bucket_target;       /* RF: rows per key bucket aimed at in that pipeline, 64..512 (256) */
+  int32_t bucket_target;       /* RF: rows per key bucket aimed at in that pipeline, 64..512 (384) */
   int32_t cut_ert;             /* ERT: largest node searched by one wave per draw, 64..2^20 (4096) */
   int32_t reserved[2];         /* must be zero */
 } dte_params;
@@ -141,8 +141,9 @@ int dte_fit_device(const float* samples_dev, const float* targets, int64_t n_row
  * failure no forest is returned. */
 /* Split-search tuning for one data set (the autotuner of the batched
  * build): trial fits of trial_trees trees (0: 16) of *params on these
- * samples, one candidate setting of cut_team, cut_block, bucket_target (RF)
- * or cut_ert (ERT) at a time, each kept when it is at least 2 % faster.
+ * samples (best of two runs each), one candidate setting of cut_team,
+ * cut_block, bucket_target (RF) or cut_ert (ERT) at a time, each kept when it
+ * is at least 3 % faster than the setting so far (the defaults first).
  * *tuned = *params with the chosen tuning fields; *trial_seconds (may be
  * NULL) = the time spent.  No counterpart in the reference, whose launch
  * shape is fixed (one block of 64 threads per node, gpudte_algorithm.cc:

@@ -147,8 +147,8 @@ def test_autotune_picks_a_valid_setting(algorithm):
     assert secs > 0
     if algorithm == capi.RANDOM_FOREST:
         assert tuned.cut_team in (256, 384, 512)
-        assert tuned.cut_block in (1024, 1536, 2048) and tuned.cut_block >= tuned.cut_team
-        assert tuned.bucket_target in (192, 256, 320, 352, 384, 416, 448)
+        assert tuned.cut_block in (1536, 2048) and tuned.cut_block >= tuned.cut_team
+        assert tuned.bucket_target in (256, 320, 384, 448)
     else:
         assert tuned.cut_ert in (1024, 4096, 16384, 65536)
     assert tuned.n_trees == 40 and tuned.seed == 3
