@@ -2530,6 +2530,86 @@ __global__ void __launch_bounds__(kBlock) k_partition_wave(DataArgs d, LevelArgs
   if (lane == 0 && my_bytes && pa.bytes) atomicAdd(pa.bytes, my_bytes);
 }
 
+// Nodes of at most SEG entries: one SEG-lane segment per node, one entry
+// per lane, a single pass (side, left statistics, stable positions from the
+// segment's ballots).  Same outputs as k_partition_wave.
+template <int MAXC, int TASK, int SEG>
+__global__ void __launch_bounds__(kBlock) k_partition_seg(DataArgs d, LevelArgs lv, PartArgs pa,
+                                                          const uint32_t* list, int64_t n_list) {
+  constexpr int NPW = kWave / SEG;
+  __shared__ uint32_t s_cls[kWaves][NPW][TASK == 0 ? MAXC : 1];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int sl = lane & (SEG - 1);
+  const int seg = lane / SEG;
+  const int wv = threadIdx.x / kWave;
+  const uint64_t seg_mask = (SEG == kWave ? ~0ull : ((1ull << SEG) - 1ull)) << (seg * SEG);
+  const uint64_t below = (1ull << lane) - 1ull;
+  unsigned long long my_bytes = 0;
+  const int64_t wave_id = (int64_t)blockIdx.x * kWaves + wv;
+  const int64_t n_waves = (int64_t)gridDim.x * kWaves;
+  for (int64_t q0 = wave_id * NPW; q0 < n_list; q0 += n_waves * NPW) {
+    const int64_t q = q0 + seg;
+    const bool node_ok = q < n_list;
+    uint32_t o = 0;
+    int m = 0;
+    int64_t st = 0;
+    float thr = 0.0f;
+    const float* Xf = d.X;
+    if (node_ok) {
+      o = list[q];
+      const uint32_t i = lv.open_idx[o];
+      m = lv.m[i];
+      st = lv.start[i];
+      thr = pa.ch_thr[o];
+      Xf = d.X + (int64_t)pa.ch_feat[o] * d.N;
+    }
+    const bool valid = node_ok && sl < m;
+    uint32_t row = 0, a = 0;
+    int64_t ey = 0;
+    bool isl = false;
+    if (valid) {
+      row = lv.row[st + sl];
+      a = lv.aux[st + sl];
+      if (TASK == 1) ey = lv.ey[st + sl];
+      isl = side_of(Xf[row], thr) == 0;
+    }
+    const uint64_t bl = __ballot(isl) & seg_mask;
+    const uint64_t bv = __ballot(valid) & seg_mask;
+    const int mL = __popcll(bl);
+    const bool ok = mL > 0 && mL < m;
+    const uint32_t w = isl ? (a & 0xFFFFu) : 0u;
+    const int64_t cnt = seg_sum_i64<SEG>((int64_t)w);
+    const int64_t hsh = seg_sum_i64<SEG>(isl ? (int64_t)((uint64_t)w * mix64(row)) : 0);
+    const int64_t sm = TASK == 1 ? seg_sum_i64<SEG>(isl ? (int64_t)w * ey : 0) : 0;
+    if constexpr (TASK == 0) {
+      uint32_t* sc = s_cls[wv][seg];
+      for (int c = sl; c < MAXC; c += SEG) sc[c] = 0u;
+      team_sync<kWave>();
+      if (isl) atomicAdd(&sc[a >> 16], w);
+      team_sync<kWave>();
+      if (node_ok)
+        for (int c = sl; c < d.C; c += SEG) pa.l_cls[(int64_t)o * d.C + c] = sc[c];
+      team_sync<kWave>();
+    }
+    if (node_ok && sl == 0) {
+      my_bytes += (unsigned long long)m * (TASK == 0 ? 12ull : 20ull);
+      pa.split_ok[o] = ok ? 1 : 0;
+      pa.l_m[o] = mL;
+      pa.l_count[o] = (int32_t)cnt;
+      pa.l_hash[o] = (uint64_t)hsh;
+      if (TASK == 1) pa.l_sum[o] = sm;
+    }
+    if (ok && valid) {
+      const uint32_t pos = isl ? (uint32_t)__popcll(bl & below)
+                               : (uint32_t)mL + (uint32_t)__popcll(bv & ~bl & below);
+      pa.nrow[st + pos] = row;
+      pa.naux[st + pos] = a;
+      if (TASK == 1) pa.ney[st + pos] = ey;
+    }
+  }
+  if (sl == 0 && my_bytes && pa.bytes) atomicAdd(pa.bytes, my_bytes);
+}
+
 // ------------------------------------------------------ children --------
 struct ChildArgs {
   int64_t n_level;         // nodes at this level
@@ -3000,6 +3080,14 @@ void launch_partition_wave(int grid, hipStream_t s, const DataArgs& d, const Lev
   hipLaunchKernelGGL((k_partition_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, pa, list, n);
 }
 
+// Nodes of at most kPartSeg entries, kWave / kPartSeg per wave.
+constexpr int kPartSeg = 32;
+template <int MAXC, int TASK>
+void launch_partition_seg(int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv, const PartArgs& pa,
+                          const uint32_t* list, int64_t n) {
+  hipLaunchKernelGGL((k_partition_seg<MAXC, TASK, kPartSeg>), dim3(grid), dim3(kBlock), 0, s, d, lv, pa, list, n);
+}
+
 
 // Scratch and bookkeeping of the big-node pipeline, reused across levels.
 struct BigState {
@@ -3217,6 +3305,14 @@ SearchFn pick_search(int task, int C) {
   if (C <= 8) return launch_search<8, 0>;
   if (C <= 16) return launch_search<16, 0>;
   return launch_search<32, 0>;
+}
+PartWaveFn pick_part_seg(int task, int C) {
+  if (task == 1) return launch_partition_seg<1, 1>;
+  if (C <= 2) return launch_partition_seg<2, 0>;
+  if (C <= 4) return launch_partition_seg<4, 0>;
+  if (C <= 8) return launch_partition_seg<8, 0>;
+  if (C <= 16) return launch_partition_seg<16, 0>;
+  return launch_partition_seg<32, 0>;
 }
 PartWaveFn pick_part_wave(int task, int C) {
   if (task == 1) return launch_partition_wave<1, 1>;
@@ -3582,6 +3678,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   BigState bigst;
   PartFn partition = pick_part(task, C);
   PartWaveFn partition_wave = pick_part_wave(task, C);
+  PartWaveFn partition_seg = pick_part_seg(task, C);
 
   // Size tiers of the split search: wave per item up to wave_cutoff entries,
   // block per item above (the block kernel itself switches to the bucketed
@@ -3815,20 +3912,24 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         for (auto& tl : tier_list) tl.reserve(n_open);
         DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
         hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(n_open), 1024)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
-                           n_open, lv.open_idx, lv.m, Cuts{{512, 0x7FFFFFFF, 0, 0, 0, 0}, 2}, tier_lists(),
+                           n_open, lv.open_idx, lv.m, Cuts{{kPartSeg, 512, 0x7FFFFFFF, 0, 0, 0}, 3}, tier_lists(),
                            tier_cnt.p);
         unsigned int pc[kMaxTiers];
         DTE_HIP(hipMemcpyAsync(pc, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
         DTE_HIP(hipStreamSynchronize(s));
         Timer tm("partition", 0.0, s);
+        constexpr int kSegPerBlock = kWaves * (kWave / kPartSeg);
         if (pc[0])
-          partition_wave((int)std::min<int64_t>((pc[0] + kWaves - 1) / kWaves, wave_grid_cap), s, d, lv, pa,
-                         tier_list[0].p, pc[0]);
-        if (pc[1]) {
+          partition_seg((int)std::min<int64_t>((pc[0] + kSegPerBlock - 1) / kSegPerBlock, wave_grid_cap), s, d, lv,
+                        pa, tier_list[0].p, pc[0]);
+        if (pc[1])
+          partition_wave((int)std::min<int64_t>((pc[1] + kWaves - 1) / kWaves, wave_grid_cap), s, d, lv, pa,
+                         tier_list[1].p, pc[1]);
+        if (pc[2]) {
           // workgroup partition over the remaining open positions
-          pa.list = tier_list[1].p;
-          pa.n_open = pc[1];
-          partition((int)std::min<int64_t>(pc[1], search_grid_cap), s, d, lv, pa);
+          pa.list = tier_list[2].p;
+          pa.n_open = pc[2];
+          partition((int)std::min<int64_t>(pc[2], search_grid_cap), s, d, lv, pa);
         }
       }
     }
