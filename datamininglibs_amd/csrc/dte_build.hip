@@ -3080,12 +3080,13 @@ void launch_partition_wave(int grid, hipStream_t s, const DataArgs& d, const Lev
   hipLaunchKernelGGL((k_partition_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, pa, list, n);
 }
 
-// Nodes of at most kPartSeg entries, kWave / kPartSeg per wave.
-constexpr int kPartSeg = 32;
-template <int MAXC, int TASK>
-void launch_partition_seg(int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv, const PartArgs& pa,
+// Nodes of at most 16 (32) entries, 4 (2) per wave; the grid covers n nodes.
+template <int MAXC, int TASK, int SEG>
+void launch_partition_seg(int grid_cap, hipStream_t s, const DataArgs& d, const LevelArgs& lv, const PartArgs& pa,
                           const uint32_t* list, int64_t n) {
-  hipLaunchKernelGGL((k_partition_seg<MAXC, TASK, kPartSeg>), dim3(grid), dim3(kBlock), 0, s, d, lv, pa, list, n);
+  constexpr int per_block = kWaves * (kWave / SEG);
+  const int grid = (int)std::min<int64_t>((n + per_block - 1) / per_block, grid_cap);
+  hipLaunchKernelGGL((k_partition_seg<MAXC, TASK, SEG>), dim3(grid), dim3(kBlock), 0, s, d, lv, pa, list, n);
 }
 
 
@@ -3306,13 +3307,14 @@ SearchFn pick_search(int task, int C) {
   if (C <= 16) return launch_search<16, 0>;
   return launch_search<32, 0>;
 }
+template <int SEG>
 PartWaveFn pick_part_seg(int task, int C) {
-  if (task == 1) return launch_partition_seg<1, 1>;
-  if (C <= 2) return launch_partition_seg<2, 0>;
-  if (C <= 4) return launch_partition_seg<4, 0>;
-  if (C <= 8) return launch_partition_seg<8, 0>;
-  if (C <= 16) return launch_partition_seg<16, 0>;
-  return launch_partition_seg<32, 0>;
+  if (task == 1) return launch_partition_seg<1, 1, SEG>;
+  if (C <= 2) return launch_partition_seg<2, 0, SEG>;
+  if (C <= 4) return launch_partition_seg<4, 0, SEG>;
+  if (C <= 8) return launch_partition_seg<8, 0, SEG>;
+  if (C <= 16) return launch_partition_seg<16, 0, SEG>;
+  return launch_partition_seg<32, 0, SEG>;
 }
 PartWaveFn pick_part_wave(int task, int C) {
   if (task == 1) return launch_partition_wave<1, 1>;
@@ -3678,7 +3680,8 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   BigState bigst;
   PartFn partition = pick_part(task, C);
   PartWaveFn partition_wave = pick_part_wave(task, C);
-  PartWaveFn partition_seg = pick_part_seg(task, C);
+  PartWaveFn partition_seg16 = pick_part_seg<16>(task, C);
+  PartWaveFn partition_seg32 = pick_part_seg<32>(task, C);
 
   // Size tiers of the split search: wave per item up to wave_cutoff entries,
   // block per item above (the block kernel itself switches to the bucketed
@@ -3912,24 +3915,22 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         for (auto& tl : tier_list) tl.reserve(n_open);
         DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
         hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(n_open), 1024)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
-                           n_open, lv.open_idx, lv.m, Cuts{{kPartSeg, 512, 0x7FFFFFFF, 0, 0, 0}, 3}, tier_lists(),
+                           n_open, lv.open_idx, lv.m, Cuts{{16, 32, 512, 0x7FFFFFFF, 0, 0}, 4}, tier_lists(),
                            tier_cnt.p);
         unsigned int pc[kMaxTiers];
         DTE_HIP(hipMemcpyAsync(pc, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
         DTE_HIP(hipStreamSynchronize(s));
         Timer tm("partition", 0.0, s);
-        constexpr int kSegPerBlock = kWaves * (kWave / kPartSeg);
-        if (pc[0])
-          partition_seg((int)std::min<int64_t>((pc[0] + kSegPerBlock - 1) / kSegPerBlock, wave_grid_cap), s, d, lv,
-                        pa, tier_list[0].p, pc[0]);
-        if (pc[1])
-          partition_wave((int)std::min<int64_t>((pc[1] + kWaves - 1) / kWaves, wave_grid_cap), s, d, lv, pa,
-                         tier_list[1].p, pc[1]);
-        if (pc[2]) {
+        if (pc[0]) partition_seg16(wave_grid_cap, s, d, lv, pa, tier_list[0].p, pc[0]);
+        if (pc[1]) partition_seg32(wave_grid_cap, s, d, lv, pa, tier_list[1].p, pc[1]);
+        if (pc[2])
+          partition_wave((int)std::min<int64_t>((pc[2] + kWaves - 1) / kWaves, wave_grid_cap), s, d, lv, pa,
+                         tier_list[2].p, pc[2]);
+        if (pc[3]) {
           // workgroup partition over the remaining open positions
-          pa.list = tier_list[2].p;
-          pa.n_open = pc[2];
-          partition((int)std::min<int64_t>(pc[2], search_grid_cap), s, d, lv, pa);
+          pa.list = tier_list[3].p;
+          pa.n_open = pc[3];
+          partition((int)std::min<int64_t>(pc[3], search_grid_cap), s, d, lv, pa);
         }
       }
     }
