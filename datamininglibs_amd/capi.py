@@ -28,6 +28,7 @@ ERR_NO_DEVICE = 4
 ERR_ADMISSION = 5
 ERR_INTERNAL = 6
 ERR_NOT_FOUND = 7
+ERR_IO = 8
 
 
 class DteError(RuntimeError):
@@ -215,6 +216,9 @@ class Forest:
         _check(lib().dte_forest_canonical(self._h, *[_ptr(out[k]) if out[k].size else None for k in order]))
         return out
 
+    def save(self, path):
+        _check(lib().dte_forest_save(self._h, os.fsencode(path)))
+
     def export(self):
         """Reference model layout: (nodes structured array, probability array)."""
         nodes = np.zeros(self.n_nodes, NODE_DTYPE)
@@ -274,6 +278,12 @@ def fit_many(X_dev_ptrs, n_rows, n_features, ys, params, max_concurrent=0):
     out = (P * n)()
     _check(lib().dte_fit_many(n, xs, ts, rows, feats, ps, int(max_concurrent), out))
     return [Forest(P(out[i])) for i in range(n)]
+
+
+def load_forest(path, device_id=0):
+    h = P()
+    _check(lib().dte_forest_load(os.fsencode(path), device_id, ctypes.byref(h)))
+    return Forest(h)
 
 
 def import_forest(nodes, probs, n_trees, n_classes, n_features, task, device_id=0):

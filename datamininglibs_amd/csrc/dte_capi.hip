@@ -6,6 +6,7 @@
 #include <chrono>
 #include <thread>
 #include <atomic>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <functional>
@@ -674,6 +675,233 @@ int dte_forest_import(const dte_node* nodes, int64_t n_nodes, const float* probs
     DTE_HIP(hipMemcpyAsync(f.threshold.p, thr.data(), 4 * n, hipMemcpyHostToDevice, s));
     DTE_HIP(hipMemcpyAsync(f.value.p, value.data(), 4 * n, hipMemcpyHostToDevice, s));
     if (task == 0) DTE_HIP(hipMemcpyAsync(f.class_counts.p, cls.data(), 4 * cls.size(), hipMemcpyHostToDevice, s));
+    build_predict_nodes(f, s);
+    DTE_HIP(hipStreamSynchronize(s));
+    *out = guard.release();
+  });
+}
+
+// ------------------------------------------------------- forest files --
+extern "C++" {
+namespace {
+constexpr char kForestMagic[8] = {'D', 'T', 'E', 'F', 'R', 'S', 'T', '1'};
+constexpr uint32_t kForestVersion = 1;
+
+uint64_t fnv1a64(const uint8_t* p, size_t n, uint64_t h = 0xcbf29ce484222325ull) {
+  for (size_t i = 0; i < n; ++i) {
+    h ^= p[i];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}
+
+struct ForestHeader {
+  int32_t n_trees, n_classes, n_features, task, algorithm, fixed_shift, has_stats;
+  int64_t n_nodes;
+};
+
+// Bytes of the arrays that follow the header.
+size_t forest_payload(const ForestHeader& h) {
+  const size_t n = (size_t)h.n_nodes;
+  size_t b = 8 * ((size_t)h.n_trees + 1) + n * (4 + 4 + 4 + 4 + 4);
+  if (h.task == 0) b += 4 * n * (size_t)h.n_classes;
+  if (h.has_stats) b += n * (4 + 4 + 4 + 8 + 8);
+  return b;
+}
+
+struct Writer {
+  std::vector<uint8_t> buf;
+  void put(const void* p, size_t n) {
+    const uint8_t* c = (const uint8_t*)p;
+    buf.insert(buf.end(), c, c + n);
+  }
+  template <typename T>
+  void put(const T& v) { put(&v, sizeof(T)); }
+};
+
+struct Reader {
+  const uint8_t* p;
+  size_t n, off = 0;
+  void get(void* dst, size_t k) {
+    if (off + k > n) fail(DTE_ERR_INVALID_ARGUMENT, "forest file is truncated");
+    std::memcpy(dst, p + off, k);
+    off += k;
+  }
+  template <typename T>
+  T get() {
+    T v;
+    get(&v, sizeof(T));
+    return v;
+  }
+};
+}  // namespace
+}  // extern "C++"
+
+int dte_forest_save(const dte_forest* forest, const char* path) {
+  return guarded([&] {
+    if (!forest || !path) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    const Forest& f = forest->f;
+    DTE_HIP(hipSetDevice(f.device));
+    hipStream_t s = library_stream(f.device);
+    const int64_t n = f.n_nodes;
+    ForestHeader h{f.n_trees, f.n_classes, f.n_features, f.task, f.algorithm, f.fixed_shift, f.has_stats ? 1 : 0, n};
+    std::vector<uint8_t> arrays(forest_payload(h));
+    size_t off = 0;
+    auto cp = [&](const void* src, size_t bytes) {
+      DTE_HIP(hipMemcpyAsync(arrays.data() + off, src, bytes, hipMemcpyDeviceToHost, s));
+      off += bytes;
+    };
+    cp(f.tree_start.p, 8 * ((size_t)f.n_trees + 1));
+    cp(f.feature.p, 4 * n);
+    cp(f.threshold.p, 4 * n);
+    cp(f.left.p, 4 * n);
+    cp(f.depth.p, 4 * n);
+    cp(f.value.p, 4 * n);
+    if (f.task == 0) cp(f.class_counts.p, 4 * (size_t)n * f.n_classes);
+    if (f.has_stats) {
+      cp(f.count.p, 4 * n);
+      cp(f.n_entries.p, 4 * n);
+      cp(f.score.p, 4 * n);
+      cp(f.rowhash.p, 8 * n);
+      cp(f.sum_fixed.p, 8 * n);
+    }
+    DTE_HIP(hipStreamSynchronize(s));
+    Writer w;
+    w.put(kForestMagic, 8);
+    w.put(kForestVersion);
+    w.put(h.n_trees);
+    w.put(h.n_classes);
+    w.put(h.n_features);
+    w.put(h.task);
+    w.put(h.algorithm);
+    w.put(h.fixed_shift);
+    w.put(h.has_stats);
+    w.put(h.n_nodes);
+    w.put(arrays.data(), arrays.size());
+    w.put(fnv1a64(w.buf.data(), w.buf.size()));
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) fail(DTE_ERR_IO, std::string("cannot open for writing: ") + path);
+    const size_t wrote = std::fwrite(w.buf.data(), 1, w.buf.size(), fp);
+    const int closed = std::fclose(fp);
+    if (wrote != w.buf.size() || closed != 0) fail(DTE_ERR_IO, std::string("write failed: ") + path);
+  });
+}
+
+int dte_forest_load(const char* path, int32_t device_id, dte_forest** out) {
+  return guarded([&] {
+    if (!path || !out) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    *out = nullptr;
+    std::vector<uint8_t> file;
+    {
+      FILE* fp = std::fopen(path, "rb");
+      if (!fp) fail(DTE_ERR_IO, std::string("cannot open: ") + path);
+      uint8_t chunk[1 << 16];
+      size_t k;
+      while ((k = std::fread(chunk, 1, sizeof(chunk), fp)) > 0) file.insert(file.end(), chunk, chunk + k);
+      const bool err = std::ferror(fp) != 0;
+      std::fclose(fp);
+      if (err) fail(DTE_ERR_IO, std::string("read failed: ") + path);
+    }
+    if (file.size() < 8 + 4 + 7 * 4 + 8 + 8) fail(DTE_ERR_INVALID_ARGUMENT, "not a forest file (too short)");
+    if (std::memcmp(file.data(), kForestMagic, 8) != 0) fail(DTE_ERR_INVALID_ARGUMENT, "not a forest file (magic)");
+    uint64_t stored;
+    std::memcpy(&stored, file.data() + file.size() - 8, 8);
+    if (fnv1a64(file.data(), file.size() - 8) != stored)
+      fail(DTE_ERR_INVALID_ARGUMENT, "forest file checksum mismatch");
+    Reader r{file.data(), file.size() - 8, 8};
+    if (r.get<uint32_t>() != kForestVersion) fail(DTE_ERR_INVALID_ARGUMENT, "unsupported forest file version");
+    ForestHeader h;
+    h.n_trees = r.get<int32_t>();
+    h.n_classes = r.get<int32_t>();
+    h.n_features = r.get<int32_t>();
+    h.task = r.get<int32_t>();
+    h.algorithm = r.get<int32_t>();
+    h.fixed_shift = r.get<int32_t>();
+    h.has_stats = r.get<int32_t>();
+    h.n_nodes = r.get<int64_t>();
+    if (h.n_trees <= 0 || h.n_nodes < h.n_trees || h.n_features <= 0 || (h.task != 0 && h.task != 1) ||
+        (h.algorithm != DTE_RANDOM_FOREST && h.algorithm != DTE_EXTRA_TREES) || (h.has_stats != 0 && h.has_stats != 1) ||
+        (h.task == 0 && (h.n_classes <= 0 || h.n_classes > 32)) || (h.task == 1 && h.n_classes != 1) ||
+        h.n_nodes > (int64_t)file.size())
+      fail(DTE_ERR_INVALID_ARGUMENT, "forest file header out of range");
+    if (r.off + forest_payload(h) != r.n) fail(DTE_ERR_INVALID_ARGUMENT, "forest file size does not match its header");
+    const int64_t n = h.n_nodes;
+    const int T = h.n_trees;
+    std::vector<int64_t> ts(T + 1);
+    std::vector<int32_t> feature(n), left(n), depth(n);
+    r.get(ts.data(), 8 * ts.size());
+    r.get(feature.data(), 4 * n);
+    const size_t thr_off = r.off;
+    r.off += 4 * n;
+    r.get(left.data(), 4 * n);
+    r.get(depth.data(), 4 * n);
+    // structure: trees tile [0, n); children adjacent inside their tree
+    if (ts[0] != 0 || ts[T] != n) fail(DTE_ERR_INVALID_ARGUMENT, "forest file tree table out of range");
+    for (int t = 0; t < T; ++t) {
+      if (ts[t + 1] <= ts[t]) fail(DTE_ERR_INVALID_ARGUMENT, "forest file tree table out of range");
+      if (depth[ts[t]] != 0) fail(DTE_ERR_INVALID_ARGUMENT, "forest file root depth");
+      for (int64_t g = ts[t]; g < ts[t + 1]; ++g) {
+        if (feature[g] < 0) {
+          if (feature[g] != -1) fail(DTE_ERR_INVALID_ARGUMENT, "forest file feature out of range");
+          continue;
+        }
+        if (feature[g] >= h.n_features) fail(DTE_ERR_INVALID_ARGUMENT, "forest file feature out of range");
+        if (left[g] <= g || (int64_t)left[g] + 1 >= ts[t + 1])
+          fail(DTE_ERR_INVALID_ARGUMENT, "forest file child link out of range");
+        if (depth[left[g]] != depth[g] + 1 || depth[left[g] + 1] != depth[g] + 1)
+          fail(DTE_ERR_INVALID_ARGUMENT, "forest file depths inconsistent");
+      }
+    }
+    set_device(device_id);
+    hipStream_t s = library_stream(device_id);
+    auto* hd = new dte_forest();
+    std::unique_ptr<dte_forest> guard(hd);
+    Forest& f = hd->f;
+    f.device = device_id;
+    f.task = h.task;
+    f.n_classes = h.n_classes;
+    f.n_features = h.n_features;
+    f.n_trees = T;
+    f.fixed_shift = h.fixed_shift;
+    f.algorithm = h.algorithm;
+    f.n_nodes = n;
+    f.has_stats = h.has_stats != 0;
+    const uint8_t* base = file.data();
+    auto up = [&](auto& buf, size_t count, size_t src_off) {
+      buf.alloc(count);
+      DTE_HIP(hipMemcpyAsync(buf.p, base + src_off, count * sizeof(*buf.p), hipMemcpyHostToDevice, s));
+    };
+    size_t o = 8 + 4 + 7 * 4 + 8;
+    up(f.tree_start, (size_t)T + 1, o);
+    o += 8 * ((size_t)T + 1);
+    up(f.feature, n, o);
+    o += 4 * n;
+    up(f.threshold, n, thr_off);
+    o += 4 * n;
+    up(f.left, n, o);
+    o += 4 * n;
+    up(f.depth, n, o);
+    o += 4 * n;
+    up(f.value, n, o);
+    o += 4 * n;
+    if (h.task == 0) {
+      up(f.class_counts, (size_t)n * h.n_classes, o);
+      o += 4 * (size_t)n * h.n_classes;
+    } else {
+      f.class_counts.alloc(1);
+    }
+    if (f.has_stats) {
+      up(f.count, n, o);
+      o += 4 * n;
+      up(f.n_entries, n, o);
+      o += 4 * n;
+      up(f.score, n, o);
+      o += 4 * n;
+      up(f.rowhash, n, o);
+      o += 8 * n;
+      up(f.sum_fixed, n, o);
+      o += 8 * n;
+    }
     build_predict_nodes(f, s);
     DTE_HIP(hipStreamSynchronize(s));
     *out = guard.release();

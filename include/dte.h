@@ -40,6 +40,7 @@ extern "C" {
 #define DTE_ERR_ADMISSION 5
 #define DTE_ERR_INTERNAL 6
 #define DTE_ERR_NOT_FOUND 7
+#define DTE_ERR_IO 8
 
 /* Algorithms: GpuRf / GpuErt (lib_ensembles/source/gpurf.h, gpuert.h). */
 #define DTE_RANDOM_FOREST 0
@@ -168,6 +169,17 @@ int dte_forest_export(const dte_forest* forest, dte_node* nodes, int64_t nodes_c
 int dte_forest_import(const dte_node* nodes, int64_t n_nodes, const float* probs, int64_t n_probs,
                       int32_t n_trees, int32_t n_classes, int32_t n_features, int32_t task,
                       int32_t device_id, dte_forest** out);
+
+/* Forest file: the canonical arrays below with a header and an FNV-1a-64
+ * checksum, little-endian (format "DTEFRST1", version 1).  The reference
+ * keeps models only in memory (MlModel kNodeArray / kProbArray,
+ * gpudte_algorithm.cc:58-88) and rebuilds them per process; a saved forest
+ * reloads into HBM bit for bit (predictions and export identical).
+ * dte_forest_load checks the header, sizes, checksum and tree links before it
+ * touches the device: DTE_ERR_IO for a file that cannot be read or written,
+ * DTE_ERR_INVALID_ARGUMENT for one that is not a valid forest file. */
+int dte_forest_save(const dte_forest* forest, const char* path);
+int dte_forest_load(const char* path, int32_t device_id, dte_forest** out);
 
 /* Canonical build arrays (tree-major, level order in each tree) for
  * parity checks.  Any output pointer may be NULL.  class_counts has

@@ -122,3 +122,59 @@ def test_autotune_validates_before_device():
     assert L.dte_autotune(X.ctypes.data_as(ctypes.c_void_p), y.ctypes.data_as(ctypes.c_void_p), 8, 2,
                           ctypes.byref(p), -1, ctypes.byref(out), None) == capi.ERR_INVALID_ARGUMENT
     assert L.dte_autotune(None, None, 8, 2, ctypes.byref(p), 0, ctypes.byref(out), None) == capi.ERR_INVALID_ARGUMENT
+
+
+def test_forest_load_rejects_bad_files_before_device(tmp_path):
+    L = capi.lib()
+    h = ctypes.c_void_p()
+    missing = tmp_path / "missing.dtef"
+    assert L.dte_forest_load(str(missing).encode(), 0, ctypes.byref(h)) == capi.ERR_IO
+    junk = tmp_path / "junk.dtef"
+    junk.write_bytes(b"not a forest file at all, just some bytes" * 4)
+    assert L.dte_forest_load(str(junk).encode(), 0, ctypes.byref(h)) == capi.ERR_INVALID_ARGUMENT
+    assert "magic" in _last_error()
+    bad = tmp_path / "bad.dtef"
+    bad.write_bytes(b"DTEFRST1" + bytes(64))
+    assert L.dte_forest_load(str(bad).encode(), 0, ctypes.byref(h)) == capi.ERR_INVALID_ARGUMENT
+    assert "checksum" in _last_error()
+    assert L.dte_forest_load(None, 0, ctypes.byref(h)) == capi.ERR_INVALID_ARGUMENT
+
+
+def _fnv1a64(b):
+    h = 0xcbf29ce484222325
+    for x in b:
+        h = ((h ^ x) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _forest_file(path, *, left=(1, -1, -1), feature=(0, -1, -1), depth=(0, 1, 1), n_features=2, classes=2):
+    """One classification tree: a root split into two leaves (format DTEFRST1, version 1)."""
+    n = 3
+    hdr = b"DTEFRST1" + np.array([1], np.uint32).tobytes() + \
+        np.array([1, classes, n_features, 0, 0, 0, 0], np.int32).tobytes() + np.array([n], np.int64).tobytes()
+    body = (np.array([0, n], np.int64).tobytes() + np.array(feature, np.int32).tobytes() +
+            np.array([0.5, -1, -1], np.float32).tobytes() + np.array(left, np.int32).tobytes() +
+            np.array(depth, np.int32).tobytes() + np.zeros(n, np.float32).tobytes() +
+            np.array([3, 2, 3, 0, 0, 2], np.uint32).tobytes())
+    data = hdr + body
+    path.write_bytes(data + np.array([_fnv1a64(data)], np.uint64).tobytes())
+
+
+def test_forest_file_structure_checked_before_device(tmp_path):
+    L = capi.lib()
+    h = ctypes.c_void_p()
+    good = tmp_path / "good.dtef"
+    _forest_file(good)
+    # a well-formed file passes every check and stops at the missing device
+    assert L.dte_forest_load(str(good).encode(), 0, ctypes.byref(h)) == capi.ERR_NO_DEVICE
+    for name, kw, msg in [("link", dict(left=(2, -1, -1)), "child link"),
+                          ("feat", dict(feature=(5, -1, -1)), "feature"),
+                          ("depth", dict(depth=(0, 2, 1)), "depth")]:
+        p = tmp_path / f"{name}.dtef"
+        _forest_file(p, **kw)
+        assert L.dte_forest_load(str(p).encode(), 0, ctypes.byref(h)) == capi.ERR_INVALID_ARGUMENT, name
+        assert msg in _last_error(), (name, _last_error())
+    trunc = tmp_path / "trunc.dtef"
+    data = good.read_bytes()[:-20]
+    trunc.write_bytes(data + np.array([_fnv1a64(data)], np.uint64).tobytes())
+    assert L.dte_forest_load(str(trunc).encode(), 0, ctypes.byref(h)) == capi.ERR_INVALID_ARGUMENT

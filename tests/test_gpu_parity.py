@@ -157,3 +157,36 @@ def test_autotune_picks_a_valid_setting(algorithm):
     for k in a:
         assert np.array_equal(a[k], b[k]), k
     Xd.free()
+
+
+@pytest.mark.parametrize("algorithm,task", [(capi.RANDOM_FOREST, capi.CLASSIFICATION),
+                                            (capi.EXTRA_TREES, capi.CLASSIFICATION),
+                                            (capi.RANDOM_FOREST, capi.REGRESSION)])
+def test_forest_file_round_trip(tmp_path, algorithm, task):
+    """Save, reload: the same canonical arrays, export and predictions; a
+    flipped byte is rejected by the checksum."""
+    if task == capi.REGRESSION:
+        X, y = synth.sine_regression(seed=4, n=4000, f=6)
+        kw = dict(task=task, n_classes=0)
+    else:
+        X, y = small_classes(9, 4000, 10, 4)
+        kw = dict(n_classes=4)
+    f = capi.fit(X, y, capi.default_params(algorithm, n_trees=7, max_depth=12, seed=11, **kw))
+    path = tmp_path / "forest.dtef"
+    f.save(path)
+    g = capi.load_forest(path)
+    a, b = f.canonical(), g.canonical()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    (na, pa), (nb, pb) = f.export(), g.export()
+    assert np.array_equal(na, nb) and np.array_equal(pa, pb)
+    Xp = X[:, :500]
+    assert np.array_equal(f.predict(Xp), g.predict(Xp))
+    raw = bytearray(path.read_bytes())
+    raw[len(raw) // 2] ^= 0x40
+    bad = tmp_path / "bad.dtef"
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(capi.DteError):
+        capi.load_forest(bad)
+    f.close()
+    g.close()
