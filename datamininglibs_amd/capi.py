@@ -280,6 +280,27 @@ def fit_many(X_dev_ptrs, n_rows, n_features, ys, params, max_concurrent=0):
     return [Forest(P(out[i])) for i in range(n)]
 
 
+def oob_evaluate(forest, X_dev_ptr, y, n_rows, permuted_feature=-1):
+    """Out-of-bag pass (dte_oob_evaluate): (pairs, correct pairs) or (pairs, squared error sum)."""
+    y = np.ascontiguousarray(y, np.float32)
+    out = (ctypes.c_double * 2)()
+    _check(lib().dte_oob_evaluate(forest.handle, P(X_dev_ptr), _ptr(y), int(n_rows), int(permuted_feature), out))
+    return out[0], out[1]
+
+
+def permutation_importance(forest, X_dev_ptr, y, n_rows):
+    """Out-of-bag score and, per feature, its drop when the feature is permuted
+    (accuracy drop for classification, MSE increase for regression)."""
+    pairs, v = oob_evaluate(forest, X_dev_ptr, y, n_rows)
+    base = v / pairs if pairs else 0.0
+    imp = np.zeros(forest.n_features)
+    for f in range(forest.n_features):
+        pf, vf = oob_evaluate(forest, X_dev_ptr, y, n_rows, f)
+        s = vf / pf if pf else 0.0
+        imp[f] = base - s if forest.task == 0 else s - base
+    return base, imp
+
+
 def load_forest(path, device_id=0):
     h = P()
     _check(lib().dte_forest_load(os.fsencode(path), device_id, ctypes.byref(h)))

@@ -4044,6 +4044,9 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   fo->n_features = F;
   fo->n_trees = T;
   fo->fixed_shift = shift;
+  fo->seed = p.seed;
+  fo->tree_offset = t0;
+  fo->fit_rows = algorithm == DTE_RANDOM_FOREST ? N : 0;
   fo->n_nodes = n_nodes;
   fo->has_stats = true;
   PoolSuspend own;  // the forest outlives the fit
@@ -4106,6 +4109,9 @@ Forest* merge_forests(std::vector<std::unique_ptr<Forest>>& parts, hipStream_t s
   fo->n_classes = p0.n_classes;
   fo->n_features = p0.n_features;
   fo->fixed_shift = p0.fixed_shift;
+  fo->seed = p0.seed;
+  fo->tree_offset = p0.tree_offset;
+  fo->fit_rows = p0.fit_rows;
   fo->has_stats = true;
   int64_t n = 0;
   int T = 0;

@@ -696,9 +696,11 @@ uint64_t fnv1a64(const uint8_t* p, size_t n, uint64_t h = 0xcbf29ce484222325ull)
 }
 
 struct ForestHeader {
-  int32_t n_trees, n_classes, n_features, task, algorithm, fixed_shift, has_stats;
-  int64_t n_nodes;
+  int32_t n_trees, n_classes, n_features, task, algorithm, fixed_shift, has_stats, tree_offset;
+  int64_t n_nodes, fit_rows;
+  uint64_t seed;
 };
+constexpr size_t kForestHeaderBytes = 8 + 4 + 8 * 4 + 3 * 8;
 
 // Bytes of the arrays that follow the header.
 size_t forest_payload(const ForestHeader& h) {
@@ -744,7 +746,8 @@ int dte_forest_save(const dte_forest* forest, const char* path) {
     DTE_HIP(hipSetDevice(f.device));
     hipStream_t s = library_stream(f.device);
     const int64_t n = f.n_nodes;
-    ForestHeader h{f.n_trees, f.n_classes, f.n_features, f.task, f.algorithm, f.fixed_shift, f.has_stats ? 1 : 0, n};
+    ForestHeader h{f.n_trees, f.n_classes, f.n_features, f.task, f.algorithm, f.fixed_shift, f.has_stats ? 1 : 0,
+                   f.tree_offset, n, f.fit_rows, f.seed};
     std::vector<uint8_t> arrays(forest_payload(h));
     size_t off = 0;
     auto cp = [&](const void* src, size_t bytes) {
@@ -776,7 +779,10 @@ int dte_forest_save(const dte_forest* forest, const char* path) {
     w.put(h.algorithm);
     w.put(h.fixed_shift);
     w.put(h.has_stats);
+    w.put(h.tree_offset);
     w.put(h.n_nodes);
+    w.put(h.fit_rows);
+    w.put(h.seed);
     w.put(arrays.data(), arrays.size());
     w.put(fnv1a64(w.buf.data(), w.buf.size()));
     FILE* fp = std::fopen(path, "wb");
@@ -802,7 +808,7 @@ int dte_forest_load(const char* path, int32_t device_id, dte_forest** out) {
       std::fclose(fp);
       if (err) fail(DTE_ERR_IO, std::string("read failed: ") + path);
     }
-    if (file.size() < 8 + 4 + 7 * 4 + 8 + 8) fail(DTE_ERR_INVALID_ARGUMENT, "not a forest file (too short)");
+    if (file.size() < kForestHeaderBytes + 8) fail(DTE_ERR_INVALID_ARGUMENT, "not a forest file (too short)");
     if (std::memcmp(file.data(), kForestMagic, 8) != 0) fail(DTE_ERR_INVALID_ARGUMENT, "not a forest file (magic)");
     uint64_t stored;
     std::memcpy(&stored, file.data() + file.size() - 8, 8);
@@ -818,11 +824,14 @@ int dte_forest_load(const char* path, int32_t device_id, dte_forest** out) {
     h.algorithm = r.get<int32_t>();
     h.fixed_shift = r.get<int32_t>();
     h.has_stats = r.get<int32_t>();
+    h.tree_offset = r.get<int32_t>();
     h.n_nodes = r.get<int64_t>();
+    h.fit_rows = r.get<int64_t>();
+    h.seed = r.get<uint64_t>();
     if (h.n_trees <= 0 || h.n_nodes < h.n_trees || h.n_features <= 0 || (h.task != 0 && h.task != 1) ||
         (h.algorithm != DTE_RANDOM_FOREST && h.algorithm != DTE_EXTRA_TREES) || (h.has_stats != 0 && h.has_stats != 1) ||
         (h.task == 0 && (h.n_classes <= 0 || h.n_classes > 32)) || (h.task == 1 && h.n_classes != 1) ||
-        h.n_nodes > (int64_t)file.size())
+        h.n_nodes > (int64_t)file.size() || h.tree_offset < 0 || h.fit_rows < 0)
       fail(DTE_ERR_INVALID_ARGUMENT, "forest file header out of range");
     if (r.off + forest_payload(h) != r.n) fail(DTE_ERR_INVALID_ARGUMENT, "forest file size does not match its header");
     const int64_t n = h.n_nodes;
@@ -866,12 +875,15 @@ int dte_forest_load(const char* path, int32_t device_id, dte_forest** out) {
     f.algorithm = h.algorithm;
     f.n_nodes = n;
     f.has_stats = h.has_stats != 0;
+    f.seed = h.seed;
+    f.tree_offset = h.tree_offset;
+    f.fit_rows = h.fit_rows;
     const uint8_t* base = file.data();
     auto up = [&](auto& buf, size_t count, size_t src_off) {
       buf.alloc(count);
       DTE_HIP(hipMemcpyAsync(buf.p, base + src_off, count * sizeof(*buf.p), hipMemcpyHostToDevice, s));
     };
-    size_t o = 8 + 4 + 7 * 4 + 8;
+    size_t o = kForestHeaderBytes;
     up(f.tree_start, (size_t)T + 1, o);
     o += 8 * ((size_t)T + 1);
     up(f.feature, n, o);
@@ -935,6 +947,21 @@ int dte_forest_canonical(const dte_forest* forest, int64_t* tree_start, int32_t*
     }
     if (f.task == 0) cp(class_counts, f.class_counts.p, 4 * (size_t)n * f.n_classes);
     DTE_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int dte_oob_evaluate(const dte_forest* forest, const float* samples_dev, const float* targets, int64_t n_rows,
+                     int32_t permuted_feature, double* out) {
+  return guarded([&] {
+    if (!forest || !samples_dev || !targets || !out) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    const Forest& f = forest->f;
+    if (f.algorithm != DTE_RANDOM_FOREST || f.fit_rows <= 0)
+      fail(DTE_ERR_INVALID_ARGUMENT, "out-of-bag evaluation needs a random forest fitted by this library");
+    if (n_rows != f.fit_rows) fail(DTE_ERR_INVALID_ARGUMENT, "n_rows differs from the fit's rows");
+    if (permuted_feature < -1 || permuted_feature >= f.n_features)
+      fail(DTE_ERR_INVALID_ARGUMENT, "permuted_feature out of range");
+    set_device(f.device);
+    oob_evaluate(f, samples_dev, targets, n_rows, permuted_feature, out, library_stream(f.device));
   });
 }
 

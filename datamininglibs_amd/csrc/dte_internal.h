@@ -186,6 +186,12 @@ struct Forest {
   int fixed_shift = 0;
   int algorithm = 0;
   int64_t n_nodes = 0;
+  // Bootstrap record of a fit (random forest): tree t drew its rows with
+  // seed and global tree id tree_offset + t from fit_rows rows; fit_rows = 0
+  // when unknown (imported forests).
+  uint64_t seed = 0;
+  int32_t tree_offset = 0;
+  int64_t fit_rows = 0;
   DevBuf<int64_t> tree_start;  // n_trees + 1
   DevBuf<PNode> pnodes;        // n_nodes
   // Build diagnostics (absent for imported forests).
@@ -210,6 +216,10 @@ Forest* fit_forest(const float* X_dev, const float* y_host, int64_t n_rows, int3
 void build_predict_nodes(Forest& f, hipStream_t stream);
 void predict_forest(const Forest& f, const float* X_dev, int64_t n_rows, int64_t ld,
                     int row_major, int32_t* votes_dev, float* sums_dev, hipStream_t stream);
+// Out-of-bag evaluation (dte_oob_evaluate): pairs evaluated, correct pairs
+// (classification) or the sum of squared errors (regression).
+void oob_evaluate(const Forest& f, const float* X_dev, const float* targets, int64_t n_rows, int perm_feature,
+                  double out[2], hipStream_t stream);
 void device_init(int device);
 hipStream_t library_stream(int device);
 const LnTable& host_ln_table();

@@ -11,12 +11,16 @@
 //    order instead of float atomics, gpudte.cu:354-357).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "dte_internal.h"
 
 namespace dte {
 namespace {
 
 constexpr int kRowsPerBlock = 64;
+constexpr int kWave = 64;
 
 __device__ __forceinline__ int walk(const PNode* __restrict__ nodes, int64_t root, const float* __restrict__ X,
                                     int64_t row, int64_t ld, int row_major, float* leaf_value) {
@@ -76,7 +80,119 @@ __global__ void __launch_bounds__(kBlock) k_predict_reg(const PNode* __restrict_
   }
 }
 
+// ---------------------------------------------------- out-of-bag pass --
+// Stream of the per-visit row draws of the permutation pass.
+constexpr uint32_t kStreamImp = 0x00706d69u;
+
+// In-bag bits: bit r of tree t is set iff tree t's bootstrap drew row r
+// (the draws of the fit's bootstrap, k_boot_count).
+__global__ void k_inbag_bits(uint32_t* bits, int64_t N, int64_t words, int32_t tree0, uint64_t seed) {
+  const int t = blockIdx.y;
+  uint32_t* bt = bits + (int64_t)t * words;
+  const uint32_t tg = (uint32_t)(tree0 + t);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = rand_u32(seed, kStreamBoot, tg, 0, (uint64_t)i) % (uint32_t)N;
+    atomicOr(&bt[r >> 5], 1u << (r & 31u));
+  }
+}
+
+// Every (tree, row) pair whose row is out of the tree's bag is routed
+// through the tree (gpurf.cu:290-353); with perm >= 0, nodes that split on
+// feature perm read the value of row draw % N instead, one counter-based
+// draw per visit keyed by (tree, row, depth) (gpurf.cu:356-438).
+// Classification counts the pairs whose tree vote equals the label;
+// regression sums the squared errors of the leaf values.
+template <int TASK>
+__global__ void __launch_bounds__(kBlock) k_oob(const PNode* __restrict__ nodes, const int64_t* __restrict__ tree_start,
+                                                const float* __restrict__ X, int64_t N, const uint32_t* __restrict__ bits,
+                                                int64_t words, const uint32_t* __restrict__ label,
+                                                const float* __restrict__ y, int32_t tree0, uint64_t seed, int perm,
+                                                unsigned long long* counts, double* sse) {
+  const int t = blockIdx.y;
+  const uint32_t tg = (uint32_t)(tree0 + t);
+  const uint32_t* bt = bits + (int64_t)t * words;
+  const int64_t root = tree_start[t];
+  unsigned long long n_pairs = 0, n_ok = 0;
+  double e2 = 0.0;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
+    if ((bt[r >> 5] >> (r & 31)) & 1u) continue;  // in the bag
+    PNode nd = nodes[root];
+    uint32_t depth = 0;
+    while (nd.feature >= 0) {
+      int64_t src = r;
+      if (nd.feature == perm) src = rand_u32(seed, kStreamImp, tg, (uint32_t)r, depth) % (uint32_t)N;
+      nd = nodes[nd.left + side_of(X[(int64_t)nd.feature * N + src], nd.thr)];
+      ++depth;
+    }
+    ++n_pairs;
+    if (TASK == 0) {
+      n_ok += (uint32_t)nd.left == label[r] ? 1ull : 0ull;
+    } else {
+      const double e = (double)nd.thr - (double)y[r];
+      e2 += e * e;
+    }
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    n_pairs += __shfl_xor(n_pairs, off, kWave);
+    n_ok += __shfl_xor(n_ok, off, kWave);
+    if (TASK == 1) e2 += __shfl_xor(e2, off, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0 && n_pairs) {
+    atomicAdd(&counts[0], n_pairs);
+    if (TASK == 0) atomicAdd(&counts[1], n_ok);
+    else atomicAdd(sse, e2);
+  }
+}
+
 }  // namespace
+
+void oob_evaluate(const Forest& f, const float* X, const float* targets, int64_t N, int perm, double out[2],
+                  hipStream_t s) {
+  const int T = f.n_trees;
+  if (T > 65535) fail(DTE_ERR_INVALID_ARGUMENT, "the out-of-bag pass handles at most 65535 trees");
+  const int64_t words = (N + 31) / 32;
+  DevBuf<uint32_t> bits((size_t)words * T);
+  DTE_HIP(hipMemsetAsync(bits.p, 0, bits.bytes(), s));
+  const int gx = (int)std::min<int64_t>((N + kBlock - 1) / kBlock, 1024);
+  hipLaunchKernelGGL(k_inbag_bits, dim3((unsigned)gx, (unsigned)T), dim3(kBlock), 0, s, bits.p, N, words,
+                     f.tree_offset, f.seed);
+  DevBuf<uint32_t> label;
+  DevBuf<float> yv;
+  if (f.task == 0) {
+    std::vector<uint32_t> lab(N);
+    for (int64_t i = 0; i < N; ++i) {
+      const int c = (int)targets[i];
+      if (c < 0 || c >= f.n_classes || (float)c != targets[i]) fail(DTE_ERR_INVALID_ARGUMENT, "class label out of range");
+      lab[i] = (uint32_t)c;
+    }
+    label.alloc(N);
+    DTE_HIP(hipMemcpyAsync(label.p, lab.data(), 4 * N, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipStreamSynchronize(s));
+  } else {
+    yv.alloc(N);
+    DTE_HIP(hipMemcpyAsync(yv.p, targets, 4 * N, hipMemcpyHostToDevice, s));
+    DTE_HIP(hipStreamSynchronize(s));
+  }
+  DevBuf<unsigned long long> counts(2);
+  DevBuf<double> sse(1);
+  DTE_HIP(hipMemsetAsync(counts.p, 0, counts.bytes(), s));
+  DTE_HIP(hipMemsetAsync(sse.p, 0, sse.bytes(), s));
+  if (f.task == 0)
+    hipLaunchKernelGGL(k_oob<0>, dim3((unsigned)gx, (unsigned)T), dim3(kBlock), 0, s, f.pnodes.p, f.tree_start.p, X,
+                       N, bits.p, words, label.p, (const float*)nullptr, f.tree_offset, f.seed, perm, counts.p, sse.p);
+  else
+    hipLaunchKernelGGL(k_oob<1>, dim3((unsigned)gx, (unsigned)T), dim3(kBlock), 0, s, f.pnodes.p, f.tree_start.p, X,
+                       N, bits.p, words, (const uint32_t*)nullptr, yv.p, f.tree_offset, f.seed, perm, counts.p, sse.p);
+  DTE_HIP(hipGetLastError());
+  unsigned long long c[2];
+  double e = 0.0;
+  DTE_HIP(hipMemcpyAsync(c, counts.p, sizeof(c), hipMemcpyDeviceToHost, s));
+  DTE_HIP(hipMemcpyAsync(&e, sse.p, sizeof(e), hipMemcpyDeviceToHost, s));
+  DTE_HIP(hipStreamSynchronize(s));
+  out[0] = (double)c[0];
+  out[1] = f.task == 0 ? (double)c[1] : e;
+}
 
 void predict_forest(const Forest& f, const float* X, int64_t n_rows, int64_t ld, int row_major,
                     int32_t* votes, float* sums, hipStream_t s) {

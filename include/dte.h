@@ -170,6 +170,22 @@ int dte_forest_import(const dte_node* nodes, int64_t n_nodes, const float* probs
                       int32_t n_trees, int32_t n_classes, int32_t n_features, int32_t task,
                       int32_t device_id, dte_forest** out);
 
+/* Out-of-bag evaluation of a random forest on its training samples (the
+ * reference's kOobEstimate / kFeatureImp kernels, lib_ensembles/source/
+ * gpurf.cu:290-438, which its driver never launches).  samples_dev: the
+ * fit's samples in HBM (feature-major, n_rows = the fit's rows); targets:
+ * host, as for dte_fit.  Every (tree, row) pair whose bootstrap left the row
+ * out is routed through the tree.  permuted_feature = -1: plain routing;
+ * f >= 0: every node that splits on f reads the value of a drawn row instead
+ * (one counter-based draw per visit, keyed by tree, row and depth; the
+ * reference draws curand % N per visit).  out[0] = pairs evaluated, out[1] =
+ * pairs whose tree vote equals the label (classification; the reference's
+ * oobCounts) or the sum of squared errors of the leaf values (regression;
+ * the reference sums the leaf values, FIX).  Permutation importance of f =
+ * accuracy(-1) - accuracy(f), or mse(f) - mse(-1). */
+int dte_oob_evaluate(const dte_forest* forest, const float* samples_dev, const float* targets, int64_t n_rows,
+                     int32_t permuted_feature, double* out);
+
 /* Forest file: the canonical arrays below with a header and an FNV-1a-64
  * checksum, little-endian (format "DTEFRST1", version 1).  The reference
  * keeps models only in memory (MlModel kNodeArray / kProbArray,

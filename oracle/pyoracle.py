@@ -177,3 +177,55 @@ def rf_best_split(values, labels, weights, n_classes):
     thr = np.zeros(1, np.float32)
     s = lib().oracle_rf_best_split(_ptr(v), _ptr(lab), _ptr(w), v.size, n_classes, _ptr(thr))
     return np.float32(s), thr[0]
+
+
+STREAM_BOOT = 0x746F6F62  # bootstrap draws (same counter layout as the build)
+STREAM_IMP = 0x00706D69   # per-visit row draws of the permutation pass
+_FLT_MAX = np.float32(3.4028234663852886e38)
+
+
+def rand_u32(seed, stream, tree, node, idx):
+    """Word idx % 4 of Philox4x32-10 at counter (idx / 4, node, tree, stream), key = seed."""
+    o = philox((idx >> 2, node, tree, stream), (seed & 0xFFFFFFFF, seed >> 32))
+    return int(o[idx & 3])
+
+
+def oob_evaluate(forest, X, y, *, seed, tree_offset=0, permuted_feature=-1):
+    """Out-of-bag pairs of a random forest, restating lib_ensembles/source/
+    gpurf.cu:290-438 (gpurf_oob_estimate / gpurf_feature_importance) with the
+    build's counter-based draws: every (tree, row) pair whose bootstrap left
+    the row out is routed through the tree; nodes splitting on
+    permuted_feature read row rand_u32(seed, STREAM_IMP, tree, row, depth) % N
+    (the reference draws curand % N per visit).  Returns (pairs, correct
+    pairs) for classification, (pairs, sum of squared errors) for
+    regression (the reference sums leaf values there).  Pure Python: for
+    test-sized inputs."""
+    X = np.asarray(X, np.float32)
+    F, N = X.shape
+    pairs, ok, sse = 0, 0, 0.0
+    for t in range(forest.n_trees):
+        tg = tree_offset + t
+        inbag = np.zeros(N, bool)
+        for i in range(N):
+            inbag[rand_u32(seed, STREAM_BOOT, tg, 0, i) % N] = True
+        for r in range(N):
+            if inbag[r]:
+                continue
+            g = int(forest.tree_start[t])
+            depth = 0
+            while forest.feature[g] >= 0:
+                f = int(forest.feature[g])
+                src = r
+                if f == permuted_feature:
+                    src = rand_u32(seed, STREAM_IMP, tg, r, depth) % N
+                x = X[f, src]
+                right = 0 if (x == -_FLT_MAX or x < forest.threshold[g]) else 1
+                g = int(forest.left[g]) + right
+                depth += 1
+            pairs += 1
+            if forest.task == 0:
+                ok += int(np.argmax(forest.class_counts[g]) == int(y[r]))
+            else:
+                e = float(forest.value[g]) - float(y[r])
+                sse += e * e
+    return pairs, (ok if forest.task == 0 else sse)

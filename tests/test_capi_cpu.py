@@ -134,7 +134,7 @@ def test_forest_load_rejects_bad_files_before_device(tmp_path):
     assert L.dte_forest_load(str(junk).encode(), 0, ctypes.byref(h)) == capi.ERR_INVALID_ARGUMENT
     assert "magic" in _last_error()
     bad = tmp_path / "bad.dtef"
-    bad.write_bytes(b"DTEFRST1" + bytes(64))
+    bad.write_bytes(b"DTEFRST1" + bytes(96))
     assert L.dte_forest_load(str(bad).encode(), 0, ctypes.byref(h)) == capi.ERR_INVALID_ARGUMENT
     assert "checksum" in _last_error()
     assert L.dte_forest_load(None, 0, ctypes.byref(h)) == capi.ERR_INVALID_ARGUMENT
@@ -150,8 +150,11 @@ def _fnv1a64(b):
 def _forest_file(path, *, left=(1, -1, -1), feature=(0, -1, -1), depth=(0, 1, 1), n_features=2, classes=2):
     """One classification tree: a root split into two leaves (format DTEFRST1, version 1)."""
     n = 3
+    # magic, version, n_trees, n_classes, n_features, task, algorithm, fixed_shift, has_stats,
+    # tree_offset, n_nodes, fit_rows, seed
     hdr = b"DTEFRST1" + np.array([1], np.uint32).tobytes() + \
-        np.array([1, classes, n_features, 0, 0, 0, 0], np.int32).tobytes() + np.array([n], np.int64).tobytes()
+        np.array([1, classes, n_features, 0, 0, 0, 0, 0], np.int32).tobytes() + \
+        np.array([n, 8], np.int64).tobytes() + np.array([7], np.uint64).tobytes()
     body = (np.array([0, n], np.int64).tobytes() + np.array(feature, np.int32).tobytes() +
             np.array([0.5, -1, -1], np.float32).tobytes() + np.array(left, np.int32).tobytes() +
             np.array(depth, np.int32).tobytes() + np.zeros(n, np.float32).tobytes() +

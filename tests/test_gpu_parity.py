@@ -190,3 +190,33 @@ def test_forest_file_round_trip(tmp_path, algorithm, task):
         capi.load_forest(bad)
     f.close()
     g.close()
+
+
+@pytest.mark.parametrize("task", [capi.CLASSIFICATION, capi.REGRESSION])
+def test_out_of_bag_pass_matches_oracle(task):
+    """Out-of-bag pairs, plain and with a feature permuted per visit, equal the
+    oracle's (counts exactly, squared errors to 1e-9); importance is the drop."""
+    if task == capi.REGRESSION:
+        X, y = synth.sine_regression(seed=6, n=1500, f=5)
+        kw = dict(task=task, n_classes=0)
+    else:
+        X, y = small_classes(10, 1500, 6, 3)
+        kw = dict(n_classes=3)
+    p = capi.default_params(capi.RANDOM_FOREST, n_trees=5, n_features_per_node=2, max_depth=10, seed=21, **kw)
+    Xd = capi.DeviceArray.from_numpy(X)
+    g = capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p)
+    o = pyoracle.fit(X, y, algorithm=capi.RANDOM_FOREST, task=task, n_trees=5, k=2, max_depth=10,
+                     n_classes=3 if task == capi.CLASSIFICATION else 0, seed=21)
+    assert_same_forest(g, o)
+    for perm in (-1, 0, 2):
+        gp, gv = capi.oob_evaluate(g, Xd.ptr, y, X.shape[1], perm)
+        op, ov = pyoracle.oob_evaluate(o, X, y, seed=21, permuted_feature=perm)
+        assert gp == op, perm
+        if task == capi.CLASSIFICATION:
+            assert gv == ov, perm
+        else:
+            assert abs(gv - ov) <= 1e-9 * max(1.0, abs(ov)), (perm, gv, ov)
+    base, imp = capi.permutation_importance(g, Xd.ptr, y, X.shape[1])
+    assert imp.shape == (X.shape[0],) and np.isfinite(imp).all()
+    g.close()
+    Xd.free()
