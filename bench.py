@@ -391,6 +391,19 @@ def main():
     pred_kernel_ms = pstats["total_ms"] / max(1, pstats["launches"])
     pred_bytes = n_pred * args.features * 4 + forest.n_nodes * 16 + n_pred * 10 * 4
 
+    # Out-of-bag pass over the training rows: every (tree, row) pair left
+    # out of the tree's bootstrap, plain and with one feature permuted.
+    capi.oob_evaluate(forest, Xd.ptr, y, args.rows)
+    t = time.perf_counter()
+    oob_pairs, oob_ok = capi.oob_evaluate(forest, Xd.ptr, y, args.rows)
+    oob_dt = time.perf_counter() - t
+    t = time.perf_counter()
+    _, perm_ok = capi.oob_evaluate(forest, Xd.ptr, y, args.rows, 0)
+    perm_dt = time.perf_counter() - t
+    oob = {"pairs": int(oob_pairs), "accuracy": round(oob_ok / oob_pairs, 4), "ms": round(1000 * oob_dt, 1),
+           "pairs_per_s": round(oob_pairs / oob_dt, 1), "permuted_pass_ms": round(1000 * perm_dt, 1),
+           "importance_feature_0": round((oob_ok - perm_ok) / oob_pairs, 5)}
+
     # Roofline of the dominant split-search kernel.
     # single-kernel families that carry algorithmic bytes (the chunk
     # pipeline's evaluation kernel carries its items' bytes)
@@ -447,6 +460,7 @@ def main():
         "predict": {"rows_per_s": round(n_pred / pred_dt, 1), "kernel_ms": round(pred_kernel_ms, 3),
                     "compulsory_GBps": round(pred_bytes / (pred_kernel_ms * 1e-3) / 1e9, 1) if pred_kernel_ms else None,
                     "train_accuracy": round(acc, 4)},
+        "oob": oob,
         "serving": serving,
         "kernel_ms_per_step": {k: round(v_["total_ms"] / args.steps, 3) for k, v_ in stats.items()},
         "roofline": roofline,

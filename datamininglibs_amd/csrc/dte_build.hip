@@ -836,6 +836,7 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
 // Entry order inside a bucket does not matter: the walk sorts by key and
 // scores only boundaries between distinct keys from integer prefix sums.
 constexpr int kV2Tile = 32768;   // entries per histogram tile
+constexpr int kScatterTile = 4096;  // entries per scatter tile (classification)
 constexpr int kV2MaxB = 4096;    // buckets per item
 constexpr int kV2Ipt = kIpt;     // entries per lane of the bucket evaluator
 constexpr int kV2Target = kDefBucketTarget;  // entries per bucket aimed at (default)
@@ -1056,7 +1057,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_scan(V2Args a, const uint32_t* li
 // so that consecutive threads write consecutive slots of a run.
 template <int TASK>
 __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv, V2Args a) {
-  constexpr int TILE = TASK == 0 ? 4096 : 2048;
+  constexpr int TILE = TASK == 0 ? kScatterTile : 2048;
   constexpr int U = TILE / kBlock;
   __shared__ uint32_t s_cnt[kV2MaxB];
   __shared__ uint32_t s_gpos[kV2MaxB];
@@ -3125,7 +3126,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
   B.host_items.resize(n_all);
   DTE_HIP(hipMemcpyAsync(B.host_items.data(), B.items.p, sizeof(V2Item) * n_all, hipMemcpyDeviceToHost, s));
   DTE_HIP(hipStreamSynchronize(s));
-  constexpr int kSTile = TASK == 0 ? 4096 : 2048;
+  constexpr int kSTile = TASK == 0 ? kScatterTile : 2048;
   int64_t k0 = 0;
   while (k0 < n_all) {
     int64_t k1 = k0, el = 0, nb = 0;
