@@ -813,6 +813,111 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
   if (t == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
 }
 
+// Wave-team tier, node-major: one wave per node walks the node's draws in
+// order.  The node's statistics, row ids and payloads are read once; the key
+// gather of the next draw is in flight while the current draw is sorted and
+// walked.  Same results as k_search_rf<.., 64, false> item by item.
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_search_rf_team(DataArgs d, LevelArgs lv, ItemArgs it) {
+  constexpr int TEAM = kWave;
+  constexpr int IPT = kTeamIpt;
+  __shared__ double s_inv[128], s_logc[128];
+  __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[kWaves];
+  load_tables(s_inv, s_logc);
+  __syncthreads();
+  const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
+  const int team = threadIdx.x / kWave;
+  TeamLds<MAXC, TASK, TEAM, IPT>& L = s_team[team];
+  const int t = threadIdx.x & (kWave - 1);
+  unsigned long long my_bytes = 0;
+  for (int64_t grp = (int64_t)blockIdx.x * kWaves + team; grp < it.n_groups; grp += (int64_t)gridDim.x * kWaves) {
+    const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
+    const uint32_t i = lv.open_idx[o];
+    const uint32_t tg = (uint32_t)lv.tree_global[lv.tree[i]];
+    const uint32_t nid = lv.id[i];
+    NodeCtx nc;
+    const bool pure = node_context<MAXC, TASK, TEAM, IPT>(d, lv, i, nc, L, lx);
+    if (pure) {
+      // Every boundary of a pure node has gain exactly 0: no positive split.
+      if (t == 0)
+        for (int jj = 0; jj < it.per_group; ++jj) {
+          const int j = it.j0 + jj;
+          const int64_t ridx = (int64_t)o * it.D + j;
+          it.res_score[ridx] = 0.0f;
+          it.res_thr[ridx] = 0.0f;
+          it.res_feat[ridx] = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)j) % (uint32_t)d.F);
+        }
+      team_sync<TEAM>();
+      continue;
+    }
+    const int m = lv.m[i];
+    const int64_t st = lv.start[i];
+    uint32_t rows[IPT], kn[IPT];
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const int e = t + q * TEAM;
+      if (e < m) {
+        rows[q] = lv.row[st + e];
+        L.aux[e] = lv.aux[st + e];
+        if (TASK == 1) L.y[e] = lv.ey[st + e];
+      }
+    }
+    int f_nx = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)it.j0) % (uint32_t)d.F);
+#pragma unroll
+    for (int q = 0; q < IPT; ++q)
+      if (t + q * TEAM < m) kn[q] = d.K[(int64_t)f_nx * d.N + rows[q]];
+    for (int jj = 0; jj < it.per_group; ++jj) {
+      const int j = it.j0 + jj;
+      const int64_t ridx = (int64_t)o * it.D + j;
+      const int f = f_nx;
+      uint32_t key[IPT];
+#pragma unroll
+      for (int q = 0; q < IPT; ++q) key[q] = kn[q];
+      if (jj + 1 < it.per_group) {
+        f_nx = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)(j + 1)) % (uint32_t)d.F);
+#pragma unroll
+        for (int q = 0; q < IPT; ++q)
+          if (t + q * TEAM < m) kn[q] = d.K[(int64_t)f_nx * d.N + rows[q]];
+      }
+      if (t == 0) {
+        for (int c = 0; c < MAXC; ++c) L.cr.run[c] = 0;
+        L.cr.run_n = 0;
+        L.cr.run_s = 0;
+        L.cr.last_key = 0;
+        L.cr.best_g = 0.0;
+        L.cr.best_key = kNone;
+        L.cr.best_prev = kNone;
+      }
+      uint32_t kmin = kNone, kmax = 0;
+#pragma unroll
+      for (int q = 0; q < IPT; ++q) {
+        const int e = t + q * TEAM;
+        if (e < m) {
+          kmin = min(kmin, key[q]);
+          kmax = max(kmax, key[q]);
+        }
+      }
+      kmin = team_reduce_min<TEAM>(kmin, L.w);
+      kmax = team_reduce_max<TEAM>(kmax, L.w);
+#pragma unroll
+      for (int q = 0; q < IPT; ++q) {
+        const int e = t + q * TEAM;
+        if (e < m) {
+          L.u.srt.k[0][e] = key[q] - kmin;
+          L.u.srt.v[0][e] = (uint16_t)e;
+        }
+      }
+      team_sync<TEAM>();
+      sort_and_eval<MAXC, TASK, TEAM, IPT>(m, bit_length(kmax - kmin), kmin, nc, L, lx);
+      team_sync<TEAM>();
+      if (t == 0) rf_write_result<MAXC, TASK, TEAM, IPT>(d, it, ridx, f, L.cr);
+      team_sync<TEAM>();
+    }
+    if (t == 0) my_bytes += (unsigned long long)m * it.per_group * (TASK == 0 ? 12ull : 20ull);
+  }
+  if (t == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
+}
+
 // ------------------------------------- chunk-parallel big-node search ---
 // Items (node, feature draw) above the workgroup's LDS capacity.  The keys of
 // an item (dense ranks of the drawn feature) are split into nb coarse buckets
@@ -3058,7 +3163,7 @@ void launch_search(int algorithm, int tier, int grid, hipStream_t s, const DataA
       case 0: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 16>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       case 1: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 32>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       case 2: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
-      case 3: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 64, false>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 3: hipLaunchKernelGGL((k_search_rf_team<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       case 4: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256, false>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       default: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256, true>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
     }
@@ -3733,7 +3838,8 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         continue;
       }
       int grid;
-      if (rf && tier <= 2) {
+      if (rf && tier <= 3) {
+        // node-major tiers: 4 / 2 / 1 / 1 nodes per wave
         const int npw = tier == 0 ? 4 : (tier == 1 ? 2 : 1);
         grid = (int)std::min<int64_t>((cnt[tier] + npw * kWaves - 1) / (npw * kWaves), wave_grid_cap);
       } else {
