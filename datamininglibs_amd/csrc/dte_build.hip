@@ -813,24 +813,25 @@ __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, 
   if (t == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
 }
 
-// Wave-team tier, node-major: one wave per node walks the node's draws in
-// order.  The node's statistics, row ids and payloads are read once; the key
-// gather of the next draw is in flight while the current draw is sorted and
-// walked.  Same results as k_search_rf<.., 64, false> item by item.
-template <int MAXC, int TASK>
+// Wave-team (TEAM = 64) and workgroup (TEAM = 256) tiers, node-major: one
+// team per node walks the node's draws in order.  The node's statistics, row
+// ids and payloads are read once; the key gather of the next draw is in
+// flight while the current draw is sorted and walked.  Same results as
+// k_search_rf<.., TEAM, false> item by item.
+template <int MAXC, int TASK, int TEAM>
 __global__ void __launch_bounds__(kBlock) k_search_rf_team(DataArgs d, LevelArgs lv, ItemArgs it) {
-  constexpr int TEAM = kWave;
-  constexpr int IPT = kTeamIpt;
+  constexpr int IPT = TEAM == kWave ? kTeamIpt : kIpt;
+  constexpr int NT = kBlock / TEAM;
   __shared__ double s_inv[128], s_logc[128];
-  __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[kWaves];
+  __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[NT];
   load_tables(s_inv, s_logc);
   __syncthreads();
   const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
-  const int team = threadIdx.x / kWave;
+  const int team = TEAM == kWave ? (int)(threadIdx.x / kWave) : 0;
   TeamLds<MAXC, TASK, TEAM, IPT>& L = s_team[team];
-  const int t = threadIdx.x & (kWave - 1);
+  const int t = team_tid<TEAM>();
   unsigned long long my_bytes = 0;
-  for (int64_t grp = (int64_t)blockIdx.x * kWaves + team; grp < it.n_groups; grp += (int64_t)gridDim.x * kWaves) {
+  for (int64_t grp = (int64_t)blockIdx.x * NT + team; grp < it.n_groups; grp += (int64_t)gridDim.x * NT) {
     const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
     const uint32_t i = lv.open_idx[o];
     const uint32_t tg = (uint32_t)lv.tree_global[lv.tree[i]];
@@ -3163,8 +3164,8 @@ void launch_search(int algorithm, int tier, int grid, hipStream_t s, const DataA
       case 0: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 16>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       case 1: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 32>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       case 2: hipLaunchKernelGGL((k_search_rf_seg<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
-      case 3: hipLaunchKernelGGL((k_search_rf_team<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
-      case 4: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256, false>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 3: hipLaunchKernelGGL((k_search_rf_team<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 4: hipLaunchKernelGGL((k_search_rf_team<MAXC, TASK, 256>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       default: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256, true>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
     }
   } else {
@@ -3842,6 +3843,8 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         // node-major tiers: 4 / 2 / 1 / 1 nodes per wave
         const int npw = tier == 0 ? 4 : (tier == 1 ? 2 : 1);
         grid = (int)std::min<int64_t>((cnt[tier] + npw * kWaves - 1) / (npw * kWaves), wave_grid_cap);
+      } else if (rf && tier == 4) {
+        grid = (int)std::min<int64_t>(cnt[tier], search_grid_cap);  // one node per workgroup
       } else {
         const int64_t items = (int64_t)cnt[tier] * it.per_group;
         const bool whole = rf ? tier == 4 : tier == 1;
