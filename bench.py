@@ -421,7 +421,7 @@ def main():
             tj = json.load(fh)
         if tj.get("family") == dom and tj.get("rows") == args.rows and tj.get("trees_per_fit") == args.trees:
             traffic = tj["hbm_bytes_per_dispatch"]
-            traffic_src = os.path.relpath(TRAFFIC_FILE, ROOT)
+            traffic_src = os.path.relpath(TRAFFIC_FILE, ROOT) + f" (measured at tuning {tj.get('tuning')})"
     ss = stats.get("split_search", {"total_ms": 0.0, "bytes": 0.0})
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
