@@ -426,6 +426,88 @@ __device__ __forceinline__ void lds_count_sort(const uint32_t* kin, const uint16
   team_sync<TEAM>();
 }
 
+// Sorts n <= TEAM*IPT (key, val) pairs whose keys lie below 2^bits, with
+// 2^bits above the counter span: a counting sort on the top digit
+// (key >> shift, shift = bits - log2(span)) from (k0, v0) into (k1, v1), then
+// every run of equal top digits ordered by each entry's rank inside its run
+// (smaller key, or equal key and earlier position), written back to (k0, v0).
+// Returns false, with (k0, v0) untouched, when a run holds more than max_run
+// entries: the caller sorts by radix passes instead.
+template <int IPT, int TEAM>
+__device__ __noinline__ bool lds_digit_sort(uint32_t* k0, uint16_t* v0, uint32_t* k1, uint16_t* v1, int n,
+                                               int bits, int span, int max_run, uint32_t* cnt, uint32_t* s_w) {
+  const int t = team_tid<TEAM>();
+  int lg = 0;
+  while ((1 << (lg + 1)) <= span) ++lg;
+  const int shift = bits - lg;
+  const int words = span >> 1;
+  for (int i = t; i < words; i += TEAM) cnt[i] = 0;
+  team_sync<TEAM>();
+  uint32_t r[IPT];
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const int e = t + q * TEAM;
+    if (e < n) {
+      const uint32_t dg = k0[e] >> shift;
+      const uint32_t sh = (dg & 1u) << 4;
+      r[q] = (atomicAdd(&cnt[dg >> 1], 1u << sh) >> sh) & 0xFFFFu;
+    }
+  }
+  team_sync<TEAM>();
+  const int per = (words + TEAM - 1) / TEAM;
+  const int lo = t * per, hi = min(lo + per, words);
+  uint32_t loc = 0, run_max = 0;
+  for (int i = lo; i < hi; ++i) {
+    const uint32_t w = cnt[i];
+    loc += (w & 0xFFFFu) + (w >> 16);
+    run_max = max(run_max, max(w & 0xFFFFu, w >> 16));
+  }
+  run_max = team_reduce_max<TEAM>(run_max, s_w);
+  if (run_max > (uint32_t)max_run) return false;  // team-uniform
+  uint32_t base = team_excl_scan<TEAM>(loc, s_w, nullptr);
+  for (int i = lo; i < hi; ++i) {
+    const uint32_t w = cnt[i];
+    const uint32_t c0 = w & 0xFFFFu;
+    cnt[i] = base | ((base + c0) << 16);
+    base += c0 + (w >> 16);
+  }
+  team_sync<TEAM>();
+  auto start = [&](uint32_t dg) -> int {
+    if ((int)dg >= span) return n;
+    return (int)((cnt[dg >> 1] >> ((dg & 1u) << 4)) & 0xFFFFu);
+  };
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const int e = t + q * TEAM;
+    if (e < n) {
+      const uint32_t k = k0[e];
+      const uint32_t dg = k >> shift;
+      const int pos = start(dg) + (int)r[q];
+      k1[pos] = k;
+      v1[pos] = v0[e];
+    }
+  }
+  team_sync<TEAM>();
+#pragma unroll
+  for (int q = 0; q < IPT; ++q) {
+    const int p = t + q * TEAM;
+    if (p < n) {
+      const uint32_t kp = k1[p];
+      const uint32_t dg = kp >> shift;
+      const int s0 = start(dg), s1 = start(dg + 1u);
+      int rank = 0;
+      for (int x = s0; x < s1; ++x) {
+        const uint32_t kx = k1[x];
+        rank += (kx < kp || (kx == kp && x < p)) ? 1 : 0;
+      }
+      k0[s0 + rank] = kp;
+      v0[s0 + rank] = v1[p];
+    }
+  }
+  team_sync<TEAM>();
+  return true;
+}
+
 // Compatibility names used by the whole-block kernels.
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
   return team_excl_scan<256>(v, s_w, total);

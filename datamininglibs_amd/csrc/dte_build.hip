@@ -46,6 +46,7 @@ namespace {
 
 constexpr int kIpt = 8;                 // elements per thread in the LDS sort
 constexpr int kTeamIpt = 8;             // entries per lane of the wave-per-item tier
+constexpr int kMaxDigitRun = 16;        // longest run of equal top digits ordered in place
 static_assert(kWave * kTeamIpt >= kMaxCutTeam, "wave tier capacity");
 constexpr int kCap = kBlock * kIpt;     // LDS chunk capacity (2048)
 constexpr int kHistWords = 8192;        // LDS histogram words (32 KB)
@@ -531,7 +532,7 @@ __device__ void eval_hist_bucket(const uint32_t* skey, const uint32_t* saux, con
 
 // Sort a chunk (keys relative to kbase, span < 2^bits) held in LDS buffer 0
 // and evaluate it.
-template <int MAXC, int TASK, int TEAM, int IPT>
+template <int MAXC, int TASK, int TEAM, int IPT, bool DIGIT = false>
 __device__ void sort_and_eval(int n, int bits, uint32_t kbase, const NodeCtx& nc,
                               TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx) {
   constexpr int kSpan = TeamLds<MAXC, TASK, TEAM, IPT>::SPAN;
@@ -539,6 +540,14 @@ __device__ void sort_and_eval(int n, int bits, uint32_t kbase, const NodeCtx& nc
     lds_count_sort<IPT, TEAM>(L.u.srt.k[0], L.u.srt.v[0], L.u.srt.k[1], L.u.srt.v[1], n, 1 << bits,
                               reinterpret_cast<uint32_t*>(L.u.srt.cnt), L.w);
     eval_sorted_chunk<MAXC, TASK, TEAM, IPT>(L.u.srt.k[1], L.u.srt.v[1], n, kbase, nc, L, lx);
+    return;
+  }
+  // Wide key spans: one counting pass on the top digit, then short runs
+  // ordered in place; radix passes when a run is long (clustered keys).
+  if (DIGIT && bits > 8 &&
+      lds_digit_sort<IPT, TEAM>(L.u.srt.k[0], L.u.srt.v[0], L.u.srt.k[1], L.u.srt.v[1], n, bits, kSpan, kMaxDigitRun,
+                                reinterpret_cast<uint32_t*>(L.u.srt.cnt), L.w)) {
+    eval_sorted_chunk<MAXC, TASK, TEAM, IPT>(L.u.srt.k[0], L.u.srt.v[0], n, kbase, nc, L, lx);
     return;
   }
   int cur = 0;
@@ -909,7 +918,7 @@ __global__ void __launch_bounds__(kBlock) k_search_rf_team(DataArgs d, LevelArgs
         }
       }
       team_sync<TEAM>();
-      sort_and_eval<MAXC, TASK, TEAM, IPT>(m, bit_length(kmax - kmin), kmin, nc, L, lx);
+      sort_and_eval<MAXC, TASK, TEAM, IPT, true>(m, bit_length(kmax - kmin), kmin, nc, L, lx);
       team_sync<TEAM>();
       if (t == 0) rf_write_result<MAXC, TASK, TEAM, IPT>(d, it, ridx, f, L.cr);
       team_sync<TEAM>();
