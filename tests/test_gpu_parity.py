@@ -220,3 +220,18 @@ def test_out_of_bag_pass_matches_oracle(task):
     assert imp.shape == (X.shape[0],) and np.isfinite(imp).all()
     g.close()
     Xd.free()
+
+
+@pytest.mark.parametrize("task", [capi.CLASSIFICATION, capi.REGRESSION])
+def test_rf_roots_above_the_bucket_table(task):
+    """Roots of 2M rows hold more than 4096 x 384 entries: every bucket of the
+    chunk pipeline overflows the wave capacity and takes the overfull-bucket
+    path; the shallow forest equals the oracle's."""
+    if task == capi.REGRESSION:
+        X, y = synth.sine_regression(seed=12, n=2_000_000, f=4)
+        kw = dict(task=task, n_classes=0)
+    else:
+        X, y = small_classes(12, 2_000_000, 4, 3)
+        kw = dict(n_classes=3)
+    g, o = fit_both(X, y, algorithm=capi.RANDOM_FOREST, n_trees=1, k=2, max_depth=2, min_node_size=5, seed=3, **kw)
+    assert_same_forest(g, o)
