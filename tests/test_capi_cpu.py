@@ -181,3 +181,30 @@ def test_forest_file_structure_checked_before_device(tmp_path):
     data = good.read_bytes()[:-20]
     trunc.write_bytes(data + np.array([_fnv1a64(data)], np.uint64).tobytes())
     assert L.dte_forest_load(str(trunc).encode(), 0, ctypes.byref(h)) == capi.ERR_INVALID_ARGUMENT
+
+
+def test_ctypes_structures_match_the_c_header(tmp_path):
+    """Sizes and field offsets of the ctypes mirrors equal the C compiler's."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    structs = {"dte_params": capi.Params, "dte_node": capi.Node, "dte_forest_info": capi.ForestInfo,
+               "dte_server_stats": capi.ServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dte.h"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'  printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-I", inc, "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f in cls._fields_:
+            assert got[(cname, f[0])] == getattr(cls, f[0]).offset, (cname, f[0])
