@@ -64,6 +64,54 @@ __global__ void __launch_bounds__(kBlock) k_predict_cls(const PNode* __restrict_
   }
 }
 
+// The same with the block's R rows staged in LDS first (coalesced reads of
+// X, feature-major or row-major), so that every visit reads its feature
+// value from LDS: block = R rows x (256 / R) tree lanes.
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_predict_cls_lds(const PNode* __restrict__ nodes,
+                                                            const int64_t* __restrict__ tree_start, int T,
+                                                            const float* __restrict__ X, int64_t n_rows, int64_t ld,
+                                                            int row_major, int C, int F, int trees_per_block,
+                                                            int32_t* __restrict__ votes) {
+  extern __shared__ uint32_t s_mem[];
+  uint32_t* s_votes = s_mem;                            // R * C
+  float* s_x = reinterpret_cast<float*>(s_mem + R * C);  // R rows of F + 1 floats
+  const int FP = F + 1;
+  const int lane = threadIdx.x % R;
+  const int group = threadIdx.x / R;
+  const int64_t row0 = (int64_t)blockIdx.x * R;
+  const int nr = (int)min<int64_t>(R, n_rows - row0);
+  const int t0 = blockIdx.y * trees_per_block;
+  const int t1 = min(T, t0 + trees_per_block);
+  for (int i = threadIdx.x; i < R * C; i += kBlock) s_votes[i] = 0;
+  if (row_major) {
+    for (int i = threadIdx.x; i < nr * F; i += kBlock) {
+      const int r = i / F, f = i - r * F;
+      s_x[r * FP + f] = X[(row0 + r) * ld + f];
+    }
+  } else {
+    for (int i = threadIdx.x; i < R * F; i += kBlock) {
+      const int f = i / R, r = i - f * R;
+      if (r < nr) s_x[r * FP + f] = X[(int64_t)f * ld + row0 + r];
+    }
+  }
+  __syncthreads();
+  if (lane < nr) {
+    const float* xr = s_x + lane * FP;
+    for (int t = t0 + group; t < t1; t += kBlock / R) {
+      PNode nd = nodes[tree_start[t]];
+      while (nd.feature >= 0) nd = nodes[nd.left + side_of(xr[nd.feature], nd.thr)];
+      atomicAdd(&s_votes[lane * C + nd.left], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < R * C; i += kBlock) {
+    const int r = i / C;
+    const uint32_t v = s_votes[i];
+    if (r < nr && v) atomicAdd(&votes[(row0 + r) * C + (i % C)], (int32_t)v);
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_predict_reg(const PNode* __restrict__ nodes,
                                                         const int64_t* __restrict__ tree_start, int T,
                                                         const float* __restrict__ X, int64_t n_rows, int64_t ld,
@@ -199,6 +247,27 @@ void predict_forest(const Forest& f, const float* X, int64_t n_rows, int64_t ld,
   if (n_rows <= 0) return;
   if (f.task == 0) {
     DTE_HIP(hipMemsetAsync(votes, 0, sizeof(int32_t) * n_rows * f.n_classes, s));
+    // Rows staged in LDS when a block's rows fit 48 KB: 64 rows up to 186
+    // features, 32 rows up to 373.
+    const int F = f.n_features, C = f.n_classes;
+    const int R = 4 * 64 * (F + 1 + C) <= 48 * 1024 ? 64 : (4 * 32 * (F + 1 + C) <= 48 * 1024 ? 32 : 0);
+    if (R) {
+      const int64_t row_blocks = (n_rows + R - 1) / R;
+      int chunks = 1;
+      while (row_blocks * chunks < 2048 && chunks * 32 < f.n_trees) chunks *= 2;
+      const int tpb = (f.n_trees + chunks - 1) / chunks;
+      const dim3 grid((unsigned)row_blocks, (unsigned)((f.n_trees + tpb - 1) / tpb));
+      const size_t lds = sizeof(uint32_t) * (size_t)R * (C + F + 1);
+      Timer tm("predict", 0.0, s);
+      if (R == 64)
+        hipLaunchKernelGGL(k_predict_cls_lds<64>, grid, dim3(kBlock), lds, s, f.pnodes.p, f.tree_start.p, f.n_trees,
+                           X, n_rows, ld, row_major, C, F, tpb, votes);
+      else
+        hipLaunchKernelGGL(k_predict_cls_lds<32>, grid, dim3(kBlock), lds, s, f.pnodes.p, f.tree_start.p, f.n_trees,
+                           X, n_rows, ld, row_major, C, F, tpb, votes);
+      DTE_HIP(hipGetLastError());
+      return;
+    }
     const int64_t row_blocks = (n_rows + kRowsPerBlock - 1) / kRowsPerBlock;
     // Enough tree chunks to fill the chip even for small requests.
     int chunks = 1;
