@@ -247,8 +247,8 @@ void predict_forest(const Forest& f, const float* X, int64_t n_rows, int64_t ld,
   if (n_rows <= 0) return;
   if (f.task == 0) {
     DTE_HIP(hipMemsetAsync(votes, 0, sizeof(int32_t) * n_rows * f.n_classes, s));
-    // Rows staged in LDS when a block's rows fit 48 KB: 64 rows up to 186
-    // features, 32 rows up to 373.
+    // Rows staged in LDS when a block's rows and votes fit 48 KB: 64 rows
+    // while F + C <= 191, 32 rows while F + C <= 383.
     const int F = f.n_features, C = f.n_classes;
     const int R = 4 * 64 * (F + 1 + C) <= 48 * 1024 ? 64 : (4 * 32 * (F + 1 + C) <= 48 * 1024 ? 32 : 0);
     if (R) {

@@ -86,6 +86,28 @@ def test_plain_equals_batched_and_oracle_over_tree_batches(algorithm):
     assert np.array_equal(gp.predict(Xp).astype(np.int64), o.predict(Xp).astype(np.int64))
 
 
+@pytest.mark.parametrize("f", [12, 250, 400])
+def test_predict_row_staging_widths(f):
+    """Classification predict stages a block's rows in LDS: 64 rows while
+    features + classes <= 191, 32 rows while <= 383, else rows are read from
+    HBM.  All three kernels, feature-major (predict) and row-major (serving),
+    give the oracle's votes; 1000 rows leave a ragged last block for both
+    row counts."""
+    X, y = small_classes(31, 800, f, 7)
+    g, o = fit_both(X, y, algorithm=capi.RANDOM_FOREST, n_trees=12, k=6, max_depth=10, min_node_size=3,
+                    n_classes=7)
+    Xp, _ = small_classes(32, 1000, f, 7)
+    want = o.predict(Xp).astype(np.int64)
+    assert np.array_equal(g.predict(Xp).astype(np.int64), want)
+    srv = capi.Server(max_batch_rows=1024)
+    try:
+        slot = srv.load(g)
+        got = srv.wait(srv.submit(slot, np.ascontiguousarray(Xp.T)))
+        assert np.array_equal(np.asarray(got).astype(np.int64), want)
+    finally:
+        srv.close()
+
+
 @pytest.mark.parametrize("algorithm", [capi.EXTRA_TREES, capi.RANDOM_FOREST])
 def test_plain_regression(algorithm):
     X, y = synth.sine_regression(seed=4, n=3000, f=6)
