@@ -4167,6 +4167,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   fo->tree_offset = t0;
   fo->fit_rows = algorithm == DTE_RANDOM_FOREST ? N : 0;
   fo->n_nodes = n_nodes;
+  fo->n_leaves = (n_nodes + T) / 2;  // per tree: nodes = 2 splits + 1, leaves = splits + 1
   fo->has_stats = true;
   PoolSuspend own;  // the forest outlives the fit
   fo->tree_start.alloc(T + 1);
@@ -4234,11 +4235,14 @@ Forest* merge_forests(std::vector<std::unique_ptr<Forest>>& parts, hipStream_t s
   fo->has_stats = true;
   int64_t n = 0;
   int T = 0;
+  int64_t leaves = 0;
   for (auto& q : parts) {
     n += q->n_nodes;
     T += q->n_trees;
+    leaves = (leaves < 0 || q->n_leaves < 0) ? -1 : leaves + q->n_leaves;
   }
   fo->n_nodes = n;
+  fo->n_leaves = leaves;
   fo->n_trees = T;
   const int C = fo->n_classes;
   const size_t W = fo->task == 0 ? (size_t)C : 1;

@@ -517,10 +517,13 @@ int dte_forest_free(dte_forest* forest) {
 int dte_forest_info_get(const dte_forest* forest, dte_forest_info* out) {
   return guarded([&] {
     if (!forest || !out) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
-    const Forest& f = forest->f;
+    Forest& f = const_cast<dte_forest*>(forest)->f;
     DTE_HIP(hipSetDevice(f.device));
-    std::vector<int32_t> feat(f.n_nodes);
-    DTE_HIP(hipMemcpy(feat.data(), f.feature.p, 4 * f.n_nodes, hipMemcpyDeviceToHost));
+    if (f.n_leaves < 0) {  // counted once from the node records, then kept
+      std::vector<int32_t> feat(f.n_nodes);
+      DTE_HIP(hipMemcpy(feat.data(), f.feature.p, 4 * f.n_nodes, hipMemcpyDeviceToHost));
+      f.n_leaves = count_leaves(feat);
+    }
     std::memset(out, 0, sizeof(*out));
     out->n_trees = f.n_trees;
     out->n_classes = f.n_classes;
@@ -529,7 +532,7 @@ int dte_forest_info_get(const dte_forest* forest, dte_forest_info* out) {
     out->algorithm = f.algorithm;
     out->fixed_shift = f.fixed_shift;
     out->n_nodes = f.n_nodes;
-    out->n_leaves = count_leaves(feat);
+    out->n_leaves = f.n_leaves;
     out->n_probabilities = out->n_leaves * (f.task == 0 ? f.n_classes : 1);
     out->device_bytes = (int64_t)(f.pnodes.bytes() + f.tree_start.bytes() + f.feature.bytes() + f.left.bytes() +
                                   f.depth.bytes() + f.count.bytes() + f.n_entries.bytes() + f.threshold.bytes() +
@@ -660,6 +663,7 @@ int dte_forest_import(const dte_node* nodes, int64_t n_nodes, const float* probs
     f.n_features = n_features;
     f.n_trees = n_trees;
     f.n_nodes = n;
+    f.n_leaves = count_leaves(feature);
     f.has_stats = false;
     f.tree_start.alloc(n_trees + 1);
     f.feature.alloc(n);
@@ -874,6 +878,7 @@ int dte_forest_load(const char* path, int32_t device_id, dte_forest** out) {
     f.fixed_shift = h.fixed_shift;
     f.algorithm = h.algorithm;
     f.n_nodes = n;
+    f.n_leaves = count_leaves(feature);
     f.has_stats = h.has_stats != 0;
     f.seed = h.seed;
     f.tree_offset = h.tree_offset;

@@ -186,6 +186,9 @@ struct Forest {
   int fixed_shift = 0;
   int algorithm = 0;
   int64_t n_nodes = 0;
+  // Leaves; -1 until known.  A built tree has one more leaf than splits
+  // (every split makes two children), so a fit knows it without a pass.
+  int64_t n_leaves = -1;
   // Bootstrap record of a fit (random forest): tree t drew its rows with
   // seed and global tree id tree_offset + t from fit_rows rows; fit_rows = 0
   // when unknown (imported forests).

@@ -76,6 +76,9 @@ def test_plain_equals_batched_and_oracle_over_tree_batches(algorithm):
     cb, cp = gb.canonical(), gp.canonical()
     for k in cb:
         assert np.array_equal(cb[k], cp[k]), k
+    # leaf counts kept from the build (batched) and from the merge of the
+    # plain path's batches equal the count over the node records
+    assert gb.info.n_leaves == gp.info.n_leaves == int((cb["feature"] < 0).sum())
     nb, pb = gb.export()
     npl, ppl = gp.export()
     assert np.array_equal(nb, npl) and np.array_equal(pb, ppl)
@@ -202,6 +205,7 @@ def test_forest_file_round_trip(tmp_path, algorithm, task):
         assert np.array_equal(a[k], b[k]), k
     (na, pa), (nb, pb) = f.export(), g.export()
     assert np.array_equal(na, nb) and np.array_equal(pa, pb)
+    assert f.info.n_leaves == g.info.n_leaves == int((a["feature"] < 0).sum())
     Xp = X[:, :500]
     assert np.array_equal(f.predict(Xp), g.predict(Xp))
     raw = bytearray(path.read_bytes())
