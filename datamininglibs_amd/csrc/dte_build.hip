@@ -3221,6 +3221,8 @@ struct BigState {
   std::vector<V2Item> host_items;
   std::vector<V2Tile> host_htiles, host_stiles, host_ctiles;
   std::vector<uint32_t> host_lists;
+  std::vector<int64_t> host_fcount;   // histogram tiles: items per feature
+  std::vector<uint32_t> host_forder;  // batch items in feature order
   int target = kV2Target;  // entries per bucket aimed at
 };
 
@@ -3254,7 +3256,6 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
       bi.boff = (int32_t)nb;
       bi.soff = el;
       const uint32_t li = (uint32_t)(k1 - k0);
-      for (int e0 = 0; e0 < bi.m; e0 += kV2Tile) B.host_htiles.push_back(V2Tile{li, (uint32_t)e0});
       for (int e0 = 0; e0 < bi.m; e0 += kSTile) B.host_stiles.push_back(V2Tile{li, (uint32_t)e0});
       for (int b0 = 0; b0 < bi.nb; b0 += kV2CTile) B.host_ctiles.push_back(V2Tile{li, (uint32_t)b0});
       el += bi.m;
@@ -3262,6 +3263,17 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
       ++k1;
     }
     const int64_t n_items = k1 - k0;
+    // Histogram tiles feature-major: the tiles in flight gather from a few
+    // feature columns of K at a time instead of from all of them, so the
+    // columns stay in the caches.  The histogram's results do not depend on
+    // the tile order (per-bucket counts and maxima, keys stored per entry).
+    B.host_fcount.assign((size_t)d.F + 1, 0);
+    for (int64_t k = k0; k < k1; ++k) ++B.host_fcount[(size_t)B.host_items[k].f + 1];
+    for (int f = 0; f < d.F; ++f) B.host_fcount[(size_t)f + 1] += B.host_fcount[(size_t)f];
+    B.host_forder.resize((size_t)n_items);
+    for (int64_t k = k0; k < k1; ++k) B.host_forder[B.host_fcount[(size_t)B.host_items[k].f]++] = (uint32_t)(k - k0);
+    for (const uint32_t li : B.host_forder)
+      for (int e0 = 0; e0 < B.host_items[k0 + li].m; e0 += kV2Tile) B.host_htiles.push_back(V2Tile{li, (uint32_t)e0});
     for (int64_t k = k0; k < k1; ++k)
       if (B.host_items[k].nb <= kV2SmallB) B.host_lists.push_back((uint32_t)(k - k0));
     const int64_t n_small = (int64_t)B.host_lists.size();
