@@ -1064,10 +1064,14 @@ __device__ __forceinline__ int64_t v2_team_item(const uint32_t* list, int64_t n_
   return k < n_list ? (int64_t)list[k] : -1;
 }
 
-template <int TASK>
+// Items of at most kV2FewB buckets take the scatter variant with small
+// bucket tables (less LDS: three workgroups per CU instead of two).
+constexpr int kV2FewB = 512;
+
+template <int TASK, int MAXB>
 __global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2Args a) {
-  __shared__ uint32_t s_cnt[kV2MaxB];
-  __shared__ uint32_t s_max[kV2MaxB];
+  __shared__ uint32_t s_cnt[MAXB];
+  __shared__ uint32_t s_max[MAXB];
   const V2Tile tl = a.tiles[blockIdx.x];
   const V2Item it = a.items[tl.item];
   const int t = threadIdx.x;
@@ -1170,12 +1174,12 @@ __global__ void __launch_bounds__(kBlock) k_v2_scan(V2Args a, const uint32_t* li
 // bucket from an LDS counter, the tile reserves one run per bucket in the
 // item's scratch, and the entries are staged in LDS in (bucket, rank) order
 // so that consecutive threads write consecutive slots of a run.
-template <int TASK>
+template <int TASK, int MAXB>
 __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv, V2Args a) {
   constexpr int TILE = TASK == 0 ? kScatterTile : 2048;
   constexpr int U = TILE / kBlock;
-  __shared__ uint32_t s_cnt[kV2MaxB];
-  __shared__ uint32_t s_gpos[kV2MaxB];
+  __shared__ uint32_t s_cnt[MAXB];
+  __shared__ uint32_t s_gpos[MAXB];
   __shared__ uint32_t s_key[TILE];
   __shared__ uint32_t s_aux[TILE];
   __shared__ uint16_t s_b[TILE];
@@ -1207,7 +1211,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
   }
   __syncthreads();
   {
-    constexpr int PER = kV2MaxB / kBlock;
+    constexpr int PER = MAXB / kBlock;
     const int b0 = t * PER;
     uint32_t c[PER], loc = 0;
 #pragma unroll
@@ -3219,7 +3223,7 @@ struct BigState {
   DevBuf<unsigned int> ctr;
   DevBuf<uint32_t> lists;  // item indices: small items (wave teams), then the rest
   std::vector<V2Item> host_items;
-  std::vector<V2Tile> host_htiles, host_stiles, host_ctiles;
+  std::vector<V2Tile> host_htiles, host_stiles, host_ctiles, host_stiles_many;
   std::vector<uint32_t> host_lists;
   std::vector<int64_t> host_fcount;   // histogram tiles: items per feature
   std::vector<uint32_t> host_forder;  // batch items in feature order
@@ -3249,6 +3253,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     int64_t k1 = k0, el = 0, nb = 0;
     B.host_htiles.clear();
     B.host_stiles.clear();
+    B.host_stiles_many.clear();
     B.host_ctiles.clear();
     B.host_lists.clear();
     while (k1 < n_all && (k1 == k0 || el + B.host_items[k1].m <= kV2Batch)) {
@@ -3256,7 +3261,8 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
       bi.boff = (int32_t)nb;
       bi.soff = el;
       const uint32_t li = (uint32_t)(k1 - k0);
-      for (int e0 = 0; e0 < bi.m; e0 += kSTile) B.host_stiles.push_back(V2Tile{li, (uint32_t)e0});
+      auto& st = bi.nb <= kV2FewB ? B.host_stiles : B.host_stiles_many;
+      for (int e0 = 0; e0 < bi.m; e0 += kSTile) st.push_back(V2Tile{li, (uint32_t)e0});
       for (int b0 = 0; b0 < bi.nb; b0 += kV2CTile) B.host_ctiles.push_back(V2Tile{li, (uint32_t)b0});
       el += bi.m;
       nb += bi.nb;
@@ -3274,6 +3280,10 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     for (int64_t k = k0; k < k1; ++k) B.host_forder[B.host_fcount[(size_t)B.host_items[k].f]++] = (uint32_t)(k - k0);
     for (const uint32_t li : B.host_forder)
       for (int e0 = 0; e0 < B.host_items[k0 + li].m; e0 += kV2Tile) B.host_htiles.push_back(V2Tile{li, (uint32_t)e0});
+    // scatter tiles of items with few buckets first, then the rest
+    const int64_t n_sfew = (int64_t)B.host_stiles.size();
+    B.host_stiles.insert(B.host_stiles.end(), B.host_stiles_many.begin(), B.host_stiles_many.end());
+    const int64_t n_sall = (int64_t)B.host_stiles.size();
     for (int64_t k = k0; k < k1; ++k)
       if (B.host_items[k].nb <= kV2SmallB) B.host_lists.push_back((uint32_t)(k - k0));
     const int64_t n_small = (int64_t)B.host_lists.size();
@@ -3353,7 +3363,11 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     a.tiles = B.htiles.p;
     {
       Timer tm("chunk_hist", 0.0, s);
-      hipLaunchKernelGGL(k_v2_hist<TASK>, dim3((unsigned)B.host_htiles.size()), dim3(kBlock), 0, s, d, lv, a);
+      // one launch in feature order: a smaller bucket table (more
+      // workgroups in flight) spreads the gathers over more columns and is
+      // slower
+      hipLaunchKernelGGL((k_v2_hist<TASK, kV2MaxB>), dim3((unsigned)B.host_htiles.size()), dim3(kBlock), 0, s, d, lv,
+                         a);
     }
     {
       Timer tm("chunk_scan", 0.0, s);
@@ -3367,7 +3381,12 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     a.tiles = B.stiles.p;
     {
       Timer tm("chunk_scatter", 0.0, s);
-      hipLaunchKernelGGL(k_v2_scatter<TASK>, dim3((unsigned)B.host_stiles.size()), dim3(kBlock), 0, s, d, lv, a);
+      if (n_sfew)
+        hipLaunchKernelGGL((k_v2_scatter<TASK, kV2FewB>), dim3((unsigned)n_sfew), dim3(kBlock), 0, s, d, lv, a);
+      a.tiles = B.stiles.p + n_sfew;
+      if (n_sall > n_sfew)
+        hipLaunchKernelGGL((k_v2_scatter<TASK, kV2MaxB>), dim3((unsigned)(n_sall - n_sfew)), dim3(kBlock), 0, s, d,
+                           lv, a);
     }
     a.tiles = B.ctiles.p;
     {
