@@ -951,6 +951,7 @@ __global__ void __launch_bounds__(kBlock) k_search_rf_team(DataArgs d, LevelArgs
 // Entry order inside a bucket does not matter: the walk sorts by key and
 // scores only boundaries between distinct keys from integer prefix sums.
 constexpr int kV2Tile = 32768;   // entries per histogram tile
+constexpr int kXcds = 8;          // XCDs of an MI355X, one L2 each
 constexpr int kScatterTile = 4096;  // entries per scatter tile (classification)
 constexpr int kV2MaxB = 4096;    // buckets per item
 constexpr int kV2Ipt = kIpt;     // entries per lane of the bucket evaluator
@@ -3227,6 +3228,7 @@ struct BigState {
   std::vector<uint32_t> host_lists;
   std::vector<int64_t> host_fcount;   // histogram tiles: items per feature
   std::vector<uint32_t> host_forder;  // batch items in feature order
+  std::vector<V2Tile> host_forder_tiles;
   int target = kV2Target;  // entries per bucket aimed at
 };
 
@@ -3279,7 +3281,21 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     B.host_forder.resize((size_t)n_items);
     for (int64_t k = k0; k < k1; ++k) B.host_forder[B.host_fcount[(size_t)B.host_items[k].f]++] = (uint32_t)(k - k0);
     for (const uint32_t li : B.host_forder)
-      for (int e0 = 0; e0 < B.host_items[k0 + li].m; e0 += kV2Tile) B.host_htiles.push_back(V2Tile{li, (uint32_t)e0});
+      for (int e0 = 0; e0 < B.host_items[k0 + li].m; e0 += kV2Tile) B.host_forder_tiles.push_back(V2Tile{li, (uint32_t)e0});
+    {
+      // Workgroups go to the 8 XCDs round robin (workgroup w to XCD w % 8),
+      // each XCD with its own L2: XCD x takes the x-th eighth of the
+      // feature-ordered tiles, so each L2 holds its own few columns.
+      const int64_t nt = (int64_t)B.host_forder_tiles.size();
+      const int64_t q = (nt + kXcds - 1) / kXcds;
+      B.host_htiles.reserve((size_t)nt);
+      for (int64_t j = 0; j < q; ++j)
+        for (int x = 0; x < kXcds; ++x) {
+          const int64_t src = x * q + j;
+          if (src < std::min<int64_t>((int64_t)(x + 1) * q, nt)) B.host_htiles.push_back(B.host_forder_tiles[src]);
+        }
+      B.host_forder_tiles.clear();
+    }
     // scatter tiles of items with few buckets first, then the rest
     const int64_t n_sfew = (int64_t)B.host_stiles.size();
     B.host_stiles.insert(B.host_stiles.end(), B.host_stiles_many.begin(), B.host_stiles_many.end());
