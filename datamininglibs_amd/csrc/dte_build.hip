@@ -133,6 +133,32 @@ struct LnCtx {
   }
 };
 
+// The same with a copy of T[0, nTL) in LDS, for kernels whose nodes are
+// small (their lookups mostly fall below nTL).
+struct LnCtxL {
+  const double* inv;
+  const double* logc;
+  const int64_t* T;
+  int64_t Tmax;
+  int F;
+  const int64_t* TL;
+  int64_t nTL;
+  __device__ __forceinline__ int64_t t(uint64_t x) const {
+    if (x < (uint64_t)nTL) return TL[x];
+    return x <= (uint64_t)Tmax ? T[x] : xlnx_fixed_slow(x, F, inv, logc);
+  }
+};
+
+// Stages T[0, min(n, Tmax + 1)) into LDS; returns the staged length (0 when
+// the fit has no table: regression).
+template <int NT>
+__device__ __forceinline__ int64_t load_t_prefix(int64_t* s_T, const int64_t* T, int64_t Tmax) {
+  if (!T) return 0;
+  const int64_t n = Tmax + 1 < NT ? Tmax + 1 : NT;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s_T[i] = T[i];
+  return n;
+}
+
 struct NodeCtx {
   int64_t n;        // sum of weights
   int64_t ssum;     // regression: fixed-point target sum
@@ -230,9 +256,9 @@ struct WalkState {
   int64_t nl;
 };
 
-template <int MAXC, int TASK, int TEAM, int IPT>
+template <int MAXC, int TASK, int TEAM, int IPT, class LX>
 __device__ __forceinline__ WalkState walk_init(const TeamLds<MAXC, TASK, TEAM, IPT>& L, const NodeCtx& nc,
-                                               const LnCtx& lx) {
+                                               const LX& lx) {
   const int t = team_tid<TEAM>();
   WalkState w{0, 0, 0};
   for (int c = 0; c < nc.C; ++c) {
@@ -244,14 +270,15 @@ __device__ __forceinline__ WalkState walk_init(const TeamLds<MAXC, TASK, TEAM, I
   return w;
 }
 
-__device__ __forceinline__ double walk_gain(const WalkState& w, const NodeCtx& nc, const LnCtx& lx) {
+template <class LX>
+__device__ __forceinline__ double walk_gain(const WalkState& w, const NodeCtx& nc, const LX& lx) {
   const int64_t q = (w.sl - lx.t((uint64_t)w.nl)) + (w.sr - lx.t((uint64_t)(nc.n - w.nl)));
   return ig_from_fixed(nc.P + q, lx.F, nc.inv);
 }
 
-template <int MAXC, int TASK, int TEAM, int IPT>
+template <int MAXC, int TASK, int TEAM, int IPT, class LX>
 __device__ __forceinline__ void walk_add(WalkState& w, TeamLds<MAXC, TASK, TEAM, IPT>& L, int lab, uint32_t add,
-                                         const LnCtx& lx) {
+                                         const LX& lx) {
   const int t = team_tid<TEAM>();
   const uint32_t lo = L.pre[lab * TEAM + t];
   const uint32_t ln = lo + add;
@@ -273,9 +300,9 @@ __device__ __forceinline__ void carry_best(Carry<MAXC, TASK>& cr, const BestCand
 
 // Evaluate every boundary of a key-sorted chunk (keys relative to kbase,
 // vals index the chunk's payload in L.aux / L.y).
-template <int MAXC, int TASK, int TEAM, int IPT>
+template <int MAXC, int TASK, int TEAM, int IPT, class LX>
 __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, int n, uint32_t kbase,
-                                  const NodeCtx& nc, TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx) {
+                                  const NodeCtx& nc, TeamLds<MAXC, TASK, TEAM, IPT>& L, const LX& lx) {
   const int t = team_tid<TEAM>();
   const int ipt = lane_span<TEAM>(n);
   const int lo = t * ipt;
@@ -532,9 +559,9 @@ __device__ void eval_hist_bucket(const uint32_t* skey, const uint32_t* saux, con
 
 // Sort a chunk (keys relative to kbase, span < 2^bits) held in LDS buffer 0
 // and evaluate it.
-template <int MAXC, int TASK, int TEAM, int IPT, bool DIGIT = false>
+template <int MAXC, int TASK, int TEAM, int IPT, bool DIGIT = false, class LX = LnCtx>
 __device__ void sort_and_eval(int n, int bits, uint32_t kbase, const NodeCtx& nc,
-                              TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx) {
+                              TeamLds<MAXC, TASK, TEAM, IPT>& L, const LX& lx) {
   constexpr int kSpan = TeamLds<MAXC, TASK, TEAM, IPT>::SPAN;
   if (bits > 4 && (1 << bits) <= kSpan) {
     lds_count_sort<IPT, TEAM>(L.u.srt.k[0], L.u.srt.v[0], L.u.srt.k[1], L.u.srt.v[1], n, 1 << bits,
@@ -569,9 +596,9 @@ __device__ __forceinline__ void load_tables(double* s_inv, double* s_logc) {
 }
 
 // Node statistics of one item, computed by every lane of the team.
-template <int MAXC, int TASK, int TEAM, int IPT>
+template <int MAXC, int TASK, int TEAM, int IPT, class LX>
 __device__ __forceinline__ bool node_context(const DataArgs& d, const LevelArgs& lv, uint32_t i, NodeCtx& nc,
-                                             TeamLds<MAXC, TASK, TEAM, IPT>& L, const LnCtx& lx) {
+                                             TeamLds<MAXC, TASK, TEAM, IPT>& L, const LX& lx) {
   const int t = team_tid<TEAM>();
   nc.C = d.C;
   nc.n = lv.count[i];
@@ -831,11 +858,21 @@ template <int MAXC, int TASK, int TEAM>
 __global__ void __launch_bounds__(kBlock) k_search_rf_team(DataArgs d, LevelArgs lv, ItemArgs it) {
   constexpr int IPT = TEAM == kWave ? kTeamIpt : kIpt;
   constexpr int NT = kBlock / TEAM;
+  // LDS copy of T's head for wave teams (still 2 workgroups per CU); the
+  // workgroup teams keep their registers for occupancy instead
+  constexpr int kTL = (TASK == 0 && TEAM == kWave) ? 1024 : 0;
   __shared__ double s_inv[128], s_logc[128];
+  __shared__ int64_t s_T[kTL > 0 ? kTL : 1];
   __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[NT];
   load_tables(s_inv, s_logc);
+  const int64_t nTL = kTL > 0 ? load_t_prefix<kTL>(s_T, d.T, d.Tmax) : 0;
   __syncthreads();
-  const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
+  const auto lx = [&]() {
+    if constexpr (kTL > 0)
+      return LnCtxL{s_inv, s_logc, d.T, d.Tmax, d.igF, s_T, nTL};
+    else
+      return LnCtx{s_inv, s_logc, d.T, d.Tmax, d.igF};
+  }();
   const int team = TEAM == kWave ? (int)(threadIdx.x / kWave) : 0;
   TeamLds<MAXC, TASK, TEAM, IPT>& L = s_team[team];
   const int t = team_tid<TEAM>();
@@ -2011,10 +2048,13 @@ __device__ __forceinline__ BestCand seg_best(BestCand b) {
 template <int MAXC, int TASK, int SEG>
 __global__ void __launch_bounds__(kBlock) k_search_rf_seg(DataArgs d, LevelArgs lv, ItemArgs it) {
   constexpr int NPW = kWave / SEG;  // nodes per wave
+  constexpr int kTL = 2048;         // LDS copy of T's head
   __shared__ double s_inv[128], s_logc[128];
+  __shared__ int64_t s_T[TASK == 0 ? kTL : 1];
   load_tables(s_inv, s_logc);
+  const int64_t nTL = TASK == 0 ? load_t_prefix<kTL>(s_T, d.T, d.Tmax) : 0;
   __syncthreads();
-  const LnCtx lx{s_inv, s_logc, d.T, d.Tmax, d.igF};
+  const LnCtxL lx{s_inv, s_logc, d.T, d.Tmax, d.igF, s_T, nTL};
   const int sl = threadIdx.x & (SEG - 1);
   const int seg = (threadIdx.x & (kWave - 1)) / SEG;
   unsigned long long my_bytes = 0;
