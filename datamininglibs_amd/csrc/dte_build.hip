@@ -2488,8 +2488,13 @@ struct PartArgs {
   const uint32_t* list;  // optional: which open positions to process
 };
 
+// Side bits of the first pass kept for the scatter pass (nodes of up to
+// 64 Ki entries), which then reads no X.
+constexpr int kSideWords = 1024;
+
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, PartArgs pa) {
+  __shared__ unsigned long long s_side[kSideWords];
   __shared__ uint32_t s_w[kWaves];
   __shared__ uint32_t s_cls[MAXC];
   __shared__ unsigned long long s_cnt, s_sum, s_hash;
@@ -2518,9 +2523,15 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
     uint64_t cnt = 0, hsh = 0;
     int64_t sm = 0;
     uint32_t ml = 0;
+    const bool keep = m <= kSideWords * kWave;
     for (int e = t; e < m; e += kBlock) {
       const uint32_t row = lv.row[st + e];
-      if (side_of(Xf[row], thr) == 0) {
+      const bool isl = side_of(Xf[row], thr) == 0;
+      if (keep) {
+        const unsigned long long bits = __ballot(isl);
+        if ((t & (kWave - 1)) == 0) s_side[e / kWave] = bits;
+      }
+      if (isl) {
         const uint32_t a = lv.aux[st + e];
         const uint32_t w = a & 0xFFFFu;
         ++ml;
@@ -2577,7 +2588,8 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
         uint32_t row = 0, isl = 0;
         if (e < m) {
           row = lv.row[st + e];
-          isl = side_of(Xf[row], thr) == 0 ? 1u : 0u;
+          isl = keep ? (uint32_t)(s_side[e / kWave] >> (e & (kWave - 1))) & 1u
+                     : (side_of(Xf[row], thr) == 0 ? 1u : 0u);
         }
         uint32_t tot;
         const uint32_t pl = block_excl_scan(isl, s_w, &tot);
@@ -2618,12 +2630,15 @@ __global__ void __launch_bounds__(kBlock) k_partition_wave(DataArgs d, LevelArgs
     uint64_t cnt = 0, hsh = 0;
     int64_t sm = 0;
     uint32_t ml = 0;
+    uint32_t side = 0;  // bit k: entry k * 64 + lane goes left (nodes of up to 2048 entries)
+    const bool keep = m <= 32 * kWave;
     for (int e0 = 0; e0 < m; e0 += kWave) {
       const int e = e0 + lane;
       bool isl = false;
       if (e < m) {
         const uint32_t row = lv.row[st + e];
         isl = side_of(Xf[row], thr) == 0;
+        if (isl && keep) side |= 1u << (e0 / kWave);
         if (isl) {
           const uint32_t a = lv.aux[st + e];
           const uint32_t w = a & 0xFFFFu;
@@ -2673,7 +2688,7 @@ __global__ void __launch_bounds__(kBlock) k_partition_wave(DataArgs d, LevelArgs
         bool isl = false;
         if (e < m) {
           row = lv.row[st + e];
-          isl = side_of(Xf[row], thr) == 0;
+          isl = keep ? ((side >> (e0 / kWave)) & 1u) != 0 : side_of(Xf[row], thr) == 0;
         }
         const uint64_t bl = __ballot(isl);
         const uint64_t br = __ballot(e < m && !isl);
