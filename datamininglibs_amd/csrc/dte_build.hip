@@ -987,7 +987,7 @@ __global__ void __launch_bounds__(kBlock) k_search_rf_team(DataArgs d, LevelArgs
 //   8. k_v2_reduce   per item: best bucket -> (score, threshold)
 // Entry order inside a bucket does not matter: the walk sorts by key and
 // scores only boundaries between distinct keys from integer prefix sums.
-constexpr int kV2Tile = 32768;   // entries per histogram tile
+constexpr int kV2Tile = 16384;   // entries per histogram tile
 constexpr int kXcds = 8;          // XCDs of an MI355X, one L2 each
 constexpr int kScatterTile = 4096;  // entries per scatter tile (classification)
 constexpr int kV2MaxB = 4096;    // buckets per item
