@@ -193,6 +193,88 @@ __global__ void __launch_bounds__(kBlock) k_oob(const PNode* __restrict__ nodes,
   }
 }
 
+// The same pairs with a block's R rows staged in LDS (feature-major X read
+// coalesced), R rows x 256 / R tree lanes, trees [t0, t1) of the block's
+// chunk; the permuted feature still reads its drawn row from X.
+template <int TASK, int R>
+__global__ void __launch_bounds__(kBlock) k_oob_lds(const PNode* __restrict__ nodes,
+                                                    const int64_t* __restrict__ tree_start, int T,
+                                                    const float* __restrict__ X, int64_t N, int F,
+                                                    const uint32_t* __restrict__ bits, int64_t words,
+                                                    const uint32_t* __restrict__ label, const float* __restrict__ y,
+                                                    int32_t tree0, uint64_t seed, int perm, int trees_per_block,
+                                                    unsigned long long* counts, double* sse) {
+  extern __shared__ float s_x[];  // R rows of F + 1 floats
+  const int FP = F + 1;
+  const int lane = threadIdx.x % R;
+  const int group = threadIdx.x / R;
+  const int64_t row0 = (int64_t)blockIdx.x * R;
+  const int nr = (int)min<int64_t>(R, N - row0);
+  const int t0 = blockIdx.y * trees_per_block;
+  const int t1 = min(T, t0 + trees_per_block);
+  for (int i = threadIdx.x; i < R * F; i += kBlock) {
+    const int f = i / R, r = i - f * R;
+    if (r < nr) s_x[r * FP + f] = X[(int64_t)f * N + row0 + r];
+  }
+  __syncthreads();
+  unsigned long long n_pairs = 0, n_ok = 0;
+  double e2 = 0.0;
+  if (lane < nr) {
+    const int64_t r = row0 + lane;
+    const float* xr = s_x + lane * FP;
+    const uint32_t lab = TASK == 0 ? label[r] : 0u;
+    const double yr = TASK == 1 ? (double)y[r] : 0.0;
+    for (int t = t0 + group; t < t1; t += kBlock / R) {
+      if ((bits[(int64_t)t * words + (r >> 5)] >> (r & 31)) & 1u) continue;  // in the bag
+      const uint32_t tg = (uint32_t)(tree0 + t);
+      PNode nd = nodes[tree_start[t]];
+      uint32_t depth = 0;
+      while (nd.feature >= 0) {
+        float v;
+        if (nd.feature == perm)
+          v = X[(int64_t)perm * N + rand_u32(seed, kStreamImp, tg, (uint32_t)r, depth) % (uint32_t)N];
+        else
+          v = xr[nd.feature];
+        nd = nodes[nd.left + side_of(v, nd.thr)];
+        ++depth;
+      }
+      ++n_pairs;
+      if (TASK == 0) {
+        n_ok += (uint32_t)nd.left == lab ? 1ull : 0ull;
+      } else {
+        const double e = (double)nd.thr - yr;
+        e2 += e * e;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    n_pairs += __shfl_xor(n_pairs, off, kWave);
+    n_ok += __shfl_xor(n_ok, off, kWave);
+    if (TASK == 1) e2 += __shfl_xor(e2, off, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0 && n_pairs) {
+    atomicAdd(&counts[0], n_pairs);
+    if (TASK == 0) atomicAdd(&counts[1], n_ok);
+    else atomicAdd(sse, e2);
+  }
+}
+
+// Rows per block of the LDS-staged kernels: 64 while the block's rows (and
+// vote counters, C) fit 48 KB, else 32, else 0 (rows read from HBM).
+int staged_rows(int F, int C) {
+  if (4 * 64 * (F + 1 + C) <= 48 * 1024) return 64;
+  if (4 * 32 * (F + 1 + C) <= 48 * 1024) return 32;
+  return 0;
+}
+
+// Tree chunks so that row blocks x chunks fill the chip even for few rows.
+int tree_chunks(int64_t row_blocks, int n_trees) {
+  int chunks = 1;
+  while (row_blocks * chunks < 2048 && chunks * 32 < n_trees) chunks *= 2;
+  return chunks;
+}
+
 }  // namespace
 
 void oob_evaluate(const Forest& f, const float* X, const float* targets, int64_t N, int perm, double out[2],
@@ -226,12 +308,34 @@ void oob_evaluate(const Forest& f, const float* X, const float* targets, int64_t
   DevBuf<double> sse(1);
   DTE_HIP(hipMemsetAsync(counts.p, 0, counts.bytes(), s));
   DTE_HIP(hipMemsetAsync(sse.p, 0, sse.bytes(), s));
-  if (f.task == 0)
+  const int R = staged_rows(f.n_features, 0);
+  if (R) {
+    const int64_t row_blocks = (N + R - 1) / R;
+    const int tpb = (T + tree_chunks(row_blocks, T) - 1) / tree_chunks(row_blocks, T);
+    const dim3 grid((unsigned)row_blocks, (unsigned)((T + tpb - 1) / tpb));
+    const size_t lds = sizeof(float) * (size_t)R * (f.n_features + 1);
+    const uint32_t* lab = label.p;
+    const float* yp = yv.p;
+    const int Fn = f.n_features;
+    if (f.task == 0 && R == 64)
+      hipLaunchKernelGGL((k_oob_lds<0, 64>), grid, dim3(kBlock), lds, s, f.pnodes.p, f.tree_start.p, T, X, N, Fn,
+                         bits.p, words, lab, yp, f.tree_offset, f.seed, perm, tpb, counts.p, sse.p);
+    else if (f.task == 0)
+      hipLaunchKernelGGL((k_oob_lds<0, 32>), grid, dim3(kBlock), lds, s, f.pnodes.p, f.tree_start.p, T, X, N, Fn,
+                         bits.p, words, lab, yp, f.tree_offset, f.seed, perm, tpb, counts.p, sse.p);
+    else if (R == 64)
+      hipLaunchKernelGGL((k_oob_lds<1, 64>), grid, dim3(kBlock), lds, s, f.pnodes.p, f.tree_start.p, T, X, N, Fn,
+                         bits.p, words, lab, yp, f.tree_offset, f.seed, perm, tpb, counts.p, sse.p);
+    else
+      hipLaunchKernelGGL((k_oob_lds<1, 32>), grid, dim3(kBlock), lds, s, f.pnodes.p, f.tree_start.p, T, X, N, Fn,
+                         bits.p, words, lab, yp, f.tree_offset, f.seed, perm, tpb, counts.p, sse.p);
+  } else if (f.task == 0) {
     hipLaunchKernelGGL(k_oob<0>, dim3((unsigned)gx, (unsigned)T), dim3(kBlock), 0, s, f.pnodes.p, f.tree_start.p, X,
                        N, bits.p, words, label.p, (const float*)nullptr, f.tree_offset, f.seed, perm, counts.p, sse.p);
-  else
+  } else {
     hipLaunchKernelGGL(k_oob<1>, dim3((unsigned)gx, (unsigned)T), dim3(kBlock), 0, s, f.pnodes.p, f.tree_start.p, X,
                        N, bits.p, words, (const uint32_t*)nullptr, yv.p, f.tree_offset, f.seed, perm, counts.p, sse.p);
+  }
   DTE_HIP(hipGetLastError());
   unsigned long long c[2];
   double e = 0.0;
@@ -250,11 +354,10 @@ void predict_forest(const Forest& f, const float* X, int64_t n_rows, int64_t ld,
     // Rows staged in LDS when a block's rows and votes fit 48 KB: 64 rows
     // while F + C <= 191, 32 rows while F + C <= 383.
     const int F = f.n_features, C = f.n_classes;
-    const int R = 4 * 64 * (F + 1 + C) <= 48 * 1024 ? 64 : (4 * 32 * (F + 1 + C) <= 48 * 1024 ? 32 : 0);
+    const int R = staged_rows(F, C);
     if (R) {
       const int64_t row_blocks = (n_rows + R - 1) / R;
-      int chunks = 1;
-      while (row_blocks * chunks < 2048 && chunks * 32 < f.n_trees) chunks *= 2;
+      const int chunks = tree_chunks(row_blocks, f.n_trees);
       const int tpb = (f.n_trees + chunks - 1) / chunks;
       const dim3 grid((unsigned)row_blocks, (unsigned)((f.n_trees + tpb - 1) / tpb));
       const size_t lds = sizeof(uint32_t) * (size_t)R * (C + F + 1);
@@ -269,9 +372,7 @@ void predict_forest(const Forest& f, const float* X, int64_t n_rows, int64_t ld,
       return;
     }
     const int64_t row_blocks = (n_rows + kRowsPerBlock - 1) / kRowsPerBlock;
-    // Enough tree chunks to fill the chip even for small requests.
-    int chunks = 1;
-    while (row_blocks * chunks < 2048 && chunks * 32 < f.n_trees) chunks *= 2;
+    const int chunks = tree_chunks(row_blocks, f.n_trees);
     const int tpb = (f.n_trees + chunks - 1) / chunks;
     const dim3 grid((unsigned)row_blocks, (unsigned)((f.n_trees + tpb - 1) / tpb));
     Timer tm("predict", 0.0, s);
