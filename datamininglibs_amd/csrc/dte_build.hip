@@ -1102,9 +1102,10 @@ __device__ __forceinline__ int64_t v2_team_item(const uint32_t* list, int64_t n_
   return k < n_list ? (int64_t)list[k] : -1;
 }
 
-// Items of at most kV2FewB buckets take the scatter variant with small
-// bucket tables (less LDS: three workgroups per CU instead of two).
+// Items of at most kV2FewB (kV2MidB) buckets take the scatter variants with
+// small bucket tables (less LDS: three workgroups per CU instead of two).
 constexpr int kV2FewB = 512;
+constexpr int kV2MidB = 1024;
 
 template <int TASK, int MAXB>
 __global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2Args a) {
@@ -3279,7 +3280,7 @@ struct BigState {
   DevBuf<unsigned int> ctr;
   DevBuf<uint32_t> lists;  // item indices: small items (wave teams), then the rest
   std::vector<V2Item> host_items;
-  std::vector<V2Tile> host_htiles, host_stiles, host_ctiles, host_stiles_many;
+  std::vector<V2Tile> host_htiles, host_stiles, host_ctiles, host_stiles_mid, host_stiles_many;
   std::vector<uint32_t> host_lists;
   std::vector<int64_t> host_fcount;   // histogram tiles: items per feature
   std::vector<uint32_t> host_forder;  // batch items in feature order
@@ -3310,6 +3311,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     int64_t k1 = k0, el = 0, nb = 0;
     B.host_htiles.clear();
     B.host_stiles.clear();
+    B.host_stiles_mid.clear();
     B.host_stiles_many.clear();
     B.host_ctiles.clear();
     B.host_lists.clear();
@@ -3318,7 +3320,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
       bi.boff = (int32_t)nb;
       bi.soff = el;
       const uint32_t li = (uint32_t)(k1 - k0);
-      auto& st = bi.nb <= kV2FewB ? B.host_stiles : B.host_stiles_many;
+      auto& st = bi.nb <= kV2FewB ? B.host_stiles : (bi.nb <= kV2MidB ? B.host_stiles_mid : B.host_stiles_many);
       for (int e0 = 0; e0 < bi.m; e0 += kSTile) st.push_back(V2Tile{li, (uint32_t)e0});
       for (int b0 = 0; b0 < bi.nb; b0 += kV2CTile) B.host_ctiles.push_back(V2Tile{li, (uint32_t)b0});
       el += bi.m;
@@ -3353,6 +3355,8 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     }
     // scatter tiles of items with few buckets first, then the rest
     const int64_t n_sfew = (int64_t)B.host_stiles.size();
+    B.host_stiles.insert(B.host_stiles.end(), B.host_stiles_mid.begin(), B.host_stiles_mid.end());
+    const int64_t n_smid = (int64_t)B.host_stiles.size();
     B.host_stiles.insert(B.host_stiles.end(), B.host_stiles_many.begin(), B.host_stiles_many.end());
     const int64_t n_sall = (int64_t)B.host_stiles.size();
     for (int64_t k = k0; k < k1; ++k)
@@ -3455,8 +3459,12 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
       if (n_sfew)
         hipLaunchKernelGGL((k_v2_scatter<TASK, kV2FewB>), dim3((unsigned)n_sfew), dim3(kBlock), 0, s, d, lv, a);
       a.tiles = B.stiles.p + n_sfew;
-      if (n_sall > n_sfew)
-        hipLaunchKernelGGL((k_v2_scatter<TASK, kV2MaxB>), dim3((unsigned)(n_sall - n_sfew)), dim3(kBlock), 0, s, d,
+      if (n_smid > n_sfew)
+        hipLaunchKernelGGL((k_v2_scatter<TASK, kV2MidB>), dim3((unsigned)(n_smid - n_sfew)), dim3(kBlock), 0, s, d,
+                           lv, a);
+      a.tiles = B.stiles.p + n_smid;
+      if (n_sall > n_smid)
+        hipLaunchKernelGGL((k_v2_scatter<TASK, kV2MaxB>), dim3((unsigned)(n_sall - n_smid)), dim3(kBlock), 0, s, d,
                            lv, a);
     }
     a.tiles = B.ctiles.p;

@@ -7,7 +7,7 @@
 //  * classification: every (row, tree) pair is walked exactly once (FIX of
 //    the truncated grid, gpudte_algorithm.cc:297-305) and votes are integer
 //    atomics, so the result is order-free;
-//  * regression: one thread per row sums its trees in tree order (fixed fp32
+//  * regression: every row's trees are summed in tree order (fixed fp32
 //    order instead of float atomics, gpudte.cu:354-357).
 #include <hip/hip_runtime.h>
 
@@ -126,6 +126,54 @@ __global__ void __launch_bounds__(kBlock) k_predict_reg(const PNode* __restrict_
     }
     sums[row] = acc;
   }
+}
+
+// Regression with a block's R rows staged in LDS.  Trees go in chunks of
+// kRegChunk: the 256 / R tree lanes of a row write the chunk's leaf values
+// to LDS, then one lane per row adds them in tree order, so every row's sum
+// is formed in the same order as the one-thread-per-row kernel.
+constexpr int kRegChunk = 32;
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_predict_reg_lds(const PNode* __restrict__ nodes,
+                                                            const int64_t* __restrict__ tree_start, int T,
+                                                            const float* __restrict__ X, int64_t n_rows, int64_t ld,
+                                                            int row_major, int F, float* __restrict__ sums) {
+  extern __shared__ float s_mem_f[];
+  float* s_val = s_mem_f;              // R x kRegChunk leaf values
+  float* s_x = s_mem_f + R * kRegChunk;  // R rows of F + 1 floats
+  const int FP = F + 1;
+  const int lane = threadIdx.x % R;
+  const int group = threadIdx.x / R;
+  const int64_t row0 = (int64_t)blockIdx.x * R;
+  const int nr = (int)min<int64_t>(R, n_rows - row0);
+  if (row_major) {
+    for (int i = threadIdx.x; i < nr * F; i += kBlock) {
+      const int r = i / F, f = i - r * F;
+      s_x[r * FP + f] = X[(row0 + r) * ld + f];
+    }
+  } else {
+    for (int i = threadIdx.x; i < R * F; i += kBlock) {
+      const int f = i / R, r = i - f * R;
+      if (r < nr) s_x[r * FP + f] = X[(int64_t)f * ld + row0 + r];
+    }
+  }
+  __syncthreads();
+  const float* xr = s_x + lane * FP;
+  float acc = 0.0f;
+  for (int c0 = 0; c0 < T; c0 += kRegChunk) {
+    const int c1 = min(T, c0 + kRegChunk);
+    if (lane < nr)
+      for (int t = c0 + group; t < c1; t += kBlock / R) {
+        PNode nd = nodes[tree_start[t]];
+        while (nd.feature >= 0) nd = nodes[nd.left + side_of(xr[nd.feature], nd.thr)];
+        s_val[lane * kRegChunk + (t - c0)] = nd.thr;
+      }
+    __syncthreads();
+    if (group == 0 && lane < nr)
+      for (int t = c0; t < c1; ++t) acc = acc + s_val[lane * kRegChunk + (t - c0)];
+    __syncthreads();
+  }
+  if (group == 0 && lane < nr) sums[row0 + lane] = acc;
 }
 
 // ---------------------------------------------------- out-of-bag pass --
@@ -379,10 +427,25 @@ void predict_forest(const Forest& f, const float* X, int64_t n_rows, int64_t ld,
     hipLaunchKernelGGL(k_predict_cls, grid, dim3(kBlock), sizeof(uint32_t) * kRowsPerBlock * f.n_classes, s,
                        f.pnodes.p, f.tree_start.p, f.n_trees, X, n_rows, ld, row_major, f.n_classes, tpb, votes);
   } else {
-    const int grid = (int)std::min<int64_t>((n_rows + kBlock - 1) / kBlock, 65535);
-    Timer tm("predict", 0.0, s);
-    hipLaunchKernelGGL(k_predict_reg, dim3(grid), dim3(kBlock), 0, s, f.pnodes.p, f.tree_start.p, f.n_trees, X,
-                       n_rows, ld, row_major, sums);
+    // Rows staged in LDS (with the chunk's leaf values) when they fit 48 KB.
+    const int F = f.n_features;
+    const int R = staged_rows(F, kRegChunk);
+    if (R) {
+      const int64_t row_blocks = (n_rows + R - 1) / R;
+      const size_t lds = sizeof(float) * (size_t)R * (kRegChunk + F + 1);
+      Timer tm("predict", 0.0, s);
+      if (R == 64)
+        hipLaunchKernelGGL(k_predict_reg_lds<64>, dim3((unsigned)row_blocks), dim3(kBlock), lds, s, f.pnodes.p,
+                           f.tree_start.p, f.n_trees, X, n_rows, ld, row_major, F, sums);
+      else
+        hipLaunchKernelGGL(k_predict_reg_lds<32>, dim3((unsigned)row_blocks), dim3(kBlock), lds, s, f.pnodes.p,
+                           f.tree_start.p, f.n_trees, X, n_rows, ld, row_major, F, sums);
+    } else {
+      const int grid = (int)std::min<int64_t>((n_rows + kBlock - 1) / kBlock, 65535);
+      Timer tm("predict", 0.0, s);
+      hipLaunchKernelGGL(k_predict_reg, dim3(grid), dim3(kBlock), 0, s, f.pnodes.p, f.tree_start.p, f.n_trees, X,
+                         n_rows, ld, row_major, sums);
+    }
   }
   DTE_HIP(hipGetLastError());
 }

@@ -111,6 +111,27 @@ def test_predict_row_staging_widths(f):
         srv.close()
 
 
+@pytest.mark.parametrize("f", [12, 250, 400])
+def test_regression_predict_row_staging_widths(f):
+    """Regression predict stages 64 rows while features + 32 <= 191, 32 rows
+    while <= 383, else reads rows from HBM; every row's sum is formed in tree
+    order in all three, so predict and serving equal the oracle bit for bit.
+    40 trees span two chunks of 32."""
+    X, y = synth.sine_regression(seed=33, n=800, f=f)
+    g, o = fit_both(X, y, algorithm=capi.RANDOM_FOREST, task=capi.REGRESSION, n_trees=40, k=6, max_depth=8,
+                    min_node_size=3)
+    Xp, _ = synth.sine_regression(seed=34, n=1000, f=f)
+    want = o.predict(Xp)
+    assert np.array_equal(g.predict(Xp), want)
+    srv = capi.Server(max_batch_rows=1024)
+    try:
+        slot = srv.load(g)
+        got = srv.wait(srv.submit(slot, np.ascontiguousarray(Xp.T)))
+        assert np.array_equal(np.asarray(got), want)
+    finally:
+        srv.close()
+
+
 @pytest.mark.parametrize("algorithm", [capi.EXTRA_TREES, capi.RANDOM_FOREST])
 def test_plain_regression(algorithm):
     X, y = synth.sine_regression(seed=4, n=3000, f=6)
