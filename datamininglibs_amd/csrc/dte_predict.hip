@@ -181,15 +181,27 @@ __global__ void __launch_bounds__(kBlock) k_predict_reg_lds(const PNode* __restr
 constexpr uint32_t kStreamImp = 0x00706d69u;
 
 // In-bag bits: bit r of tree t is set iff tree t's bootstrap drew row r
-// (the draws of the fit's bootstrap, k_boot_count).
-__global__ void k_inbag_bits(uint32_t* bits, int64_t N, int64_t words, int32_t tree0, uint64_t seed) {
-  const int t = blockIdx.y;
-  uint32_t* bt = bits + (int64_t)t * words;
+// (the draws of the fit's bootstrap, k_boot_count).  One workgroup per
+// (tree, range of kBagRange rows): every draw of the tree is generated, the
+// range's bits are set in LDS and written out once.
+constexpr int kBagThreads = 1024;
+constexpr int64_t kBagRange = (int64_t)1 << 20;  // rows per workgroup: 128 KB of bits
+__global__ void __launch_bounds__(kBagThreads) k_inbag_bits_lds(uint32_t* bits, int64_t N, int64_t words,
+                                                                int32_t tree0, uint64_t seed) {
+  __shared__ uint32_t s_bits[kBagRange / 32];
+  const int t = blockIdx.x;
+  const int64_t r0 = (int64_t)blockIdx.y * kBagRange;
+  const int64_t nw = (min(kBagRange, N - r0) + 31) / 32;  // words of this range
+  for (int64_t w = threadIdx.x; w < nw; w += kBagThreads) s_bits[w] = 0;
+  __syncthreads();
   const uint32_t tg = (uint32_t)(tree0 + t);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t r = rand_u32(seed, kStreamBoot, tg, 0, (uint64_t)i) % (uint32_t)N;
-    atomicOr(&bt[r >> 5], 1u << (r & 31u));
+  for (int64_t i = threadIdx.x; i < N; i += kBagThreads) {
+    const int64_t r = (int64_t)(rand_u32(seed, kStreamBoot, tg, 0, (uint64_t)i) % (uint32_t)N) - r0;
+    if (r >= 0 && r < kBagRange) atomicOr(&s_bits[r >> 5], 1u << (r & 31));
   }
+  __syncthreads();
+  uint32_t* bt = bits + (int64_t)t * words + r0 / 32;
+  for (int64_t w = threadIdx.x; w < nw; w += kBagThreads) bt[w] = s_bits[w];
 }
 
 // Every (tree, row) pair whose row is out of the tree's bag is routed
@@ -330,11 +342,10 @@ void oob_evaluate(const Forest& f, const float* X, const float* targets, int64_t
   const int T = f.n_trees;
   if (T > 65535) fail(DTE_ERR_INVALID_ARGUMENT, "the out-of-bag pass handles at most 65535 trees");
   const int64_t words = (N + 31) / 32;
-  DevBuf<uint32_t> bits((size_t)words * T);
-  DTE_HIP(hipMemsetAsync(bits.p, 0, bits.bytes(), s));
+  DevBuf<uint32_t> bits((size_t)words * T);  // every word is written by k_inbag_bits_lds
   const int gx = (int)std::min<int64_t>((N + kBlock - 1) / kBlock, 1024);
-  hipLaunchKernelGGL(k_inbag_bits, dim3((unsigned)gx, (unsigned)T), dim3(kBlock), 0, s, bits.p, N, words,
-                     f.tree_offset, f.seed);
+  hipLaunchKernelGGL(k_inbag_bits_lds, dim3((unsigned)T, (unsigned)((N + kBagRange - 1) / kBagRange)),
+                     dim3(kBagThreads), 0, s, bits.p, N, words, f.tree_offset, f.seed);
   DevBuf<uint32_t> label;
   DevBuf<float> yv;
   if (f.task == 0) {

@@ -272,6 +272,25 @@ def test_out_of_bag_pass_matches_oracle(task, nf):
     Xd.free()
 
 
+def test_out_of_bag_pairs_over_row_ranges():
+    """2.2 M rows: the in-bag bits are built in three ranges of 2^20 rows.
+    The out-of-bag pairs of tree t are the rows its bootstrap never drew, N
+    minus the distinct rows at its root, so the pass must count exactly
+    sum_t (N - root entries)."""
+    X, y = small_classes(14, 2_200_000, 4, 3)
+    n = X.shape[1]
+    p = capi.default_params(capi.RANDOM_FOREST, n_trees=3, n_features_per_node=2, max_depth=3, n_classes=3, seed=5)
+    Xd = capi.DeviceArray.from_numpy(X)
+    g = capi.fit_device(Xd.ptr, n, X.shape[0], y, p)
+    c = g.canonical()
+    roots = c["tree_start"][:-1]
+    want = int(sum(n - int(c["n_entries"][r]) for r in roots))
+    pairs, correct = capi.oob_evaluate(g, Xd.ptr, y, n, -1)
+    assert pairs == want and 0 <= correct <= pairs
+    g.close()
+    Xd.free()
+
+
 @pytest.mark.parametrize("task", [capi.CLASSIFICATION, capi.REGRESSION])
 def test_rf_roots_above_the_bucket_table(task):
     """Roots of 2M rows hold more than 4096 x 384 entries: every bucket of the
