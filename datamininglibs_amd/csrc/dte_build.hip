@@ -1102,15 +1102,13 @@ __device__ __forceinline__ int64_t v2_team_item(const uint32_t* list, int64_t n_
   return k < n_list ? (int64_t)list[k] : -1;
 }
 
-// Items of at most kV2FewB (kV2MidB) buckets take the scatter variants with
-// small bucket tables (less LDS: three workgroups per CU instead of two).
-constexpr int kV2FewB = 512;
-constexpr int kV2MidB = 1024;
-
-template <int TASK, int MAXB>
+// Tiles are issued feature-major and split in eighths across the XCDs (see
+// big_pipeline): a larger grid of smaller bucket tables was slower, since
+// more tiles in flight gather from more columns of K at once.
+template <int TASK>
 __global__ void __launch_bounds__(kBlock) k_v2_hist(DataArgs d, LevelArgs lv, V2Args a) {
-  __shared__ uint32_t s_cnt[MAXB];
-  __shared__ uint32_t s_max[MAXB];
+  __shared__ uint32_t s_cnt[kV2MaxB];
+  __shared__ uint32_t s_max[kV2MaxB];
   const V2Tile tl = a.tiles[blockIdx.x];
   const V2Item it = a.items[tl.item];
   const int t = threadIdx.x;
@@ -1213,6 +1211,11 @@ __global__ void __launch_bounds__(kBlock) k_v2_scan(V2Args a, const uint32_t* li
 // bucket from an LDS counter, the tile reserves one run per bucket in the
 // item's scratch, and the entries are staged in LDS in (bucket, rank) order
 // so that consecutive threads write consecutive slots of a run.
+// Items of at most kV2FewB (kV2MidB) buckets take the scatter variants with
+// small bucket tables (less LDS: three workgroups per CU instead of two).
+constexpr int kV2FewB = 512;
+constexpr int kV2MidB = 1024;
+
 template <int TASK, int MAXB>
 __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv, V2Args a) {
   constexpr int TILE = TASK == 0 ? kScatterTile : 2048;
@@ -3338,7 +3341,8 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     B.host_forder.resize((size_t)n_items);
     for (int64_t k = k0; k < k1; ++k) B.host_forder[B.host_fcount[(size_t)B.host_items[k].f]++] = (uint32_t)(k - k0);
     for (const uint32_t li : B.host_forder)
-      for (int e0 = 0; e0 < B.host_items[k0 + li].m; e0 += kV2Tile) B.host_forder_tiles.push_back(V2Tile{li, (uint32_t)e0});
+      for (int e0 = 0; e0 < B.host_items[k0 + li].m; e0 += kV2Tile)
+        B.host_forder_tiles.push_back(V2Tile{li, (uint32_t)e0});
     {
       // Workgroups go to the 8 XCDs round robin (workgroup w to XCD w % 8),
       // each XCD with its own L2: XCD x takes the x-th eighth of the
@@ -3438,11 +3442,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     a.tiles = B.htiles.p;
     {
       Timer tm("chunk_hist", 0.0, s);
-      // one launch in feature order: a smaller bucket table (more
-      // workgroups in flight) spreads the gathers over more columns and is
-      // slower
-      hipLaunchKernelGGL((k_v2_hist<TASK, kV2MaxB>), dim3((unsigned)B.host_htiles.size()), dim3(kBlock), 0, s, d, lv,
-                         a);
+      hipLaunchKernelGGL(k_v2_hist<TASK>, dim3((unsigned)B.host_htiles.size()), dim3(kBlock), 0, s, d, lv, a);
     }
     {
       Timer tm("chunk_scan", 0.0, s);
