@@ -2968,14 +2968,12 @@ __global__ void k_boot_compact(const uint32_t* mult, const uint32_t* pos, int64_
                                const int64_t* yfix, int C, uint32_t* row, uint32_t* aux, int64_t* ey,
                                uint32_t* root_cls, int64_t* root_sum, uint64_t* root_hash,
                                uint32_t* overflow) {
-  // class counts: one LDS column per thread (no shared counters to contend
-  // for), summed per class at the end
-  __shared__ uint32_t s_cls[32][kBlock];
+  __shared__ uint32_t s_cls[32];
   const int tr = blockIdx.y;
   const uint32_t* mt = mult + (int64_t)tr * N;
   const uint32_t* pt = pos + (int64_t)tr * N;
-  if (!yfix)
-    for (int c = 0; c < C; ++c) s_cls[c][threadIdx.x] = 0;
+  if (threadIdx.x < 32) s_cls[threadIdx.x] = 0;
+  __syncthreads();
   uint64_t hsh = 0;
   int64_t sm = 0;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
@@ -2990,16 +2988,13 @@ __global__ void k_boot_compact(const uint32_t* mult, const uint32_t* pos, int64_
       ey[p] = yfix[r];
       sm += (int64_t)w * yfix[r];
     } else {
-      s_cls[lab][threadIdx.x] += w;
+      atomicAdd(&s_cls[lab], w);
     }
     hsh += (uint64_t)w * mix64((uint64_t)r);
   }
   __syncthreads();
-  if (!yfix && (int)threadIdx.x < C) {
-    uint32_t tot = 0;
-    for (int i = 0; i < kBlock; ++i) tot += s_cls[threadIdx.x][(threadIdx.x + i) % kBlock];
-    if (tot) atomicAdd(&root_cls[(int64_t)tr * C + threadIdx.x], tot);
-  }
+  if (!yfix && (int)threadIdx.x < C && s_cls[threadIdx.x])
+    atomicAdd(&root_cls[(int64_t)tr * C + threadIdx.x], s_cls[threadIdx.x]);
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     hsh += __shfl_xor(hsh, off, kWave);
