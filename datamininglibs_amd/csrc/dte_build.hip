@@ -2050,7 +2050,7 @@ __device__ __forceinline__ BestCand seg_best(BestCand b) {
 }
 
 template <int MAXC, int TASK, int SEG>
-__global__ void __launch_bounds__(kBlock, SEG == 64 ? 3 : 1) k_search_rf_seg(DataArgs d, LevelArgs lv, ItemArgs it) {
+__global__ void __launch_bounds__(kBlock, SEG == 64 ? 4 : 1) k_search_rf_seg(DataArgs d, LevelArgs lv, ItemArgs it) {
   constexpr int NPW = kWave / SEG;  // nodes per wave
   constexpr int kTL = 2048;         // LDS copy of T's head
   __shared__ double s_inv[128], s_logc[128];
