@@ -652,10 +652,10 @@ int oracle_forest_recompute_scores(const oracle_forest* f, const float* X, const
   int64_t n = f->n_nodes;
   int C = f->C;
   for (int64_t i = 0; i < n; ++i) out[i] = 0.0;
-  /* Membership is recovered from the stored row hash path: re-run the
-   * bootstrap is not possible here without the sample, so the recompute uses
-   * the stored per-node class counts / sums, which the parity tests pin
-   * separately against the GPU; the score formula is the independent part. */
+  /* The recompute uses the stored per-node class counts / sums (the node
+   * membership they come from is re-derived from the raw inputs by
+   * tests/test_oracle_split_choice.py); the score formula is the
+   * independent part here. */
   for (int64_t i = 0; i < n; ++i) {
     if (a.feature[i] < 0) continue;
     int64_t l = a.left[i], r = l + 1;

@@ -14,7 +14,9 @@
  *
  * Pinning: Philox4x32-10 known-answer vectors (Random123), closed-form
  * entropy / variance cases, brute-force best split on tiny nodes, the fp64
- * libm recompute of every chosen split score, whole-forest invariants, the
+ * libm recompute of every chosen split score, whole-forest invariants, an
+ * independent numpy re-derivation of every node's split choice (feature
+ * draws, retry rule, ERT threshold, empty-side score, tie rule), the
  * reference's own CSV fixtures (toy parity CSV, spambase).  See tests/.
  */
 #pragma once
@@ -58,8 +60,11 @@ void oracle_forest_get(const oracle_forest* f, int64_t* tree_start, int32_t* fea
                        uint32_t* class_counts /* n_nodes * C */,
                        int64_t* sum_fixed /* n_nodes */, float* value /* n_nodes */);
 
-/* fp64 recompute of each internal node's chosen split score with libm log,
- * straight from the float inputs (no fixed point, no x ln x table). */
+/* fp64 recompute of each internal node's chosen split score with libm log
+ * (no fixed point, no x ln x table), from the stored class counts / target
+ * sums of the node and its two children.  X and y are not read: the
+ * recompute checks the score formula, not the node membership (which
+ * tests/test_oracle_split_choice.py re-derives from the raw inputs). */
 int oracle_forest_recompute_scores(const oracle_forest* f, const float* X, const float* y,
                                    int64_t n_rows, int32_t n_features, double* out);
 
