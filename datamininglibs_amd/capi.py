@@ -96,6 +96,12 @@ class ServerStats(ctypes.Structure):
         ("requests_rejected", ctypes.c_int64),
         ("busy_ms", ctypes.c_double),
         ("hbm_in_use", ctypes.c_int64),
+        ("stage_ms", ctypes.c_double),
+        ("h2d_ms", ctypes.c_double),
+        ("kernel_ms", ctypes.c_double),
+        ("d2h_ms", ctypes.c_double),
+        ("scatter_ms", ctypes.c_double),
+        ("groups_launched", ctypes.c_int64),
     ]
 
 
@@ -229,6 +235,8 @@ class Forest:
     def predict(self, X):
         """X: (F, n) feature-major float32.  Votes (n, C) or sums (n,)."""
         X = np.ascontiguousarray(X, np.float32)
+        if X.ndim != 2 or X.shape[0] != self.n_features:
+            raise ValueError(f"X must be (n_features={self.n_features}, n_rows), got {X.shape}")
         n = X.shape[1]
         if self.task == CLASSIFICATION:
             out = np.zeros((n, self.n_classes), np.float32)
@@ -404,6 +412,9 @@ class Server:
     def submit(self, slot, rows):
         """rows: (n, F) row-major float32.  Returns a ticket."""
         rows = np.ascontiguousarray(rows, np.float32)
+        # an unknown slot is the library's to report (DTE_ERR_NOT_FOUND)
+        if slot in self._meta and (rows.ndim != 2 or rows.shape[1] != self._meta[slot][0]):
+            raise ValueError(f"rows must be (n_rows, n_features={self._meta[slot][0]}), got {rows.shape}")
         t = ctypes.c_uint64()
         _check(lib().dte_server_submit(self._h, slot, _ptr(rows), rows.shape[0], ctypes.byref(t)))
         return t.value, rows.shape[0], slot

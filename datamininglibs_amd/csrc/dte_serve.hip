@@ -8,13 +8,15 @@
 // buffers sized at load time under the server's HBM budget:
 //
 //   submit   -> the request (row-major rows, copied) joins the pending FIFO;
-//   schedule -> whenever a slot is free, the pending requests of the head
-//               request's forest are packed into it (a request larger than
-//               the slot is split across batches), rows staged, copied,
-//               traversed and the votes copied back on the slot's stream;
-//   finish   -> when the slot's event completes, every request's rows are
-//               scattered into its result; a request whose last rows landed
-//               is done.
+//   schedule -> whenever a slot is free, pending requests are packed into it
+//               in arrival order, whatever forest they are for (a request
+//               larger than the slot is split across batches); the batch's
+//               rows are grouped by forest, staged, copied in one transfer,
+//               traversed by one launch per forest and the results copied
+//               back in one transfer on the slot's stream;
+//   finish   -> when the slot's last event completes, every request's rows
+//               are scattered into its result; a request whose last rows
+//               landed is done.
 //
 // Requests that arrive while batches are on the GPU are admitted into the
 // next free slot, so the copies and traversals of consecutive batches
@@ -23,6 +25,10 @@
 // bounded, a submit beyond it is refused with DTE_ERR_ADMISSION.  Each row's
 // result depends only on its own features and the forest, so a result is
 // the same whatever batch it rides in.
+//
+// A batch records four events (start, rows copied, traversals done, results
+// copied) and the scheduler times its host staging and scatter, so the
+// server reports where a batch's time goes (dte_server_stats).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,8 +56,9 @@ struct Resident {
 
 struct Request {
   uint64_t ticket = 0;
-  int slot = 0;
+  const Resident* forest = nullptr;  // heap object: never moves while the server lives
   int64_t n_rows = 0;
+  int n_features = 0;
   int width = 1;  // result floats per row
   std::vector<float> rows;
   std::vector<float> out;
@@ -64,21 +71,30 @@ struct Piece {
   Request* req;
   int64_t row0;  // first row of the request
   int64_t n;
-  int64_t off;   // first row in the batch
+  int64_t off;   // first row in the batch (rows of one forest are contiguous)
+};
+
+// Rows of one forest inside a batch: rows [row0, row0 + n) of the batch,
+// whose features start at x_off floats and whose results start at o_off
+// result words of the slot buffers.
+struct Group {
+  const Resident* forest;
+  int64_t row0, n, x_off, o_off;
 };
 
 struct Slot {
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev_in = nullptr, ev_k = nullptr, ev1 = nullptr;
   float* h_rows = nullptr;   // pinned
-  void* h_out = nullptr;     // pinned, int32 votes or fp32 sums
+  void* h_out = nullptr;     // pinned, int32 votes or fp32 sums per group
   DevBuf<float> d_rows;
-  DevBuf<int32_t> d_votes;
-  DevBuf<float> d_sums;
+  DevBuf<int32_t> d_out;     // int32 votes or fp32 sums (same 4-byte words)
   bool busy = false;
-  int forest = -1;
   int64_t n = 0;
+  int64_t x_floats = 0, o_words = 0;
+  double stage_ms = 0.0;
   std::vector<Piece> pieces;
+  std::vector<Group> groups;
 };
 
 }  // namespace
@@ -110,6 +126,12 @@ struct dte_server {
 namespace dte {
 namespace {
 
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
 int64_t forest_bytes(const Forest& f) { return (int64_t)(f.pnodes.bytes() + f.tree_start.bytes()); }
 
 void free_slot_buffers(dte_server& s) {
@@ -119,25 +141,23 @@ void free_slot_buffers(dte_server& s) {
     sl.h_rows = nullptr;
     sl.h_out = nullptr;
     sl.d_rows.release();
-    sl.d_votes.release();
-    sl.d_sums.release();
+    sl.d_out.release();
   }
   s.st.hbm_in_use -= s.slot_bytes;
   s.slot_bytes = 0;
 }
 
-// (Re)sizes every slot for f features and c result floats per row.
+// (Re)sizes every slot for f features and c result words per row.
 void size_slots(dte_server& s, int f, int c) {
-  const int64_t need = (int64_t)kSlots * s.max_rows * ((int64_t)f * 4 + (int64_t)c * 4 + 4);
+  const int64_t need = (int64_t)kSlots * s.max_rows * ((int64_t)f * 4 + (int64_t)c * 4);
   const int64_t others = s.st.hbm_in_use - s.slot_bytes;
   if (others + need > s.budget) fail(DTE_ERR_ADMISSION, "slot buffers do not fit the server's HBM budget");
   free_slot_buffers(s);
   for (auto& sl : s.slots) {
     sl.d_rows.alloc((size_t)s.max_rows * f);
-    sl.d_votes.alloc((size_t)s.max_rows * c);
-    sl.d_sums.alloc((size_t)s.max_rows);
+    sl.d_out.alloc((size_t)s.max_rows * c);
     DTE_HIP(hipHostMalloc((void**)&sl.h_rows, sizeof(float) * s.max_rows * f, hipHostMallocDefault));
-    DTE_HIP(hipHostMalloc(&sl.h_out, sizeof(float) * s.max_rows * c, hipHostMallocDefault));
+    DTE_HIP(hipHostMalloc(&sl.h_out, sizeof(int32_t) * s.max_rows * c, hipHostMallocDefault));
   }
   s.f_cap = f;
   s.c_cap = c;
@@ -145,67 +165,105 @@ void size_slots(dte_server& s, int f, int c) {
   s.st.hbm_in_use += need;
 }
 
-// Packs pending requests of the head request's forest into slot sl.
-// Called with the lock held; returns false when nothing was taken.
+// Packs pending requests into slot sl in arrival order, for any forest, up
+// to max_rows rows; then lays the pieces out grouped by forest.  Called with
+// the lock held; returns false when nothing was taken.
 bool take_batch(dte_server& s, Slot& sl) {
   if (s.pending.empty()) return false;
-  const int fslot = s.pending.front()->slot;
   sl.pieces.clear();
+  sl.groups.clear();
   sl.n = 0;
-  sl.forest = fslot;
-  for (auto it = s.pending.begin(); it != s.pending.end() && sl.n < s.max_rows;) {
-    Request* r = *it;
-    if (r->slot != fslot) {
-      ++it;
-      continue;
-    }
+  while (!s.pending.empty() && sl.n < s.max_rows) {
+    Request* r = s.pending.front();
     const int64_t take = std::min(r->n_rows - r->next_row, s.max_rows - sl.n);
-    sl.pieces.push_back(Piece{r, r->next_row, take, sl.n});
+    sl.pieces.push_back(Piece{r, r->next_row, take, 0});
     r->next_row += take;
     sl.n += take;
     s.pending_rows -= take;
-    if (r->next_row == r->n_rows)
-      it = s.pending.erase(it);
-    else
-      ++it;
+    if (r->next_row == r->n_rows) s.pending.pop_front();
   }
+  // Group by forest, in the order of each forest's first piece; the pieces
+  // of a forest keep their order.  Every forest's F <= f_cap and width <=
+  // c_cap, so the groups fit the slot buffers.
+  std::vector<const Resident*> order;
+  for (const Piece& p : sl.pieces)
+    if (std::find(order.begin(), order.end(), p.req->forest) == order.end()) order.push_back(p.req->forest);
+  auto rank = [&](const Piece& p) { return std::find(order.begin(), order.end(), p.req->forest) - order.begin(); };
+  std::stable_sort(sl.pieces.begin(), sl.pieces.end(), [&](const Piece& a, const Piece& b) { return rank(a) < rank(b); });
+  int64_t row = 0, x = 0, o = 0;
+  for (Piece& p : sl.pieces) {
+    const Resident* fr = p.req->forest;
+    if (sl.groups.empty() || sl.groups.back().forest != fr) sl.groups.push_back(Group{fr, row, 0, x, o});
+    Group& g = sl.groups.back();
+    p.off = row;
+    g.n += p.n;
+    row += p.n;
+    x += p.n * p.req->n_features;
+    o += p.n * p.req->width;
+  }
+  sl.x_floats = x;
+  sl.o_words = o;
   return sl.n > 0;
 }
 
 // Stages and launches the batch in sl (lock not held: only the scheduler
-// touches a busy slot, and requests with pieces in flight stay alive).
-void launch_batch(dte_server& s, Slot& sl) {
-  const Forest& f = s.forests[sl.forest]->f;
-  const int F = f.n_features;
-  for (const Piece& p : sl.pieces)
-    std::memcpy(sl.h_rows + p.off * F, p.req->rows.data() + p.row0 * F, sizeof(float) * p.n * F);
-  DTE_HIP(hipEventRecord(sl.ev0, sl.stream));
-  DTE_HIP(hipMemcpyAsync(sl.d_rows.p, sl.h_rows, sizeof(float) * sl.n * F, hipMemcpyHostToDevice, sl.stream));
-  if (f.task == 0) {
-    predict_forest(f, sl.d_rows.p, sl.n, F, 1, sl.d_votes.p, nullptr, sl.stream);
-    DTE_HIP(hipMemcpyAsync(sl.h_out, sl.d_votes.p, sizeof(int32_t) * sl.n * f.n_classes, hipMemcpyDeviceToHost,
-                           sl.stream));
-  } else {
-    predict_forest(f, sl.d_rows.p, sl.n, F, 1, nullptr, sl.d_sums.p, sl.stream);
-    DTE_HIP(hipMemcpyAsync(sl.h_out, sl.d_sums.p, sizeof(float) * sl.n, hipMemcpyDeviceToHost, sl.stream));
+// touches a busy slot, requests with pieces in flight stay alive and
+// resident forests are never freed while the server runs).
+void launch_batch(Slot& sl) {
+  const Clock::time_point t0 = Clock::now();
+  for (const Group& g : sl.groups) {
+    const int F = g.forest->f.n_features;
+    for (const Piece& p : sl.pieces) {
+      if (p.req->forest != g.forest) continue;
+      std::memcpy(sl.h_rows + g.x_off + (p.off - g.row0) * F, p.req->rows.data() + p.row0 * F,
+                  sizeof(float) * p.n * F);
+    }
   }
+  sl.stage_ms = ms_since(t0);
+  DTE_HIP(hipEventRecord(sl.ev0, sl.stream));
+  DTE_HIP(hipMemcpyAsync(sl.d_rows.p, sl.h_rows, sizeof(float) * sl.x_floats, hipMemcpyHostToDevice, sl.stream));
+  DTE_HIP(hipEventRecord(sl.ev_in, sl.stream));
+  for (const Group& g : sl.groups) {
+    const Forest& f = g.forest->f;
+    const float* x = sl.d_rows.p + g.x_off;
+    if (f.task == 0)
+      predict_forest(f, x, g.n, f.n_features, 1, sl.d_out.p + g.o_off, nullptr, sl.stream);
+    else
+      predict_forest(f, x, g.n, f.n_features, 1, nullptr, reinterpret_cast<float*>(sl.d_out.p + g.o_off),
+                     sl.stream);
+  }
+  DTE_HIP(hipEventRecord(sl.ev_k, sl.stream));
+  DTE_HIP(hipMemcpyAsync(sl.h_out, sl.d_out.p, sizeof(int32_t) * sl.o_words, hipMemcpyDeviceToHost, sl.stream));
   DTE_HIP(hipEventRecord(sl.ev1, sl.stream));
 }
 
 // Scatters a finished batch into its requests (lock held).
 void finish_batch(dte_server& s, Slot& sl) {
-  const Forest& f = s.forests[sl.forest]->f;
-  float ms = 0.f;
-  if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) s.st.busy_ms += ms;
+  const Clock::time_point t0 = Clock::now();
+  float a = 0.f, b = 0.f, c = 0.f;
+  if (hipEventElapsedTime(&a, sl.ev0, sl.ev_in) == hipSuccess &&
+      hipEventElapsedTime(&b, sl.ev_in, sl.ev_k) == hipSuccess &&
+      hipEventElapsedTime(&c, sl.ev_k, sl.ev1) == hipSuccess) {
+    s.st.h2d_ms += a;
+    s.st.kernel_ms += b;
+    s.st.d2h_ms += c;
+    s.st.busy_ms += (double)a + b + c;
+  }
+  s.st.stage_ms += sl.stage_ms;
+  size_t gi = 0;
   for (const Piece& p : sl.pieces) {
+    while (p.off >= sl.groups[gi].row0 + sl.groups[gi].n) ++gi;
+    const Group& g = sl.groups[gi];
     Request* r = p.req;
+    const Forest& f = g.forest->f;
     if (f.task == 0) {
       const int C = f.n_classes;
-      const int32_t* v = static_cast<const int32_t*>(sl.h_out) + p.off * C;
+      const int32_t* v = static_cast<const int32_t*>(sl.h_out) + g.o_off + (p.off - g.row0) * C;
       float* o = r->out.data() + p.row0 * C;
       for (int64_t i = 0; i < p.n * C; ++i) o[i] = (float)v[i];
     } else {
-      std::memcpy(r->out.data() + p.row0, static_cast<const float*>(sl.h_out) + p.off, sizeof(float) * p.n);
+      std::memcpy(r->out.data() + p.row0, static_cast<const float*>(sl.h_out) + g.o_off + (p.off - g.row0),
+                  sizeof(float) * p.n);
     }
     r->rows_done += p.n;
     if (r->rows_done == r->n_rows) {
@@ -215,8 +273,11 @@ void finish_batch(dte_server& s, Slot& sl) {
       s.st.rows_done += r->n_rows;
     }
   }
+  s.st.groups_launched += (int64_t)sl.groups.size();
   sl.pieces.clear();
+  sl.groups.clear();
   sl.busy = false;
+  s.st.scatter_ms += ms_since(t0);
 }
 
 void scheduler(dte_server* s) {
@@ -249,7 +310,7 @@ void scheduler(dte_server* s) {
         lk.unlock();
         std::string err;
         try {
-          launch_batch(*s, sl);
+          launch_batch(sl);
         } catch (const std::exception& e) {
           err = e.what();
         }
@@ -307,6 +368,8 @@ int dte_server_create(int32_t device_id, int64_t hbm_budget_bytes, int32_t max_b
     for (auto& sl : s->slots) {
       DTE_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
       DTE_HIP(hipEventCreate(&sl.ev0));
+      DTE_HIP(hipEventCreate(&sl.ev_in));
+      DTE_HIP(hipEventCreate(&sl.ev_k));
       DTE_HIP(hipEventCreate(&sl.ev1));
     }
     s->th = std::thread(scheduler, s.get());
@@ -382,14 +445,15 @@ int dte_server_submit(dte_server* s, int32_t slot, const float* rows, int64_t n_
       s->st.requests_rejected += 1;
       fail(DTE_ERR_ADMISSION, "request backlog full");
     }
-    const Forest& f = s->forests[slot]->f;
+    const Resident* fr = s->forests[slot].get();
     auto r = std::make_unique<Request>();
     r->ticket = s->next_ticket++;
-    r->slot = slot;
+    r->forest = fr;
     r->n_rows = n_rows;
-    r->width = f.task == 0 ? f.n_classes : 1;
+    r->n_features = fr->f.n_features;
+    r->width = fr->f.task == 0 ? fr->f.n_classes : 1;
     lk.unlock();
-    r->rows.assign(rows, rows + n_rows * f.n_features);
+    r->rows.assign(rows, rows + n_rows * r->n_features);
     r->out.assign((size_t)n_rows * r->width, 0.0f);
     lk.lock();
     Request* raw = r.get();
@@ -420,8 +484,17 @@ int dte_server_wait(dte_server* s, uint64_t ticket, float* out) {
   return run_guarded([&] {
     if (!s) fail(DTE_ERR_INVALID_ARGUMENT, "NULL argument");
     std::unique_lock<std::mutex> lk(s->mu);
-    Request* r = find(s, ticket);
-    s->cv_done.wait(lk, [&] { return r->done || !s->fatal.empty(); });
+    (void)find(s, ticket);
+    // The request is looked up again after every wake-up: another caller
+    // that polled or waited on the same ticket may have taken the result
+    // (and freed the request) in the meantime.
+    Request* r = nullptr;
+    s->cv_done.wait(lk, [&] {
+      auto it = s->requests.find(ticket);
+      r = it == s->requests.end() ? nullptr : it->second.get();
+      return !r || r->done || !s->fatal.empty();
+    });
+    if (!r) fail(DTE_ERR_NOT_FOUND, "the ticket's result was taken by another caller");
     if (!r->done) fail(DTE_ERR_HIP, s->fatal);
     deliver(s, ticket, r, out);
   });
@@ -450,8 +523,8 @@ int dte_server_destroy(dte_server* s) {
     }
     free_slot_buffers(*s);
     for (auto& sl : s->slots) {
-      if (sl.ev0) (void)hipEventDestroy(sl.ev0);
-      if (sl.ev1) (void)hipEventDestroy(sl.ev1);
+      for (hipEvent_t e : {sl.ev0, sl.ev_in, sl.ev_k, sl.ev1})
+        if (e) (void)hipEventDestroy(e);
       if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     delete s;

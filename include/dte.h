@@ -209,7 +209,9 @@ int dte_forest_free(dte_forest* forest);
 
 /* Predict.  Replaces GpuDteAlgorithm<T>::Predict + kPredict
  * (gpudte_algorithm.cc:269-331, gpudte.cu:311-359).  samples: host,
- * feature-major n_features x n_rows.  out: n_rows x n_classes vote counts
+ * feature-major n_features x n_rows, where n_features is the forest's
+ * (dte_forest_info): n_features * n_rows floats are read.  out: n_rows x
+ * n_classes vote counts
  * (classification) or n_rows sums over trees in tree order (regression;
  * mean = sum / n_trees), as MlResultData receives them. */
 int dte_predict(const dte_forest* forest, const float* samples, int64_t n_rows, float* out);
@@ -231,14 +233,26 @@ typedef struct dte_server_stats {
   int64_t rows_done;
   int64_t batches_launched;
   int64_t requests_rejected;
-  double busy_ms;
+  double busy_ms;          /* device time of the batches: h2d + kernel + d2h */
   int64_t hbm_in_use;
+  /* Where the batches' time went (sums over finished batches, ms): host
+   * staging of rows into pinned memory, the rows' host-to-device copy, the
+   * traversal launches, the results' device-to-host copy, and the host
+   * scatter of results into the requests. */
+  double stage_ms;
+  double h2d_ms;
+  double kernel_ms;
+  double d2h_ms;
+  double scatter_ms;
+  int64_t groups_launched; /* traversal launches: one per forest per batch */
 } dte_server_stats;
 
 int dte_server_create(int32_t device_id, int64_t hbm_budget_bytes, int32_t max_batch_rows,
                       dte_server** out);
 int dte_server_load_forest(dte_server* server, const dte_forest* forest, int32_t* slot);
-/* rows: host, row-major n_rows x n_features.  The server copies them. */
+/* rows: host, row-major n_rows x n_features, where n_features is that of
+ * the forest in the slot: the server copies n_rows * n_features floats.
+ * A batch carries requests for any of the resident forests. */
 int dte_server_submit(dte_server* server, int32_t slot, const float* rows, int64_t n_rows,
                       uint64_t* ticket);
 /* done = 1 when finished; out receives n_rows x n_classes votes or n_rows sums. */

@@ -208,3 +208,20 @@ def test_ctypes_structures_match_the_c_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for f in cls._fields_:
             assert got[(cname, f[0])] == getattr(cls, f[0]).offset, (cname, f[0])
+
+
+def test_wrappers_check_the_feature_dimension():
+    """Forest.predict and Server.submit refuse arrays whose feature extent
+    differs from the forest's before any library call (dte_predict reads
+    n_rows * n_features floats, dte_server_submit the same)."""
+    from types import SimpleNamespace
+    fake = SimpleNamespace(n_features=5, task=capi.CLASSIFICATION, n_classes=3)
+    with pytest.raises(ValueError):
+        capi.Forest.predict(fake, np.zeros((4, 10), np.float32))
+    with pytest.raises(ValueError):
+        capi.Forest.predict(fake, np.zeros(10, np.float32))
+    srv = SimpleNamespace(_meta={0: (5, 3)}, _h=None)
+    with pytest.raises(ValueError):
+        capi.Server.submit(srv, 0, np.zeros((10, 4), np.float32))
+    with pytest.raises(ValueError):
+        capi.Server.submit(srv, 0, np.zeros(5, np.float32))

@@ -153,3 +153,19 @@ def test_fit_many_concurrent_equals_alone():
         alone.close()
         fm.close()
         d.free()
+
+
+def test_rf_1m_x_128_trees_equal_oracle():
+    """The headline workload itself (1,000,000 x 128 N(0,1), 16 informative
+    features, 10 classes, k = 11, depth 20, min node 5, seed 324123): its
+    first two trees, fitted on the GPU, equal the oracle's bit for bit
+    (gpurf.cu:131-259, 618-768).  Draws are keyed by tree id, so these are
+    trees 0 and 1 of the 200-tree forest."""
+    X, y = synth.gaussian_classes(seed=0)
+    p = capi.default_params(capi.RANDOM_FOREST, n_trees=2, n_features_per_node=11, max_depth=20, min_node_size=5,
+                            n_classes=10, seed=324123)
+    g = capi.fit(X, y, p)
+    o = pyoracle.fit(X, y, algorithm=0, n_trees=2, k=11, max_depth=20, min_node_size=5, n_classes=10,
+                     seed=324123, n_threads=2)
+    assert_same_forest(g, o)
+    g.close()
