@@ -125,6 +125,7 @@ def lib():
         L.dte_forest_canonical.argtypes = [P] * 13
         L.dte_forest_free.argtypes = [P]
         L.dte_predict.argtypes = [P, P, ctypes.c_int64, P]
+        L.dte_predict_noalloc.argtypes = [P, P, ctypes.c_int64, P]
         L.dte_predict_device.argtypes = [P, P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, P, P, P]
         L.dte_server_create.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(P)]
         L.dte_server_load_forest.argtypes = [P, P, ctypes.POINTER(ctypes.c_int32)]
@@ -243,6 +244,17 @@ class Forest:
         else:
             out = np.zeros(n, np.float32)
         _check(lib().dte_predict(self._h, _ptr(X), n, _ptr(out)))
+        return out
+
+    def predict_noalloc(self, X):
+        """predict() through dte_predict_noalloc: no device allocation, the
+        library's fixed 64 MiB scratch holds the samples and votes."""
+        X = np.ascontiguousarray(X, np.float32)
+        if X.ndim != 2 or X.shape[0] != self.n_features:
+            raise ValueError(f"X must be (n_features={self.n_features}, n_rows), got {X.shape}")
+        n = X.shape[1]
+        out = np.zeros((n, self.n_classes) if self.task == CLASSIFICATION else n, np.float32)
+        _check(lib().dte_predict_noalloc(self._h, _ptr(X), n, _ptr(out)))
         return out
 
 

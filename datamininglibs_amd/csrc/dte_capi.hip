@@ -223,13 +223,39 @@ void profile_record(const char* name, hipEvent_t a, hipEvent_t b, double bytes) 
 
 void upload_ln_table(int device);
 
+// ------------------------------------------- no-allocation predict ----
+// Device scratch of dte_predict_noalloc: the staged samples and the votes /
+// sums of one call.  A file-scope device array: every GPU the library runs
+// on holds its own copy, at its own address, until the process ends.  In
+// host code the name `scratch` is only a stand-in for the symbol, so its
+// device address is taken once per device at initialisation
+// (hipGetSymbolAddress, with that device current) and that address is what
+// kernels and copies receive.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+namespace {
+std::map<int, void*> g_scratch;                // device -> address of its scratch (under g_dev_mu)
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu[kMaxScratchDevices];   // one call at a time per device
+}  // namespace
+
 void device_init(int device) {
   std::lock_guard<std::mutex> lk(g_dev_mu);
   if (g_streams.count(device)) return;
   upload_ln_table(device);
+  void* sp = nullptr;
+  DTE_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  g_scratch[device] = sp;
   hipStream_t s;
   DTE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   g_streams[device] = s;
+}
+
+void* noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  auto it = g_scratch.find(device);
+  if (it == g_scratch.end() || !it->second) fail(DTE_ERR_INTERNAL, "device not initialised");
+  return it->second;
 }
 
 // k-th stream of the concurrent-fit workers of a device (created on first
@@ -1000,6 +1026,41 @@ int dte_predict(const dte_forest* forest, const float* samples, int64_t n_rows, 
       DTE_HIP(hipMemcpyAsync(out, v.p, 4 * n_rows, hipMemcpyDeviceToHost, s));
       DTE_HIP(hipStreamSynchronize(s));
     }
+  });
+}
+
+int dte_predict_noalloc(const dte_forest* forest, const float* samples, int64_t n_rows, float* out) {
+  return guarded([&] {
+    if (!forest || !samples || !out) fail(DTE_ERR_INVALID_ARGUMENT, "NULL pointer argument");
+    if (n_rows < 0) fail(DTE_ERR_INVALID_ARGUMENT, "n_rows < 0");
+    const Forest& f = forest->f;
+    if (n_rows == 0) return;
+    const int64_t W = f.task == 0 ? f.n_classes : 1;
+    const int64_t F = f.n_features;
+    // samples (256-byte aligned) then results, inside the scratch
+    if (n_rows > (int64_t)(kNoAllocScratchBytes / 4)) fail(DTE_ERR_INVALID_ARGUMENT, "dte_predict_noalloc: too many rows");
+    const int64_t x_bytes = (4 * n_rows * F + 255) / 256 * 256;
+    const int64_t o_bytes = 4 * n_rows * W;
+    if (x_bytes + o_bytes > (int64_t)kNoAllocScratchBytes)
+      fail(DTE_ERR_INVALID_ARGUMENT,
+           "dte_predict_noalloc: n_rows x (n_features + result width) x 4 bytes = " + std::to_string(x_bytes + o_bytes) +
+               " exceeds its " + std::to_string(kNoAllocScratchBytes) + "-byte device scratch; use dte_predict");
+    if (f.device >= kMaxScratchDevices) fail(DTE_ERR_NO_DEVICE, "dte_predict_noalloc: device id above 63");
+    set_device(f.device);
+    char* base = static_cast<char*>(noalloc_scratch(f.device));
+    std::lock_guard<std::mutex> lk(g_scratch_mu[f.device]);
+    hipStream_t s = library_stream(f.device);
+    float* X = reinterpret_cast<float*>(base);
+    void* o = base + x_bytes;
+    DTE_HIP(hipMemcpyAsync(X, samples, 4 * n_rows * F, hipMemcpyHostToDevice, s));
+    if (f.task == 0) {
+      predict_forest(f, X, n_rows, n_rows, 0, static_cast<int32_t*>(o), nullptr, s);  // clears the votes itself
+      votes_to_float(static_cast<int32_t*>(o), n_rows * W, s);
+    } else {
+      predict_forest(f, X, n_rows, n_rows, 0, nullptr, static_cast<float*>(o), s);
+    }
+    DTE_HIP(hipMemcpyAsync(out, o, o_bytes, hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipStreamSynchronize(s));
   });
 }
 

@@ -219,6 +219,8 @@ Forest* fit_forest(const float* X_dev, const float* y_host, int64_t n_rows, int3
 void build_predict_nodes(Forest& f, hipStream_t stream);
 void predict_forest(const Forest& f, const float* X_dev, int64_t n_rows, int64_t ld,
                     int row_major, int32_t* votes_dev, float* sums_dev, hipStream_t stream);
+// In place: int32 vote counts -> the same values as float.
+void votes_to_float(int32_t* votes, int64_t n, hipStream_t stream);
 // Out-of-bag evaluation (dte_oob_evaluate): pairs evaluated, correct pairs
 // (classification) or the sum of squared errors (regression).
 void oob_evaluate(const Forest& f, const float* X_dev, const float* targets, int64_t n_rows, int perm_feature,

@@ -320,6 +320,19 @@ __global__ void __launch_bounds__(kBlock) k_oob_lds(const PNode* __restrict__ no
   }
 }
 
+__global__ void __launch_bounds__(kBlock) k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) p[i] = 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_votes_to_float(int32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const float v = (float)p[i];
+    reinterpret_cast<float*>(p)[i] = v;
+  }
+}
+
+unsigned elementwise_grid(int64_t n) { return (unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 4096); }
+
 // Rows per block of the LDS-staged kernels: 64 while the block's rows (and
 // vote counters, C) fit 48 KB, else 32, else 0 (rows read from HBM).
 int staged_rows(int F, int C) {
@@ -405,11 +418,19 @@ void oob_evaluate(const Forest& f, const float* X, const float* targets, int64_t
   out[1] = f.task == 0 ? (double)c[1] : e;
 }
 
+void votes_to_float(int32_t* votes, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_votes_to_float, dim3(elementwise_grid(n)), dim3(kBlock), 0, s, votes, n);
+  DTE_HIP(hipGetLastError());
+}
+
 void predict_forest(const Forest& f, const float* X, int64_t n_rows, int64_t ld, int row_major,
                     int32_t* votes, float* sums, hipStream_t s) {
   if (n_rows <= 0) return;
   if (f.task == 0) {
-    DTE_HIP(hipMemsetAsync(votes, 0, sizeof(int32_t) * n_rows * f.n_classes, s));
+    // votes cleared by a kernel (no memset node if the caller captures)
+    const int64_t nv = n_rows * f.n_classes;
+    hipLaunchKernelGGL(k_zero_u32, dim3(elementwise_grid(nv)), dim3(kBlock), 0, s, reinterpret_cast<uint32_t*>(votes), nv);
     // Rows staged in LDS when a block's rows and votes fit 48 KB: 64 rows
     // while F + C <= 191, 32 rows while F + C <= 383.
     const int F = f.n_features, C = f.n_classes;

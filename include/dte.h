@@ -216,6 +216,19 @@ int dte_forest_free(dte_forest* forest);
  * mean = sum / n_trees), as MlResultData receives them. */
 int dte_predict(const dte_forest* forest, const float* samples, int64_t n_rows, float* out);
 
+/* Predict without allocating: the same inputs, outputs and results as
+ * dte_predict, with the samples and votes staged in a fixed 64 MiB device
+ * scratch array that the library holds on every GPU it runs on until the
+ * process ends, so the call makes no hipMalloc / hipFree and no other
+ * allocation of device memory.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation).  Limit: n_rows x (n_features +
+ * result width) x 4 bytes (samples rounded up to 256 bytes) must be at most
+ * 64 MiB, else DTE_ERR_INVALID_ARGUMENT.  Calls on one GPU share its array:
+ * the library runs them one at a time (a lock per device) and the array
+ * keeps the last call's contents; a call clears what must start at zero. */
+int dte_predict_noalloc(const dte_forest* forest, const float* samples, int64_t n_rows, float* out);
+
 /* Predict on device buffers.  samples_dev: feature-major with leading
  * dimension ld (>= n_rows) when row_major == 0, else row-major with leading
  * dimension ld (>= n_features).  votes_dev (n_rows x n_classes int32) or
