@@ -292,6 +292,63 @@ void regression_double() {
   std::cout << "[          ] rmse = " << rmse << std::endl;
   EXPECT(rmse < 0.2);
 }
+// T = double: the device computes in fp32, so a double data frame is
+// rounded to fp32 at the boundary (the reference's ERT computes in double,
+// gpuert.cu:383-384).  Rows that differ only below fp32 resolution land
+// together: a double frame whose feature 0 holds v + 1e-12 (label 1) and
+// v - 1e-12 (label 0) for fp32 values v cannot be split on that difference.
+// The caller gets exactly the model of the fp32-rounded frame, every split
+// point is an fp32 value, and each pair of rows gets the same votes.
+void double_frame_rounds_to_fp32() {
+  const int n = 1200, F = 3;
+  col_array<double> Xd((size_t)n * F), y(n);
+  uint64_t s = 777;
+  auto u = [&] {
+    s = s * 6364136223846793005ull + 1442695040888963407ull;
+    return (double)(s >> 11) / 9007199254740992.0;
+  };
+  for (int i = 0; i < n; i += 2) {  // rows i, i + 1: a pair equal in fp32
+    const double v = (double)(float)(0.25 + 0.5 * u());  // an fp32 value
+    const double x1 = u(), x2 = u() < 0.5 ? 0.0 : 1.0;
+    for (int j = 0; j < 2; ++j) {
+      Xd[i + j] = v + (j ? 1e-12 : -1e-12);  // not fp32-representable
+      y[i + j] = j;
+      Xd[(size_t)n + i + j] = x1;
+      Xd[(size_t)2 * n + i + j] = x2;
+    }
+  }
+  for (int i = 0; i < n; ++i) EXPECT((double)(float)Xd[i] != Xd[i]);
+  col_array<float> Xf(Xd.begin(), Xd.end());
+  col_array<float> yf(y.begin(), y.end());
+  auto dd = DataLib::GetInstance().CreateDataFrame<double>();
+  dd->AddSampleData(Xd);
+  dd->AddTargetData(y);
+  dd->AddFeatureMap(col_array<col_map<string, double>>(F));
+  dd->AddTargetMap(col_map<string, double>{{"0", 0.0}, {"1", 1.0}});  // two classes
+  auto df = DataLib::GetInstance().CreateDataFrame<float>();
+  df->AddSampleData(Xf);
+  df->AddTargetData(yf);
+  df->AddFeatureMap(col_array<col_map<string, float>>(F));
+  df->AddTargetMap(col_map<string, float>{{"0", 0.0f}, {"1", 1.0f}});
+  auto params = ensembles_face.CreateGpuRfParamPack();
+  params->Set(EnsemblesLib::kNrTrees, 16);
+  params->Set(EnsemblesLib::kAlgoType, AlgorithmsLib::kClassification);
+  auto md = ensembles_face.CreateGpuRfAlgorithm<double>()->Fit(dd, params);
+  auto mf = ensembles_face.CreateGpuRfAlgorithm<float>()->Fit(df, params);
+  using NodeD = lib_ensembles::GpuDteAlgorithmShared::gpuDTE_NodeHeader_Classify<double>;
+  auto& a = md->Get<col_array<NodeD>>(ModelsLib::kNodeArray);
+  auto& b = mf->Get<col_array<Node>>(ModelsLib::kNodeArray);
+  EXPECT(a.size() == b.size());
+  for (size_t i = 0; i < a.size(); ++i) {
+    EXPECT(a[i].child_start == b[i].child_start && a[i].child_count == b[i].child_count &&
+           a[i].probability_start == b[i].probability_start && a[i].attribute == b[i].attribute);
+    EXPECT(a[i].split_point == (double)b[i].split_point);  // fp32 split points, widened
+  }
+  auto algo = ensembles_face.CreateGpuRfAlgorithm<double>();
+  auto r = algo->Predict(dd, md, params);
+  const auto& pr = r->GetPredictions();
+  for (int i = 0; i + 1 < n; i += 2) EXPECT(pr[i] == pr[i + 1]);  // each pair routed together
+}
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -316,7 +373,8 @@ int main(int argc, char** argv) {
          {"gpuert fit_predict_double_rawdata", [] { fit_predict_rawdata<double>(false, 200); }},
          {"gpurf fit_predict_float_csvdata", spambase_rf_400},
          {"parts_concatenate_to_one_forest", parts_concatenate_to_one_forest},
-         {"regression_double", regression_double}});
+         {"regression_double", regression_double},
+         {"double_frame_rounds_to_fp32", double_frame_rounds_to_fp32}});
   }
   std::cout << (g_failed ? "FAILED " : "PASSED ") << g_failed << std::endl;
   return g_failed ? 1 : 0;
