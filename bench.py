@@ -236,6 +236,8 @@ def other_workload(name, args):
                      n_threads=cores)
         dto = time.perf_counter() - t
         ss = st.get("split_search", {"total_ms": 0, "bytes": 0})
+        lg = st.get("level_graph", {"total_ms": 0, "bytes": 0})
+        ss = {"total_ms": ss["total_ms"] + lg["total_ms"], "bytes": ss["bytes"] + lg["bytes"]}
         line = {"workload": "random-forest regression 4,000,000 x 64, 100 trees, variance, k 21, depth 18",
                 "gpu_trees_per_s": round(100 / dt, 2), "gpu_plain_trees_per_s": round(100 / dtp, 2),
                 "plain_sample": "10 of the 100 trees",
@@ -423,13 +425,22 @@ def main():
             traffic = tj["hbm_bytes_per_dispatch"]
             traffic_src = os.path.relpath(TRAFFIC_FILE, ROOT) + f" (measured at tuning {tj.get('tuning')})"
     ss = stats.get("split_search", {"total_ms": 0.0, "bytes": 0.0})
+    # levels replayed from captured hipGraphs: their split-search bytes over
+    # the whole level-step time (partition and bookkeeping included)
+    lg = stats.get("level_graph", {"total_ms": 0.0, "bytes": 0.0, "launches": 0})
+    lg_replays = stats.get("level_graph_replays", {"launches": 0})["launches"]
+    ss_all_ms = ss["total_ms"] + lg["total_ms"]
+    ss_all_bytes = ss["bytes"] + lg["bytes"]
+    level_graphs = {"replays_per_fit": round(lg_replays / args.steps, 1),
+                    "ms_per_fit": round(lg["total_ms"] / args.steps, 2),
+                    "split_search_GB_per_fit": round(lg["bytes"] / args.steps / 1e9, 3)}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": KERNEL_SYMBOL.get(dom, dom), "family": dom, "launches": ds["launches"],
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": round(bytes_per_launch),
                 "traffic_source": traffic_src,
-                "split_search_all_tiers_GBps": round(ss["bytes"] / (ss["total_ms"] * 1e-3) / 1e9, 1)
-                if ss["total_ms"] > 0 else None}
+                "split_search_all_tiers_GBps": round(ss_all_bytes / (ss_all_ms * 1e-3) / 1e9, 1)
+                if ss_all_ms > 0 else None}
     serving = None
     if args.serve_requests > 0:
         serving = serve_bench(capi, synth, np, args.serve_requests)
@@ -460,6 +471,7 @@ def main():
         "predict": {"rows_per_s": round(n_pred / pred_dt, 1), "kernel_ms": round(pred_kernel_ms, 3),
                     "compulsory_GBps": round(pred_bytes / (pred_kernel_ms * 1e-3) / 1e9, 1) if pred_kernel_ms else None,
                     "train_accuracy": round(acc, 4)},
+        "level_graphs": level_graphs,
         "oob": oob,
         "serving": serving,
         "kernel_ms_per_step": {k: round(v_["total_ms"] / args.steps, 3) for k, v_ in stats.items()},

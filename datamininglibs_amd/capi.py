@@ -19,6 +19,7 @@ CLASSIFICATION = 0
 REGRESSION = 1
 BUILD_BATCHED = 0
 BUILD_PLAIN = 1
+BUILD_BATCHED_HOST = 2
 
 OK = 0
 ERR_INVALID_ARGUMENT = 1

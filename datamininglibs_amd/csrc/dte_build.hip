@@ -60,6 +60,7 @@ __device__ __forceinline__ void load_ln_table(double* s_inv, double* s_logc) {
 }
 
 // ------------------------------------------------------------- args -----
+constexpr int kMaxTiers = 7;
 struct DataArgs {
   const float* X;
   const uint32_t* K;
@@ -78,6 +79,22 @@ struct DataArgs {
   int64_t Tmax;
 };
 
+// Counts and offsets of the level being built, in device memory, for the
+// level steps that run without a host round trip (captured in a hipGraph and
+// replayed level after level): the kernels read them at entry instead of
+// taking them as launch arguments.
+struct DevLevel {
+  int64_t off;                    // level-major offset of the level
+  int64_t n;                      // nodes of the level
+  int64_t depth;
+  unsigned int n_open;            // open nodes (total of the open-flag scan)
+  unsigned int n_retry;           // open nodes that draw retries
+  unsigned int n_next;            // nodes of the next level
+  unsigned int tier[kMaxTiers];   // split-search tier sizes, first phase
+  unsigned int rtier[kMaxTiers];  // the same, retry phase
+  unsigned int ptier[kMaxTiers];  // partition tier sizes
+};
+
 struct LevelArgs {
   // level node slice (level-local index i)
   const int32_t* tree;   // batch tree index
@@ -92,6 +109,9 @@ struct LevelArgs {
   const uint32_t* row;   // current entry buffer
   const uint32_t* aux;
   const int64_t* ey;
+  // device-described level: the slice pointers above are the level-major
+  // bases and *off_dev the level's offset (NULL: already offset)
+  const int64_t* off_dev;
 };
 
 struct ItemArgs {
@@ -109,7 +129,26 @@ struct ItemArgs {
   uint32_t* scr_aux;
   int64_t* scr_y;
   int64_t scr_stride;
+  const unsigned int* n_dev;  // device-described level: n_groups = *n_dev
 };
+
+// Level slice and item count of a device-described level (a no-op for
+// launches whose arguments carry them).
+__device__ __forceinline__ void resolve_level(LevelArgs& lv, int C) {
+  if (lv.off_dev) {
+    const int64_t off = *lv.off_dev;
+    lv.tree += off;
+    lv.id += off;
+    lv.count += off;
+    lv.m += off;
+    if (lv.cls) lv.cls += off * C;
+    if (lv.sum) lv.sum += off;
+    lv.off_dev = nullptr;
+  }
+}
+__device__ __forceinline__ void resolve_items(ItemArgs& it) {
+  if (it.n_dev) it.n_groups = *it.n_dev;
+}
 
 // ------------------------------------------------- per-node context -----
 // Fixed-point x ln x with the fit-wide shift F.  Counts never exceed the
@@ -808,6 +847,9 @@ template <int MAXC, int TASK, int TEAM, bool BIG>
 __global__ void __launch_bounds__(kBlock) k_search_rf(DataArgs d, LevelArgs lv, ItemArgs it) {
   constexpr int IPT = TEAM == kWave ? kTeamIpt : kIpt;
   constexpr int NT = kBlock / TEAM;
+  resolve_level(lv, d.C);
+  resolve_items(it);
+  if ((int64_t)blockIdx.x * NT >= it.n_groups * it.per_group) return;
   __shared__ double s_inv[128], s_logc[128];
   __shared__ TeamLds<MAXC, TASK, TEAM, IPT> s_team[NT];
   __shared__ uint32_t s_boff[BIG ? kMaxBuckets + 1 : 1];
@@ -858,6 +900,9 @@ template <int MAXC, int TASK, int TEAM>
 __global__ void __launch_bounds__(kBlock) k_search_rf_team(DataArgs d, LevelArgs lv, ItemArgs it) {
   constexpr int IPT = TEAM == kWave ? kTeamIpt : kIpt;
   constexpr int NT = kBlock / TEAM;
+  resolve_level(lv, d.C);
+  resolve_items(it);
+  if ((int64_t)blockIdx.x * NT >= it.n_groups) return;  // no node for this workgroup
   // LDS copy of T's head for wave teams (still 2 workgroups per CU); the
   // workgroup teams keep their registers for occupancy instead
   constexpr int kTL = (TASK == 0 && TEAM == kWave) ? 1024 : 0;
@@ -1754,6 +1799,9 @@ __global__ void k_v2_items(DataArgs d, LevelArgs lv, const uint32_t* groups, int
 // histogram (gpuert.cu:287-301) or per-side target sums (gpuert.cu:348-361).
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_search_ert(DataArgs d, LevelArgs lv, ItemArgs it) {
+  resolve_level(lv, d.C);
+  resolve_items(it);
+  if ((int64_t)blockIdx.x >= it.n_groups * it.per_group) return;
   __shared__ double s_inv[128], s_logc[128];
   __shared__ uint32_t s_left[MAXC];
   __shared__ unsigned long long s_nl;
@@ -1894,6 +1942,9 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_search_rf_wave(DataArgs d, LevelArgs lv, ItemArgs it) {
+  resolve_level(lv, d.C);
+  resolve_items(it);
+  if ((int64_t)blockIdx.x * kWaves >= it.n_groups * it.per_group) return;
   __shared__ double s_inv[128], s_logc[128];
   load_tables(s_inv, s_logc);
   __syncthreads();
@@ -2052,6 +2103,9 @@ __device__ __forceinline__ BestCand seg_best(BestCand b) {
 template <int MAXC, int TASK, int SEG>
 __global__ void __launch_bounds__(kBlock, SEG == 64 ? 4 : 1) k_search_rf_seg(DataArgs d, LevelArgs lv, ItemArgs it) {
   constexpr int NPW = kWave / SEG;  // nodes per wave
+  resolve_level(lv, d.C);
+  resolve_items(it);
+  if ((int64_t)blockIdx.x * kWaves * NPW >= it.n_groups) return;
   constexpr int kTL = 2048;         // LDS copy of T's head
   __shared__ double s_inv[128], s_logc[128];
   __shared__ int64_t s_T[TASK == 0 ? kTL : 1];
@@ -2216,6 +2270,9 @@ __global__ void __launch_bounds__(kBlock, SEG == 64 ? 4 : 1) k_search_rf_seg(Dat
 // ERT, one wave per item (any node size; lanes stride over the entries).
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArgs lv, ItemArgs it) {
+  resolve_level(lv, d.C);
+  resolve_items(it);
+  if ((int64_t)blockIdx.x * kWaves >= it.n_groups * it.per_group) return;
   __shared__ double s_inv[128], s_logc[128];
   load_ln_table(s_inv, s_logc);
   __syncthreads();
@@ -2309,7 +2366,6 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArg
 // item writes its own result slot.
 // Wave-aggregated: one global atomic per wave and tier.  Tier k holds the
 // groups with cuts[k-1] < m <= cuts[k].
-constexpr int kMaxTiers = 7;
 struct Cuts {
   int c[kMaxTiers];  // ascending; the last tier takes everything above
   int n;             // number of tiers
@@ -2332,7 +2388,9 @@ __device__ __forceinline__ int tier_of(int m, const Cuts& cuts) {
 // Each wave owns a contiguous range of the list: it counts its tiers, takes
 // one slot range per tier with a single atomic, then writes its entries.
 __global__ void k_tier_split(const uint32_t* list, int64_t n, const uint32_t* open_idx, const int32_t* node_m, Cuts cuts,
-                             TierLists out, unsigned int* counts) {
+                             TierLists out, unsigned int* counts, const unsigned int* n_dev, const int64_t* off_dev) {
+  if (n_dev) n = *n_dev;            // device-described level
+  if (off_dev) node_m += *off_dev;
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t wave = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
   const int64_t n_waves = (int64_t)gridDim.x * kWaves;
@@ -2398,9 +2456,13 @@ struct DecideArgs {
   uint32_t* retry_flag;
   const uint32_t* retry_list;
   int64_t n_retry;
+  const unsigned int* n_open_dev;   // device-described level
+  const unsigned int* n_retry_dev;
 };
 
 __global__ void k_decide1(DataArgs d, LevelArgs lv, DecideArgs a, int algorithm) {
+  resolve_level(lv, d.C);
+  if (a.n_open_dev) a.n_open = *a.n_open_dev;
   for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < a.n_open;
        o += (int64_t)gridDim.x * blockDim.x) {
     const float* rs = a.res_score + o * a.D;
@@ -2446,6 +2508,7 @@ __global__ void k_decide1(DataArgs d, LevelArgs lv, DecideArgs a, int algorithm)
 
 // Phase 2: continue the sequential retry rule on the nodes that needed it.
 __global__ void k_decide2(DecideArgs a) {
+  if (a.n_retry_dev) a.n_retry = *a.n_retry_dev;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.n_retry;
        q += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t o = a.retry_list[q];
@@ -2490,6 +2553,7 @@ struct PartArgs {
   uint8_t* split_ok;
   unsigned long long* bytes;
   const uint32_t* list;  // optional: which open positions to process
+  const unsigned int* n_dev;  // device-described level: the launch's node count
 };
 
 // Side bits of the first pass kept for the scatter pass (nodes of up to
@@ -2498,6 +2562,9 @@ constexpr int kSideWords = 1024;
 
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, PartArgs pa) {
+  resolve_level(lv, d.C);
+  if (pa.n_dev) pa.n_open = *pa.n_dev;
+  if ((int64_t)blockIdx.x >= pa.n_open) return;
   __shared__ unsigned long long s_side[kSideWords];
   __shared__ uint32_t s_w[kWaves];
   __shared__ uint32_t s_cls[MAXC];
@@ -2617,6 +2684,8 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
 template <int MAXC, int TASK>
 __global__ void __launch_bounds__(kBlock) k_partition_wave(DataArgs d, LevelArgs lv, PartArgs pa,
                                                            const uint32_t* list, int64_t n_list) {
+  resolve_level(lv, d.C);
+  if (pa.n_dev) n_list = *pa.n_dev;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   unsigned long long my_bytes = 0;
@@ -2717,6 +2786,8 @@ template <int MAXC, int TASK, int SEG>
 __global__ void __launch_bounds__(kBlock) k_partition_seg(DataArgs d, LevelArgs lv, PartArgs pa,
                                                           const uint32_t* list, int64_t n_list) {
   constexpr int NPW = kWave / SEG;
+  resolve_level(lv, d.C);
+  if (pa.n_dev) n_list = *pa.n_dev;
   __shared__ uint32_t s_cls[kWaves][NPW][TASK == 0 ? MAXC : 1];
   const int lane = threadIdx.x & (kWave - 1);
   const int sl = lane & (SEG - 1);
@@ -2829,9 +2900,20 @@ struct ChildArgs {
   int64_t* n_start;
   uint8_t* n_open;
   const int64_t* start;    // current level starts
+  const DevLevel* lvl;     // device-described level: n_level, offsets, depth
 };
 
+__device__ __forceinline__ void resolve_children(ChildArgs& a) {
+  if (a.lvl) {
+    a.n_level = a.lvl->n;
+    a.off_level = a.lvl->off;
+    a.off_next = a.off_level + a.n_level;
+    a.depth = (int)a.lvl->depth;
+  }
+}
+
 __global__ void k_children_count(ChildArgs a) {
+  resolve_children(a);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n_level;
        i += (int64_t)gridDim.x * blockDim.x) {
     uint32_t nc = 0;
@@ -2844,6 +2926,7 @@ __global__ void k_children_count(ChildArgs a) {
 }
 
 __global__ void k_children_write(DataArgs d, ChildArgs a) {
+  resolve_children(a);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n_level;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t gi = a.off_level + i;
@@ -3122,6 +3205,110 @@ __global__ void k_predict_nodes(int64_t n, int C, int task, const int32_t* featu
     }
     out[i] = p;
   }
+}
+
+// ---------------------------------------- device-described level step --
+// Partition tiers: 16- and 32-lane segments, a wave, a workgroup per node.
+constexpr Cuts kPartCuts{{16, 32, 512, 0x7FFFFFFF, 0, 0}, 4};
+
+__global__ void k_open_flags_dev(const uint8_t* open, const DevLevel* L, uint32_t* flag) {
+  const int64_t n = L->n;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    flag[i] = open[i] ? 1u : 0u;
+}
+
+__global__ void k_open_list_dev(const uint8_t* open, const uint32_t* pos, const DevLevel* L, uint32_t* list) {
+  const int64_t n = L->n;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (open[i]) list[pos[i]] = (uint32_t)i;
+}
+
+__global__ void k_compact_list_dev(const uint32_t* flag, const uint32_t* pos, const unsigned int* n_dev,
+                                   uint32_t* list) {
+  const int64_t n = *n_dev;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (flag[i]) list[pos[i]] = (uint32_t)i;
+}
+
+// Exclusive scan of n u32 (n read from the device) into out[0..n], out[n] =
+// total, also stored to *total: kDScanBlocks workgroups each own a
+// contiguous range (per-range sums, one workgroup scans the sums, then every
+// range is rescanned with its offset in tiles of kScanTile).
+constexpr int kDScanBlocks = 512;
+
+__device__ __forceinline__ int64_t dscan_n(const int64_t* n64, const unsigned int* n32) {
+  return n64 ? *n64 : (int64_t)*n32;
+}
+
+__global__ void __launch_bounds__(kBlock) k_dscan_sums(const uint32_t* in, const int64_t* n64,
+                                                       const unsigned int* n32, uint32_t* sums) {
+  __shared__ uint32_t s_w[kWaves];
+  const int64_t n = dscan_n(n64, n32);
+  const int64_t per = (n + kDScanBlocks - 1) / kDScanBlocks;
+  const int64_t lo = blockIdx.x * per, hi = min(lo + per, n);
+  uint32_t acc = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc += in[i];
+  uint32_t tot;
+  block_excl_scan(acc, s_w, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kBlock) k_dscan_top(uint32_t* sums, uint32_t* out, const int64_t* n64,
+                                                      const unsigned int* n32, unsigned int* total) {
+  static_assert(kDScanBlocks == 2 * kBlock, "two sums per thread");
+  __shared__ uint32_t s_w[kWaves];
+  const int t = threadIdx.x;
+  const uint32_t a = sums[2 * t], b = sums[2 * t + 1];
+  uint32_t tot;
+  const uint32_t base = block_excl_scan(a + b, s_w, &tot);
+  sums[2 * t] = base;
+  sums[2 * t + 1] = base + a;
+  if (t == 0) {
+    out[dscan_n(n64, n32)] = tot;
+    if (total) *total = tot;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_dscan_apply(const uint32_t* in, const int64_t* n64,
+                                                        const unsigned int* n32, const uint32_t* offs, uint32_t* out) {
+  __shared__ uint32_t s_w[kWaves];
+  const int64_t n = dscan_n(n64, n32);
+  const int64_t per = (n + kDScanBlocks - 1) / kDScanBlocks;
+  const int64_t lo = blockIdx.x * per, hi = min(lo + per, n);
+  uint32_t carry = offs[blockIdx.x];
+  for (int64_t t0 = lo; t0 < hi; t0 += kScanTile) {
+    uint32_t v[16];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int64_t idx = t0 + (int64_t)threadIdx.x * 16 + k;
+      v[k] = idx < hi ? in[idx] : 0u;
+      sum += v[k];
+    }
+    uint32_t tile_tot;
+    uint32_t base = block_excl_scan(sum, s_w, &tile_tot) + carry;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int64_t idx = t0 + (int64_t)threadIdx.x * 16 + k;
+      if (idx < hi) out[idx] = base;
+      base += v[k];
+    }
+    carry += tile_tot;
+    __syncthreads();  // s_w is reused by the next tile's scan
+  }
+}
+
+// End of a level step: the next level becomes current; its counters start
+// at zero (a kernel, so that a captured step holds no memset node).
+__global__ void k_level_advance(DevLevel* L) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  L->off += L->n;
+  L->n = L->n_next;
+  L->depth += 1;
+  L->n_open = 0;
+  L->n_retry = 0;
+  L->n_next = 0;
+  for (int k = 0; k < kMaxTiers; ++k) L->tier[k] = L->rtier[k] = L->ptier[k] = 0;
 }
 
 // ------------------------------------------------------ small helpers ---
@@ -3939,7 +4126,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     for (auto& tl : tier_list) tl.reserve(it.n_groups);
     DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
     hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(it.n_groups), 1024)), dim3(kBlock), 0, s, it.list, it.n_groups,
-                       lv.open_idx, lv.m, cuts, tier_lists(), tier_cnt.p);
+                       lv.open_idx, lv.m, cuts, tier_lists(), tier_cnt.p, nullptr, nullptr);
     unsigned int cnt[kMaxTiers];
     DTE_HIP(hipMemcpyAsync(cnt, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
     DTE_HIP(hipStreamSynchronize(s));
@@ -4012,7 +4199,312 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   uint32_t* nxt_aux = auxB.p;
   int64_t* nxt_ey = eyB.p;
 
+  // ---- device-described level steps.  Once every open node of a level
+  // fits the device-sized tiers, all further levels run from two captured
+  // hipGraphs (even / odd: the ping-pong buffers swap every level) whose
+  // kernels read the level's offset and counts from a DevLevel in device
+  // memory and run on capped grids; counters are cleared by a kernel.  The
+  // buffers are sized once for the largest level possible: a level has at
+  // most one node per entry, and at most T * N / (min_node + 1) open nodes.
+  DevBuf<unsigned long long> graph_bytes(kMaxTiers + 1);  // search tiers, partition
+  int64_t graph_replays = 0;
+  auto device_levels = [&](int depth0) {
+    const int64_t min1 = (int64_t)std::max(p.min_node_size, 0) + 1;
+    const int64_t NO = std::max<int64_t>(1, std::min<int64_t>(E, (int64_t)T * N / min1));
+    const int64_t NL = std::max<int64_t>(n_level, std::min<int64_t>(E, 2 * NO));
+    auto fit_to = [](auto& b, int64_t n) {
+      if ((int64_t)b.n < n) b.alloc((size_t)n);
+    };
+    {  // the current level's working arrays keep their contents
+      DevBuf<int64_t> st((size_t)NL + 1);
+      DevBuf<uint8_t> op((size_t)NL + 1);
+      DTE_HIP(hipMemcpyAsync(st.p, w_start.p, 8 * n_level, hipMemcpyDeviceToDevice, s));
+      DTE_HIP(hipMemcpyAsync(op.p, w_open.p, n_level, hipMemcpyDeviceToDevice, s));
+      w_start = std::move(st);
+      w_open = std::move(op);
+    }
+    fit_to(w_start_next, NL + 1);
+    fit_to(w_open_next, NL + 1);
+    fit_to(flag32, NL);
+    fit_to(open_pos, NL + 1);
+    fit_to(open_idx, NO + 1);
+    fit_to(res_score, NO * D);
+    fit_to(res_thr, NO * D);
+    fit_to(res_feat, NO * D);
+    fit_to(ch_score, NO);
+    fit_to(ch_thr, NO);
+    fit_to(ch_feat, NO);
+    fit_to(retry_flag, NO);
+    fit_to(retry_pos, NO + 1);
+    fit_to(retry_list, NO + 1);
+    fit_to(l_m, NO);
+    fit_to(l_count, NO);
+    fit_to(l_cls, task == 0 ? NO * C : 1);
+    fit_to(l_sum, NO);
+    fit_to(l_hash, NO);
+    fit_to(split_ok, NO);
+    for (auto& tl : tier_list) fit_to(tl, NO);
+    fit_to(nchild, NL);
+    fit_to(cpos, NL + 1);
+    DevBuf<uint32_t> dsums(kDScanBlocks);
+    DevBuf<DevLevel> dlv(1);
+    DevLevel h{};
+    h.off = off_level;
+    h.n = n_level;
+    h.depth = depth0;
+    DTE_HIP(hipMemcpyAsync(dlv.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    DTE_HIP(hipMemsetAsync(graph_bytes.p, 0, graph_bytes.bytes(), s));
+    DevLevel* L = dlv.p;
+    const bool rf = algorithm == DTE_RANDOM_FOREST;
+    constexpr int kElemGrid = 1024;
+
+    auto dscan = [&](const uint32_t* in, const int64_t* n64, const unsigned int* n32, uint32_t* out,
+                     unsigned int* total) {
+      hipLaunchKernelGGL(k_dscan_sums, dim3(kDScanBlocks), dim3(kBlock), 0, s, in, n64, n32, dsums.p);
+      hipLaunchKernelGGL(k_dscan_top, dim3(1), dim3(kBlock), 0, s, dsums.p, out, n64, n32, total);
+      hipLaunchKernelGGL(k_dscan_apply, dim3(kDScanBlocks), dim3(kBlock), 0, s, in, n64, n32, dsums.p, out);
+    };
+    // One level step; par = 1 swaps the ping-pong buffers.
+    auto enqueue = [&](int par) {
+      int64_t* cs = par ? w_start_next.p : w_start.p;
+      int64_t* ns = par ? w_start.p : w_start_next.p;
+      uint8_t* co = par ? w_open_next.p : w_open.p;
+      uint8_t* no = par ? w_open.p : w_open_next.p;
+      uint32_t* clf = par ? lf_next.p : lf.p;
+      uint32_t* nlf = par ? lf.p : lf_next.p;
+      uint32_t* cbase = par ? base_next.p : base.p;
+      uint32_t* nbase = par ? base.p : base_next.p;
+      uint32_t* crow = par ? nxt_row : cur_row;
+      uint32_t* nrw = par ? cur_row : nxt_row;
+      uint32_t* caux = par ? nxt_aux : cur_aux;
+      uint32_t* nax = par ? cur_aux : nxt_aux;
+      int64_t* cey = par ? nxt_ey : cur_ey;
+      int64_t* ney = par ? cur_ey : nxt_ey;
+      // open list
+      hipLaunchKernelGGL(k_open_flags_dev, dim3(kElemGrid), dim3(kBlock), 0, s, co, (const DevLevel*)L, flag32.p);
+      dscan(flag32.p, &L->n, nullptr, open_pos.p, &L->n_open);
+      hipLaunchKernelGGL(k_open_list_dev, dim3(kElemGrid), dim3(kBlock), 0, s, co, open_pos.p, (const DevLevel*)L,
+                         open_idx.p);
+      LevelArgs lvd{};
+      lvd.tree = o_tree.p;
+      lvd.id = o_id.p;
+      lvd.count = o_count.p;
+      lvd.m = o_m.p;
+      lvd.cls = task == 0 ? o_cls.p : nullptr;
+      lvd.sum = task == 1 ? o_sum.p : nullptr;
+      lvd.start = cs;
+      lvd.open_idx = open_idx.p;
+      lvd.tree_global = tree_global.p;
+      lvd.row = crow;
+      lvd.aux = caux;
+      lvd.ey = cey;
+      lvd.off_dev = &L->off;
+      ItemArgs it{};
+      it.D = D;
+      it.res_score = res_score.p;
+      it.res_thr = res_thr.p;
+      it.res_feat = res_feat.p;
+      // split search of one phase over the tiers (tier 5, the chunk
+      // pipeline, never occurs: no node exceeds cut_block here)
+      auto phase = [&](const uint32_t* list, const unsigned int* n_dev, unsigned int* cnt, int per_group, int j0) {
+        hipLaunchKernelGGL(k_tier_split, dim3(kElemGrid), dim3(kBlock), 0, s, list, (int64_t)0,
+                           (const uint32_t*)open_idx.p, (const int32_t*)o_m.p, cuts, tier_lists(), cnt, n_dev,
+                           (const int64_t*)&L->off);
+        for (int tier = 0; tier < cuts.n; ++tier) {
+          if (rf && tier == 5) continue;
+          ItemArgs a = it;
+          a.list = tier_list[tier].p;
+          a.n_groups = 0;
+          a.n_dev = cnt + tier;
+          a.per_group = per_group;
+          a.j0 = j0;
+          a.bytes = graph_bytes.p + tier;
+          const int grid = rf ? (tier <= 3 ? wave_grid_cap : search_grid_cap) : (tier == 0 ? wave_grid_cap : search_grid_cap);
+          search(algorithm, tier, grid, s, d, lvd, a);
+        }
+      };
+      phase(nullptr, &L->n_open, L->tier, D0, 0);
+      DecideArgs da{};
+      da.D0 = D0;
+      da.R = R;
+      da.D = D;
+      da.res_score = res_score.p;
+      da.res_thr = res_thr.p;
+      da.res_feat = res_feat.p;
+      da.ch_score = ch_score.p;
+      da.ch_thr = ch_thr.p;
+      da.ch_feat = ch_feat.p;
+      da.retry_flag = retry_flag.p;
+      da.n_open_dev = &L->n_open;
+      hipLaunchKernelGGL(k_decide1, dim3(kElemGrid), dim3(kBlock), 0, s, d, lvd, da, algorithm);
+      if (R > 0) {
+        dscan(retry_flag.p, nullptr, &L->n_open, retry_pos.p, &L->n_retry);
+        hipLaunchKernelGGL(k_compact_list_dev, dim3(kElemGrid), dim3(kBlock), 0, s, (const uint32_t*)retry_flag.p,
+                           (const uint32_t*)retry_pos.p, (const unsigned int*)&L->n_open, retry_list.p);
+        phase(retry_list.p, &L->n_retry, L->rtier, R, D0);
+        da.retry_list = retry_list.p;
+        da.n_retry_dev = &L->n_retry;
+        hipLaunchKernelGGL(k_decide2, dim3(kElemGrid), dim3(kBlock), 0, s, da);
+      }
+      // partition
+      PartArgs pa{};
+      pa.ch_feat = ch_feat.p;
+      pa.ch_thr = ch_thr.p;
+      pa.nrow = nrw;
+      pa.naux = nax;
+      pa.ney = ney;
+      pa.l_m = l_m.p;
+      pa.l_count = l_count.p;
+      pa.l_cls = l_cls.p;
+      pa.l_sum = l_sum.p;
+      pa.l_hash = l_hash.p;
+      pa.split_ok = split_ok.p;
+      pa.bytes = graph_bytes.p + kMaxTiers;
+      hipLaunchKernelGGL(k_tier_split, dim3(kElemGrid), dim3(kBlock), 0, s, (const uint32_t*)nullptr, (int64_t)0,
+                         (const uint32_t*)open_idx.p, (const int32_t*)o_m.p, kPartCuts, tier_lists(), L->ptier,
+                         (const unsigned int*)&L->n_open, (const int64_t*)&L->off);
+      PartArgs p0 = pa, p1 = pa, p2 = pa, p3 = pa;
+      p0.n_dev = L->ptier + 0;
+      p1.n_dev = L->ptier + 1;
+      p2.n_dev = L->ptier + 2;
+      p3.n_dev = L->ptier + 3;
+      p3.list = tier_list[3].p;
+      partition_seg16(wave_grid_cap, s, d, lvd, p0, tier_list[0].p, NO);  // grid from the bound; count from p0.n_dev
+      partition_seg32(wave_grid_cap, s, d, lvd, p1, tier_list[1].p, NO);
+      partition_wave(wave_grid_cap, s, d, lvd, p2, tier_list[2].p, NO);
+      partition(search_grid_cap, s, d, lvd, p3);
+      // children
+      ChildArgs ca{};
+      ca.lvl = L;
+      ca.open = co;
+      ca.open_pos = open_pos.p;
+      ca.split_ok = split_ok.p;
+      ca.ch_feat = ch_feat.p;
+      ca.ch_thr = ch_thr.p;
+      ca.ch_score = ch_score.p;
+      ca.l_m = l_m.p;
+      ca.l_count = l_count.p;
+      ca.l_cls = l_cls.p;
+      ca.l_sum = l_sum.p;
+      ca.l_hash = l_hash.p;
+      ca.nchild = nchild.p;
+      ca.cpos = cpos.p;
+      ca.lf = clf;
+      ca.base = cbase;
+      ca.o_tree = o_tree.p;
+      ca.o_id = o_id.p;
+      ca.o_depth = o_depth.p;
+      ca.o_count = o_count.p;
+      ca.o_m = o_m.p;
+      ca.o_cls = task == 0 ? o_cls.p : nullptr;
+      ca.o_sum = o_sum.p;
+      ca.o_hash = o_hash.p;
+      ca.o_value = o_value.p;
+      ca.o_feature = o_feature.p;
+      ca.o_thr = o_thr.p;
+      ca.o_score = o_score.p;
+      ca.o_left = o_left.p;
+      ca.start = cs;
+      ca.n_start = ns;
+      ca.n_open = no;
+      hipLaunchKernelGGL(k_children_count, dim3(kElemGrid), dim3(kBlock), 0, s, ca);
+      dscan(nchild.p, &L->n, nullptr, cpos.p, &L->n_next);
+      hipLaunchKernelGGL(k_children_write, dim3(kElemGrid), dim3(kBlock), 0, s, d, ca);
+      hipLaunchKernelGGL(k_tree_update, dim3(grid_for(T + 1)), dim3(kBlock), 0, s, T, (const uint32_t*)cpos.p,
+                         (const uint32_t*)clf, (const uint32_t*)cbase, nlf, nbase);
+      hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(kWave), 0, s, L);
+      DTE_HIP(hipGetLastError());
+    };
+
+    struct Graphs {
+      hipGraphExec_t e[2] = {nullptr, nullptr};
+      ~Graphs() {
+        for (auto x : e)
+          if (x) (void)hipGraphExecDestroy(x);
+      }
+    } gr;
+    for (int par = 0; par < 2; ++par) {
+      hipGraph_t g = nullptr;
+      DTE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue(par);
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      DTE_HIP(hipStreamEndCapture(s, &g));
+      const hipError_t e = hipGraphInstantiate(&gr.e[par], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      DTE_HIP(e);
+    }
+    int par = 0;
+    auto replay = [&](int64_t k) {
+      for (int64_t i = 0; i < k; ++i) {
+        DTE_HIP(hipGraphLaunch(gr.e[par], s));
+        par ^= 1;
+        ++graph_replays;
+      }
+    };
+    {
+      Timer tm("level_graph", 0.0, s);
+      const int64_t steps = (int64_t)p.max_depth - depth0 + 1;  // levels depth0 .. max_depth
+      if (steps <= 64) {
+        replay(std::max<int64_t>(steps, 1));
+      } else {
+        // deep trees: batches of 8 steps; the next batch is queued before the
+        // previous one's node count is looked at, so the device never waits
+        constexpr int kStepBatch = 8;
+        int64_t* pin = nullptr;
+        DTE_HIP(hipHostMalloc((void**)&pin, 2 * sizeof(int64_t), hipHostMallocDefault));
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        try {
+          DTE_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+          DTE_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+          for (int64_t b = 0;; ++b) {
+            replay(kStepBatch);
+            DTE_HIP(hipMemcpyAsync(pin + (b & 1), &L->n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+            DTE_HIP(hipEventRecord(ev[b & 1], s));
+            if (b > 0) {
+              DTE_HIP(hipEventSynchronize(ev[(b - 1) & 1]));
+              if (pin[(b - 1) & 1] == 0) break;
+            }
+          }
+        } catch (...) {
+          (void)hipStreamSynchronize(s);
+          for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+          (void)hipHostFree(pin);
+          throw;
+        }
+        DTE_HIP(hipStreamSynchronize(s));
+        for (auto e : ev) (void)hipEventDestroy(e);
+        (void)hipHostFree(pin);
+      }
+    }
+    int64_t fin[2];  // DevLevel.off, DevLevel.n
+    DTE_HIP(hipMemcpyAsync(fin, dlv.p, sizeof(fin), hipMemcpyDeviceToHost, s));
+    DTE_HIP(hipStreamSynchronize(s));
+    if (fin[1] != 0) fail(DTE_ERR_INTERNAL, "level graph: nodes left after the last level step");
+    off_level = fin[0];
+    n_level = 0;
+    if (par) {  // the final per-tree counts are in the other ping-pong buffer
+      std::swap(lf, lf_next);
+      std::swap(base, base_next);
+      std::swap(w_start, w_start_next);
+      std::swap(w_open, w_open_next);
+      std::swap(cur_row, nxt_row);
+      std::swap(cur_aux, nxt_aux);
+      std::swap(cur_ey, nxt_ey);
+    }
+  };
+  const bool level_graphs = !plain && p.build_mode == DTE_BUILD_BATCHED;
+
   for (int depth = 0; n_level > 0; ++depth) {
+    if (level_graphs && (algorithm != DTE_RANDOM_FOREST || max_m <= cut_block)) {
+      device_levels(depth);
+      break;
+    }
     double t_level0 = 0;
     if (trace) {
       DTE_HIP(hipStreamSynchronize(s));
@@ -4144,8 +4636,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         for (auto& tl : tier_list) tl.reserve(n_open);
         DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
         hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(n_open), 1024)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
-                           n_open, lv.open_idx, lv.m, Cuts{{16, 32, 512, 0x7FFFFFFF, 0, 0}, 4}, tier_lists(),
-                           tier_cnt.p);
+                           n_open, lv.open_idx, lv.m, kPartCuts, tier_lists(), tier_cnt.p, nullptr, nullptr);
         unsigned int pc[kMaxTiers];
         DTE_HIP(hipMemcpyAsync(pc, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
         DTE_HIP(hipStreamSynchronize(s));
@@ -4253,6 +4744,18 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     }
     profile_add_bytes("split_search", total);
     profile_add_bytes("partition", (double)read_one(part_bytes.p, s));
+    if (graph_replays) {
+      unsigned long long gb[kMaxTiers + 1];
+      DTE_HIP(hipMemcpyAsync(gb, graph_bytes.p, sizeof(gb), hipMemcpyDeviceToHost, s));
+      DTE_HIP(hipStreamSynchronize(s));
+      double g_search = 0;
+      for (int tier = 0; tier < kMaxTiers; ++tier) g_search += (double)gb[tier];
+      // level steps of the graphs: their split-search bytes (partition bytes
+      // beside them) over the whole step time
+      profile_add_bytes("level_graph", g_search);
+      profile_add_bytes("level_graph_partition", (double)gb[kMaxTiers]);
+    }
+    profile_count("level_graph_replays", graph_replays);
   }
 
   // ---- canonical tree-major arrays

@@ -98,7 +98,8 @@ void check_params(const dte_params* p) {
   if (p->min_node_size < 0) fail(DTE_ERR_INVALID_ARGUMENT, "min_node_size must be >= 0");
   if (p->task == DTE_CLASSIFICATION && (p->n_classes <= 0 || p->n_classes > 32))
     fail(DTE_ERR_INVALID_ARGUMENT, "n_classes must be in [1, 32]");
-  if (p->build_mode != DTE_BUILD_BATCHED && p->build_mode != DTE_BUILD_PLAIN)
+  if (p->build_mode != DTE_BUILD_BATCHED && p->build_mode != DTE_BUILD_PLAIN &&
+      p->build_mode != DTE_BUILD_BATCHED_HOST)
     fail(DTE_ERR_INVALID_ARGUMENT, "unknown build_mode");
   if (p->tree_offset < 0 || (int64_t)p->tree_offset + p->n_trees > 0x7FFFFFFF)
     fail(DTE_ERR_INVALID_ARGUMENT, "tree_offset out of range");
@@ -277,6 +278,18 @@ hipStream_t library_stream(int device) {
   auto it = g_streams.find(device);
   if (it == g_streams.end()) fail(DTE_ERR_INTERNAL, "device not initialised");
   return it->second;
+}
+
+// Add a count to a family's launches without a time (e.g. graph replays).
+void profile_count(const char* name, int64_t n) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  drain_pending();
+  auto it = std::find_if(g_stats.begin(), g_stats.end(), [&](const KernelStat& k) { return k.name == name; });
+  if (it == g_stats.end()) {
+    g_stats.push_back(KernelStat{name, 0, 0.0, 0.0});
+    it = g_stats.end() - 1;
+  }
+  it->launches += n;
 }
 
 // Add algorithmic bytes to a kernel family (known only after the launches

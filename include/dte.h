@@ -52,11 +52,18 @@ extern "C" {
 #define DTE_REGRESSION 1
 
 /* Build drivers.  BATCHED: all open nodes of every tree of a level in one
- * set of launches.  PLAIN: the reference's launch structure (tree batches of
- * <= 100, node chunks of <= 512, one block per node, host round trips per
- * chunk; gpudte_algorithm.cc:105-253).  Both produce identical forests. */
+ * set of launches; once every open node fits the device-sized tiers (random
+ * forest: at most cut_block entries; extra trees: from the root) each level
+ * step is a replay of a captured hipGraph whose kernels read the level's
+ * counts from device memory, with no host round trip between levels.
+ * BATCHED_HOST: the same launches sized on the host from counts read back
+ * at every level (no level graphs).  PLAIN: the reference's launch
+ * structure (tree batches of <= 100, node chunks of <= 512, one block per
+ * node, host round trips per chunk; gpudte_algorithm.cc:105-253).  All
+ * produce identical forests. */
 #define DTE_BUILD_BATCHED 0
 #define DTE_BUILD_PLAIN 1
+#define DTE_BUILD_BATCHED_HOST 2
 
 /* Fit parameters.  Mirrors the int-keyed param pack of
  * EnsemblesInterface::CreateGpuRfParamPack / CreateGpuErtParamPack
@@ -72,7 +79,7 @@ typedef struct dte_params {
   int32_t n_classes;           /* MlDataFrame::GetNrTargets(); classification only */
   int32_t device_id;           /* kDevId */
   uint64_t seed;               /* RNG key; defaults 324123 (RF) / 123512 (ERT) as gpurf.cu:55, gpuert.cu:18 */
-  int32_t build_mode;          /* DTE_BUILD_BATCHED (default) | DTE_BUILD_PLAIN */
+  int32_t build_mode;          /* DTE_BUILD_BATCHED (default) | DTE_BUILD_PLAIN | DTE_BUILD_BATCHED_HOST */
   int32_t tree_offset;         /* index of the first tree: the tree counter of every random draw.  Parts
                                   built with offsets 0, n0, n0+n1, ... concatenate to the forest of one
                                   build (what SplitParameterPack + AggregateModels need, gpu_algorithm.cc:
