@@ -3352,13 +3352,24 @@ __global__ void k_strided_gather(const uint32_t* in, int64_t stride, int n, uint
   if (i < n) out[i] = in[(int64_t)i * stride];
 }
 
-__global__ void k_max_m(const int32_t* m, int64_t n, uint32_t* out) {
-  uint32_t v = 0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    v = max(v, (uint32_t)m[i]);
+// Largest node of a level (out[0]) and the entries of its nodes above
+// `cut` entries (out[1]).
+__global__ void k_level_sizes(const int32_t* m, int64_t n, int cut, unsigned long long* out) {
+  unsigned long long v = 0, big = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long x = (unsigned long long)m[i];
+    v = max(v, x);
+    big += x > (unsigned long long)cut ? x : 0ull;
+  }
 #pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, kWave));
-  if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(out, v);
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    v = max(v, (unsigned long long)__shfl_xor(v, off, kWave));
+    big += __shfl_xor(big, off, kWave);
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    atomicMax(&out[0], v);
+    if (big) atomicAdd(&out[1], big);
+  }
 }
 
 // ------------------------------------------------------ host side -------
@@ -3783,6 +3794,10 @@ constexpr int64_t kPlainChunk = 512;
 // Largest count with a precomputed x ln x term (512 MB of table).
 constexpr int64_t kTMaxCap = (int64_t)1 << 26;
 
+// Largest node a level graph takes (entries): its workgroup-per-item kernel
+// needs a scratch slab of this many entries per workgroup.
+constexpr int64_t kGraphMaxM = 65536;
+
 __global__ void k_build_T(int64_t* T, int64_t tmax, int F) {
   for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x <= tmax; x += (int64_t)gridDim.x * blockDim.x)
     T[x] = xlnx_fixed((uint64_t)x, F, g_ln_table.inv, g_ln_table.logc);
@@ -4191,6 +4206,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   int64_t n_level = T;
   int64_t max_m = 0;
   for (int t = 0; t < T; ++t) max_m = std::max(max_m, root_m[t]);
+  int64_t big_entries = max_m > cut_block ? E : 0;  // entries in nodes above cut_block
 
   uint32_t* cur_row = rowA.p;
   uint32_t* cur_aux = auxA.p;
@@ -4246,6 +4262,15 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     for (auto& tl : tier_list) fit_to(tl, NO);
     fit_to(nchild, NL);
     fit_to(cpos, NL + 1);
+    // scratch slabs of the workgroup-per-item kernel for nodes above
+    // cut_block (random forest), one per workgroup of its grid
+    const bool big_tier = algorithm == DTE_RANDOM_FOREST && max_m > cut_block;
+    if (big_tier && scr_stride < max_m) {
+      scr_key.alloc((size_t)search_grid_cap * max_m);
+      scr_aux.alloc((size_t)search_grid_cap * max_m);
+      if (task == 1) scr_y.alloc((size_t)search_grid_cap * max_m);
+      scr_stride = max_m;
+    }
     DevBuf<uint32_t> dsums(kDScanBlocks);
     DevBuf<DevLevel> dlv(1);
     DevLevel h{};
@@ -4304,14 +4329,18 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       it.res_score = res_score.p;
       it.res_thr = res_thr.p;
       it.res_feat = res_feat.p;
-      // split search of one phase over the tiers (tier 5, the chunk
-      // pipeline, never occurs: no node exceeds cut_block here)
+      it.scr_key = scr_key.p;
+      it.scr_aux = scr_aux.p;
+      it.scr_y = scr_y.p;
+      it.scr_stride = scr_stride;
+      // split search of one phase over the tiers; nodes above cut_block
+      // (tier 5) take one workgroup per item through its scratch slab
       auto phase = [&](const uint32_t* list, const unsigned int* n_dev, unsigned int* cnt, int per_group, int j0) {
         hipLaunchKernelGGL(k_tier_split, dim3(kElemGrid), dim3(kBlock), 0, s, list, (int64_t)0,
                            (const uint32_t*)open_idx.p, (const int32_t*)o_m.p, cuts, tier_lists(), cnt, n_dev,
                            (const int64_t*)&L->off);
         for (int tier = 0; tier < cuts.n; ++tier) {
-          if (rf && tier == 5) continue;
+          if (rf && tier == 5 && !big_tier) continue;  // no node above cut_block in these levels
           ItemArgs a = it;
           a.list = tier_list[tier].p;
           a.n_groups = 0;
@@ -4320,7 +4349,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
           a.j0 = j0;
           a.bytes = graph_bytes.p + tier;
           const int grid = rf ? (tier <= 3 ? wave_grid_cap : search_grid_cap) : (tier == 0 ? wave_grid_cap : search_grid_cap);
-          search(algorithm, tier, grid, s, d, lvd, a);
+          search(algorithm, rf && tier == 5 ? 6 : tier, grid, s, d, lvd, a);
         }
       };
       phase(nullptr, &L->n_open, L->tier, D0, 0);
@@ -4501,7 +4530,12 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   const bool level_graphs = !plain && p.build_mode == DTE_BUILD_BATCHED;
 
   for (int depth = 0; n_level > 0; ++depth) {
-    if (level_graphs && (algorithm != DTE_RANDOM_FOREST || max_m <= cut_block)) {
+    // Random forests switch to the level graphs once the nodes above
+    // cut_block hold at most 1/64 of the entries and none exceeds
+    // kGraphMaxM: inside a graph such nodes take one workgroup per
+    // (node, draw) with a scratch slab each (the plain driver's kernel).
+    if (level_graphs && (algorithm != DTE_RANDOM_FOREST ||
+                         (max_m <= kGraphMaxM && big_entries * 64 <= E))) {
       device_levels(depth);
       break;
     }
@@ -4722,13 +4756,18 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     }
     off_level += n_level;
     n_level = n_next;
-    // largest node of the next level (for scratch sizing)
+    // largest node of the next level (for scratch sizing) and the entries
+    // in nodes above cut_block (whether the level graphs can take over)
     if (n_level > 0 && algorithm == DTE_RANDOM_FOREST) {
-      DevBuf<uint32_t> mx(1);
-      DTE_HIP(hipMemsetAsync(mx.p, 0, 4, s));
-      hipLaunchKernelGGL(k_max_m, dim3(std::min(grid_for(n_level), 1024)), dim3(kBlock), 0, s,
-                         o_m.p + off_level, n_level, mx.p);
-      max_m = read_one(mx.p, s);
+      DevBuf<unsigned long long> mx(2);
+      DTE_HIP(hipMemsetAsync(mx.p, 0, 16, s));
+      hipLaunchKernelGGL(k_level_sizes, dim3(std::min(grid_for(n_level), 1024)), dim3(kBlock), 0, s,
+                         o_m.p + off_level, n_level, cut_block, mx.p);
+      unsigned long long hm[2];
+      DTE_HIP(hipMemcpyAsync(hm, mx.p, 16, hipMemcpyDeviceToHost, s));
+      DTE_HIP(hipStreamSynchronize(s));
+      max_m = (int64_t)hm[0];
+      big_entries = (int64_t)hm[1];
     }
   }
   const int64_t n_nodes = off_level;

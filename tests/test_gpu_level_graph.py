@@ -37,24 +37,29 @@ def _same(a, b):
 
 
 CASES = [
-    # algorithm, task, rows, features, classes, k, depth, min node, trees
-    (capi.RANDOM_FOREST, capi.CLASSIFICATION, 30000, 16, 5, 4, 18, 5, 6),    # chunk pipeline above, graphs below
-    (capi.RANDOM_FOREST, capi.REGRESSION, 20000, 12, 0, 4, 16, 5, 4),
-    (capi.EXTRA_TREES, capi.CLASSIFICATION, 8000, 20, 3, 0, 1000, 5, 10),  # deep trees: replays in batches
-    (capi.EXTRA_TREES, capi.REGRESSION, 6000, 10, 0, 3, 25, 10, 6),
-    (capi.RANDOM_FOREST, capi.CLASSIFICATION, 1500, 8, 2, 3, 1000, 1, 8),  # deep RF, every level in graphs
+    # algorithm, task, rows, features, classes, k, depth, min node, trees, tuning
+    (capi.RANDOM_FOREST, capi.CLASSIFICATION, 30000, 16, 5, 4, 18, 5, 6, {}),  # chunk pipeline above, graphs below
+    (capi.RANDOM_FOREST, capi.REGRESSION, 20000, 12, 0, 4, 16, 5, 4, {}),
+    (capi.EXTRA_TREES, capi.CLASSIFICATION, 8000, 20, 3, 0, 1000, 5, 10, {}),  # deep trees: replays in batches
+    (capi.EXTRA_TREES, capi.REGRESSION, 6000, 10, 0, 3, 25, 10, 6, {}),
+    (capi.RANDOM_FOREST, capi.CLASSIFICATION, 1500, 8, 2, 3, 1000, 1, 8, {}),  # deep RF, every level in graphs
+    # nodes above a low workgroup cut inside the graphs (one workgroup per
+    # item through a scratch slab)
+    (capi.RANDOM_FOREST, capi.CLASSIFICATION, 20000, 10, 4, 3, 20, 2, 4, {"cut_team": 64, "cut_block": 64}),
+    (capi.RANDOM_FOREST, capi.REGRESSION, 12000, 8, 0, 3, 20, 2, 3, {"cut_team": 64, "cut_block": 64}),
 ]
 
 
-@pytest.mark.parametrize("case", CASES, ids=[f"alg{c[0]}-task{c[1]}-n{c[2]}-d{c[6]}" for c in CASES])
+@pytest.mark.parametrize("case", CASES, ids=[f"alg{c[0]}-task{c[1]}-n{c[2]}-d{c[6]}-cut{c[9].get('cut_block', 0)}"
+                                             for c in CASES])
 def test_graph_levels_equal_host_levels_and_oracle(case):
-    alg, task, n, f, c, k, depth, min_node, trees = case
+    alg, task, n, f, c, k, depth, min_node, trees, tuning = case
     if task == capi.CLASSIFICATION:
         X, y = synth.gaussian_classes(seed=n + f, n=n, f=f, n_informative=min(f, 6), n_classes=c)
     else:
         X, y = synth.sine_regression(seed=n + f, n=n, f=f)
     kw = dict(algorithm=alg, task=task, n_trees=trees, n_features_per_node=k, max_depth=depth,
-              min_node_size=min_node, n_classes=c, seed=77)
+              min_node_size=min_node, n_classes=c, seed=77, **tuning)
     g, replays = _fit(X, y, capi.BUILD_BATCHED, **kw)
     h, replays_host = _fit(X, y, capi.BUILD_BATCHED_HOST, **kw)
     assert replays > 0 and replays_host == 0, (replays, replays_host)
