@@ -186,6 +186,8 @@ def serve_bench(capi, synth, np, n_requests, seed=0):
 
 def other_workload(name, args):
     """Measurement lines of the other workload shapes (not the headline)."""
+    import ctypes
+
     import numpy as np
     from datamininglibs_amd import capi, synth
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -266,16 +268,38 @@ def other_workload(name, args):
             nodes += f.n_nodes
             f.close()
         dt_seq = time.perf_counter() - t
-        # all 64 fits at once on 8 concurrent streams (dte_fit_many)
+        # all 64 forests in one joint build (dte_fit_many: every level step
+        # serves the open nodes of every forest)
         args = ([Xd.ptr for Xd in devs], [X.shape[1] for X, _ in data], [X.shape[0] for X, _ in data],
                 [y for _, y in data], params)
-        for f in capi.fit_many(*args, max_concurrent=8):
+        for f in capi.fit_many(*args):
             f.close()
+        capi.profile_reset()
+        capi.profiling(True)
         t = time.perf_counter()
-        forests = capi.fit_many(*args, max_concurrent=8)
+        forests = capi.fit_many(*args)
         dt = time.perf_counter() - t
+        capi.profiling(False)
+        replays = capi.profile_stats().get("level_graph_replays", {"launches": 0})["launches"]
         assert sum(f.n_nodes for f in forests) == nodes
         for f in forests:
+            f.close()
+        # the same 64 fits as separate fits on 8 concurrent streams, each
+        # sizing its levels on the host (no joint build, no level graphs)
+        hp = []
+        for p in params:
+            q = capi.default_params(capi.EXTRA_TREES)
+            ctypes.memmove(ctypes.addressof(q), ctypes.addressof(p), ctypes.sizeof(q))
+            q.build_mode = capi.BUILD_BATCHED_HOST
+            hp.append(q)
+        hargs = args[:4] + (hp,)
+        for f in capi.fit_many(*hargs, max_concurrent=8):
+            f.close()
+        t = time.perf_counter()
+        hforests = capi.fit_many(*hargs, max_concurrent=8)
+        dt_streams = time.perf_counter() - t
+        assert sum(f.n_nodes for f in hforests) == nodes
+        for f in hforests:
             f.close()
         small = sorted(range(64), key=lambda i: specs[i]["n"] * specs[i]["f"])[:8]
         t = time.perf_counter()
@@ -287,7 +311,9 @@ def other_workload(name, args):
         ntr = sum(specs[i]["n"] for i in small)
         line = {"workload": "64 extra-trees forests, 2k..200k rows, 16..256 features, 2..8 classes, 50 deep trees each",
                 "gpu_trees_per_s": round(64 * 50 / dt, 1), "gpu_seconds_all": round(dt, 2), "nodes": nodes,
-                "gpu_concurrency": "8 fits at a time on 8 streams (dte_fit_many)",
+                "gpu_path": "one joint build of the 64 forests (dte_fit_many), every level step one hipGraph replay",
+                "level_graph_replays": replays,
+                "gpu_trees_per_s_8_streams_host_levels": round(64 * 50 / dt_streams, 1),
                 "tuning": {"cut_ert": tuned.cut_ert, "autotune_s": round(tune_s, 2)},
                 "gpu_trees_per_s_one_after_another": round(64 * 50 / dt_seq, 1),
                 "cpu_sample": f"8 smallest forests ({ntr} rows total), {dto:.1f} s on {cores} threads",

@@ -288,7 +288,9 @@ def autotune(X_dev_ptr, n_rows, n_features, y, params, trial_trees=0):
 
 
 def fit_many(X_dev_ptrs, n_rows, n_features, ys, params, max_concurrent=0):
-    """Independent fits run concurrently on separate streams (dte_fit_many)."""
+    """Independent fits (dte_fit_many): extra-trees forests of one task are
+    built jointly, one level step serving all of them; other fits run
+    concurrently on separate streams."""
     n = len(X_dev_ptrs)
     ys = [np.ascontiguousarray(y, np.float32) for y in ys]
     xs = (P * n)(*[P(x) for x in X_dev_ptrs])

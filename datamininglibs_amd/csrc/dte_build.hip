@@ -77,7 +77,19 @@ struct DataArgs {
   int igF;           // fixed-point shift of the x ln x terms (per fit)
   const int64_t* T;  // T[x] = xlnx_fixed(x, igF) for x <= Tmax (classification)
   int64_t Tmax;
+  // Joint build of several extra-trees forests in one set of level steps:
+  // the data of batch tree t is forests[forest_of_tree[t]], whose D0 / R
+  // are its mandatory and retry draws; C is the joint class width in every
+  // forest's entry (classes past a forest's own count hold zero counts,
+  // which add nothing to the entropy sums).  NULL for a fit of one forest.
+  const struct DataArgs* forests;
+  const int32_t* forest_of_tree;
+  int D0, R;
 };
+
+__device__ __forceinline__ DataArgs forest_data(const DataArgs& d, int tree) {
+  return d.forests ? d.forests[d.forest_of_tree[tree]] : d;
+}
 
 // Counts and offsets of the level being built, in device memory, for the
 // level steps that run without a host round trip (captured in a hipGraph and
@@ -148,6 +160,17 @@ __device__ __forceinline__ void resolve_level(LevelArgs& lv, int C) {
 }
 __device__ __forceinline__ void resolve_items(ItemArgs& it) {
   if (it.n_dev) it.n_groups = *it.n_dev;
+}
+
+// Draw index of item position q of a node whose forest data is dd: j0 + q
+// for a fit of one forest.  A joint build pads every node to the widest
+// forest's draws: the first phase covers draws 0 .. D0-1 of the node's own
+// forest, the retry phase (j0 > 0) its draws D0 .. D0+R-1; -1 marks a
+// position past them.
+__device__ __forceinline__ int item_draw(const DataArgs& d, const DataArgs& dd, const ItemArgs& it, int q) {
+  if (!d.forests) return it.j0 + q;
+  if (it.j0 == 0) return q < dd.D0 ? q : -1;
+  return q < dd.R ? dd.D0 + q : -1;
 }
 
 // ------------------------------------------------- per-node context -----
@@ -1814,23 +1837,26 @@ __global__ void __launch_bounds__(kBlock) k_search_ert(DataArgs d, LevelArgs lv,
   const int64_t n_items = it.n_groups * it.per_group;
   for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
     const int64_t grp = it.node_major ? item % it.n_groups : item / it.per_group;
-    const int j = it.j0 + (int)(it.node_major ? item / it.n_groups : item % it.per_group);
+    const int q = (int)(it.node_major ? item / it.n_groups : item % it.per_group);
     const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
     const uint32_t i = lv.open_idx[o];
     const int tr = lv.tree[i];
+    const DataArgs dd = forest_data(d, tr);
+    const int j = item_draw(d, dd, it, q);
+    if (j < 0) continue;  // past this forest's draws (joint build); block-uniform
     const uint32_t tg = (uint32_t)lv.tree_global[tr];
     const uint32_t nid = lv.id[i];
-    const int f = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)j) % (uint32_t)d.F);
+    const int f = (int)(rand_u32(dd.seed, kStreamFeat, tg, nid, (uint64_t)j) % (uint32_t)dd.F);
     const int m = lv.m[i];
     const int64_t st = lv.start[i];
-    const float* Xf = d.X + (int64_t)f * d.N;
+    const float* Xf = dd.X + (int64_t)f * dd.N;
     if (t < MAXC) s_left[t] = 0;
     if (t == 0) {
       s_nl = 0;
       s_sl = 0;
       float acc = 0.0f;
       for (int s = 0; s < 10; ++s) {
-        const uint32_t pos = rand_u32(d.seed, kStreamThr, tg, nid, (uint64_t)j * 10 + s) % (uint32_t)m;
+        const uint32_t pos = rand_u32(dd.seed, kStreamThr, tg, nid, (uint64_t)j * 10 + s) % (uint32_t)m;
         acc = acc + Xf[lv.row[st + pos]];
       }
       s_thr = acc / 10.0f;
@@ -1889,7 +1915,7 @@ __global__ void __launch_bounds__(kBlock) k_search_ert(DataArgs d, LevelArgs lv,
         score = -FLT_MAX;
       } else if (TASK == 0) {
         const uint32_t* nclsp = lv.cls + (int64_t)i * d.C;
-        const int F = d.igF;
+        const int F = dd.igF;
         const double inv = 1.0 / ((double)n * kLn2);
         int64_t P = xlnx_fixed((uint64_t)n, F, s_inv, s_logc);
         int64_t sls = 0, srs = 0;
@@ -1903,7 +1929,7 @@ __global__ void __launch_bounds__(kBlock) k_search_ert(DataArgs d, LevelArgs lv,
         score = (float)ig_from_fixed(P + Q, F, inv);
       } else {
         const int64_t slt = (int64_t)s_sl;
-        score = (float)variance_gain(nlt, slt, n - nlt, lv.sum[i] - slt, d.shift);
+        score = (float)variance_gain(nlt, slt, n - nlt, lv.sum[i] - slt, dd.shift);
       }
       const int64_t ridx = (int64_t)o * it.D + j;
       it.res_score[ridx] = score;
@@ -2282,19 +2308,22 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArg
   const int64_t n_items = it.n_groups * it.per_group;
   for (int64_t item = (int64_t)blockIdx.x * kWaves + wv; item < n_items; item += (int64_t)gridDim.x * kWaves) {
     const int64_t grp = item / it.per_group;
-    const int j = it.j0 + (int)(item % it.per_group);
     const uint32_t o = it.list ? it.list[grp] : (uint32_t)grp;
     const uint32_t i = lv.open_idx[o];
-    const uint32_t tg = (uint32_t)lv.tree_global[lv.tree[i]];
+    const int tri = lv.tree[i];
+    const DataArgs dd = forest_data(d, tri);
+    const int j = item_draw(d, dd, it, (int)(item % it.per_group));
+    if (j < 0) continue;  // past this forest's draws (joint build); wave-uniform
+    const uint32_t tg = (uint32_t)lv.tree_global[tri];
     const uint32_t nid = lv.id[i];
-    const int f = (int)(rand_u32(d.seed, kStreamFeat, tg, nid, (uint64_t)j) % (uint32_t)d.F);
+    const int f = (int)(rand_u32(dd.seed, kStreamFeat, tg, nid, (uint64_t)j) % (uint32_t)dd.F);
     const int m = lv.m[i];
     const int64_t st = lv.start[i];
     const int64_t n = lv.count[i];
-    const float* Xf = d.X + (int64_t)f * d.N;
+    const float* Xf = dd.X + (int64_t)f * dd.N;
     float xs = 0.0f;
     if (lane < 10) {
-      const uint32_t pos = rand_u32(d.seed, kStreamThr, tg, nid, (uint64_t)j * 10 + lane) % (uint32_t)m;
+      const uint32_t pos = rand_u32(dd.seed, kStreamThr, tg, nid, (uint64_t)j * 10 + lane) % (uint32_t)m;
       xs = Xf[lv.row[st + pos]];
     }
     float acc = 0.0f;
@@ -2334,7 +2363,7 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArg
         for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
         lc = (lane == c) ? v : lc;
       }
-      const int F = d.igF;
+      const int F = dd.igF;
       const double inv = 1.0 / ((double)n * kLn2);
       int64_t tn = 0, tl = 0, tr = 0;
       if (lane < d.C) {
@@ -2349,7 +2378,7 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArg
       score = (float)ig_from_fixed(P + Q, F, inv);
     } else {
       sl = wave_sum_i64(sl);
-      score = (float)variance_gain(nl, sl, n - nl, lv.sum[i] - sl, d.shift);
+      score = (float)variance_gain(nl, sl, n - nl, lv.sum[i] - sl, dd.shift);
     }
     if (lane == 0) {
       const int64_t ridx = (int64_t)o * it.D + j;
@@ -2468,16 +2497,22 @@ __global__ void k_decide1(DataArgs d, LevelArgs lv, DecideArgs a, int algorithm)
     const float* rs = a.res_score + o * a.D;
     const float* rt = a.res_thr + o * a.D;
     const int32_t* rf = a.res_feat + o * a.D;
+    int D0 = a.D0, R = a.R;
+    if (d.forests) {  // joint build: the node's own forest's draws
+      const DataArgs dd = forest_data(d, lv.tree[lv.open_idx[o]]);
+      D0 = dd.D0;
+      R = dd.R;
+    }
     float bs = rs[0], bt = rt[0];
     int bf = rf[0];
-    for (int j = 1; j < a.D0; ++j)
+    for (int j = 1; j < D0; ++j)
       if (split_better(rs[j], rf[j], rt[j], bs, bf, bt)) {
         bs = rs[j];
         bf = rf[j];
         bt = rt[j];
       }
     uint32_t need = 0;
-    if (a.R > 0 && (double)bs <= 1e-3) {
+    if (R > 0 && (double)bs <= 1e-3) {
       const uint32_t i = lv.open_idx[o];
       bool pure = false;
       if (algorithm == 0 && d.C > 0 && lv.cls) {
@@ -2487,8 +2522,8 @@ __global__ void k_decide1(DataArgs d, LevelArgs lv, DecideArgs a, int algorithm)
       }
       if (pure) {
         const uint32_t tg = (uint32_t)lv.tree_global[lv.tree[i]];
-        for (int r = 0; r < a.R; ++r) {
-          const int f = (int)(rand_u32(d.seed, kStreamFeat, tg, lv.id[i], (uint64_t)(a.D0 + r)) % (uint32_t)d.F);
+        for (int r = 0; r < R; ++r) {
+          const int f = (int)(rand_u32(d.seed, kStreamFeat, tg, lv.id[i], (uint64_t)(D0 + r)) % (uint32_t)d.F);
           if (split_better(0.0f, f, 0.0f, bs, bf, bt)) {
             bs = 0.0f;
             bf = f;
@@ -2507,7 +2542,8 @@ __global__ void k_decide1(DataArgs d, LevelArgs lv, DecideArgs a, int algorithm)
 }
 
 // Phase 2: continue the sequential retry rule on the nodes that needed it.
-__global__ void k_decide2(DecideArgs a) {
+__global__ void k_decide2(DataArgs d, LevelArgs lv, DecideArgs a) {
+  resolve_level(lv, d.C);
   if (a.n_retry_dev) a.n_retry = *a.n_retry_dev;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < a.n_retry;
        q += (int64_t)gridDim.x * blockDim.x) {
@@ -2517,9 +2553,15 @@ __global__ void k_decide2(DecideArgs a) {
     const float* rs = a.res_score + (int64_t)o * a.D;
     const float* rt = a.res_thr + (int64_t)o * a.D;
     const int32_t* rf = a.res_feat + (int64_t)o * a.D;
-    for (int r = 0; r < a.R; ++r) {
+    int D0 = a.D0, R = a.R;
+    if (d.forests) {  // joint build: the node's own forest's draws
+      const DataArgs dd = forest_data(d, lv.tree[lv.open_idx[o]]);
+      D0 = dd.D0;
+      R = dd.R;
+    }
+    for (int r = 0; r < R; ++r) {
       if ((double)bs > 1e-3) break;
-      const int j = a.D0 + r;
+      const int j = D0 + r;
       if (split_better(rs[j], rf[j], rt[j], bs, bf, bt)) {
         bs = rs[j];
         bf = rf[j];
@@ -2579,7 +2621,8 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
     const int64_t st = lv.start[i];
     const int f = pa.ch_feat[o];
     const float thr = pa.ch_thr[o];
-    const float* Xf = d.X + (int64_t)f * d.N;
+    const DataArgs dd = forest_data(d, lv.tree[i]);
+    const float* Xf = dd.X + (int64_t)f * dd.N;
     if (t < MAXC) s_cls[t] = 0;
     if (t == 0) {
       s_cnt = 0;
@@ -2696,7 +2739,8 @@ __global__ void __launch_bounds__(kBlock) k_partition_wave(DataArgs d, LevelArgs
     const int m = lv.m[i];
     const int64_t st = lv.start[i];
     const float thr = pa.ch_thr[o];
-    const float* Xf = d.X + (int64_t)pa.ch_feat[o] * d.N;
+    const DataArgs dd = forest_data(d, lv.tree[i]);
+    const float* Xf = dd.X + (int64_t)pa.ch_feat[o] * dd.N;
     uint32_t loc[MAXC];
 #pragma unroll
     for (int c = 0; c < MAXC; ++c) loc[c] = 0;
@@ -2812,7 +2856,12 @@ __global__ void __launch_bounds__(kBlock) k_partition_seg(DataArgs d, LevelArgs 
       m = lv.m[i];
       st = lv.start[i];
       thr = pa.ch_thr[o];
-      Xf = d.X + (int64_t)pa.ch_feat[o] * d.N;
+      if (d.forests) {
+        const DataArgs dd = forest_data(d, lv.tree[i]);
+        Xf = dd.X + (int64_t)pa.ch_feat[o] * dd.N;
+      } else {
+        Xf = d.X + (int64_t)pa.ch_feat[o] * d.N;
+      }
     }
     const bool valid = node_ok && sl < m;
     uint32_t row = 0, a = 0;
@@ -2939,6 +2988,7 @@ __global__ void k_children_write(DataArgs d, ChildArgs a) {
     }
     const uint32_t o = a.open_pos[i];
     const int tr = a.o_tree[gi];
+    const DataArgs dd = forest_data(d, tr);
     a.o_feature[gi] = a.ch_feat[o];
     a.o_thr[gi] = a.ch_thr[o];
     a.o_score[gi] = a.ch_score[o];
@@ -2971,10 +3021,10 @@ __global__ void k_children_write(DataArgs d, ChildArgs a) {
       } else {
         const int64_t sm = s == 0 ? a.l_sum[o] : a.o_sum[gi] - a.l_sum[o];
         a.o_sum[g] = sm;
-        a.o_value[g] = (float)ldexp_d((double)sm / (double)cc, -d.shift);
+        a.o_value[g] = (float)ldexp_d((double)sm / (double)cc, -dd.shift);
       }
       a.n_start[p + s] = s == 0 ? st : st + mL;
-      a.n_open[p + s] = (cc > d.min_node && a.depth + 1 < d.max_depth) ? 1 : 0;
+      a.n_open[p + s] = (cc > dd.min_node && a.depth + 1 < dd.max_depth) ? 1 : 0;
     }
   }
 }
@@ -3900,28 +3950,61 @@ void prepare_data(FitData& fd, const float* X_dev_in, const float* y_host, int64
   }
 }
 
+// A joint build of several extra-trees forests (dte_fit_many): the trees of
+// forest f are batch trees [t0, t0 + T) of one build whose level steps serve
+// every forest at once.
+struct JointForest {
+  FitData* fd;
+  int64_t N;
+  int32_t F;
+  int T;
+  int t0;
+  dte_params p;
+};
+struct JointSpec {
+  std::vector<JointForest> f;
+  DataArgs d;  // joint data: forests / forest_of_tree set, C = joint class width
+  int C;       // joint class width (the widest forest's)
+  int D0, R;   // the widest mandatory and retry draw counts
+  int F;       // the widest feature count
+};
+
 // Builds trees [t0, t0 + T) of the fit level by level.  plain = 1 is the
 // per-node launch structure of the reference (gpudte_algorithm.cc:116-217):
 // open nodes in chunks of at most 512, one workgroup per node walking its
 // feature draws in order, a host round trip after every chunk; the same
 // device functions compute every result, so both modes give the same bits.
 Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_params& p, hipStream_t s,
-                  bool plain) {
+                  bool plain, const JointSpec* J = nullptr) {
   const int task = p.task;
   const int algorithm = p.algorithm;
-  const int C = task == 0 ? p.n_classes : 1;
+  const int C = J ? J->C : (task == 0 ? p.n_classes : 1);
   const int k = p.n_features_per_node > 0 ? p.n_features_per_node
                                           : (int)std::round(std::log((double)F)) + 1;
-  const int D0 = std::min(k, F);
-  const int R = k < F ? std::min(10, F - k) : 0;
+  const int D0 = J ? J->D0 : std::min(k, F);
+  const int R = J ? J->R : (k < F ? std::min(10, F - k) : 0);
   const int D = D0 + R;
-  const DataArgs d = fd.d;
+  const DataArgs d = J ? J->d : fd.d;
   const uint32_t* label = fd.label.p;
   const int64_t* yfix = fd.yfix.p;
   const int shift = fd.shift;
   std::vector<DevBuf<uint32_t>> scan_tmp;
+  // per batch tree: rows at its root, forest (joint build), global tree id
+  std::vector<int64_t> tN(T, N);
+  std::vector<int32_t> tf(T, 0);
   std::vector<int32_t> tg_h(T);
   std::iota(tg_h.begin(), tg_h.end(), t0);
+  if (J) {
+    for (size_t f = 0; f < J->f.size(); ++f)
+      for (int q = 0; q < J->f[f].T; ++q) {
+        const int t = J->f[f].t0 + q;
+        tN[t] = J->f[f].N;
+        tf[t] = (int32_t)f;
+        tg_h[t] = J->f[f].p.tree_offset + q;
+      }
+  }
+  auto tree_min_node = [&](int t) { return J ? J->f[tf[t]].p.min_node_size : p.min_node_size; };
+  auto tree_max_depth = [&](int t) { return J ? J->f[tf[t]].p.max_depth : p.max_depth; };
   DevBuf<int32_t> tree_global(T);
   DTE_HIP(hipMemcpyAsync(tree_global.p, tg_h.data(), 4 * T, hipMemcpyHostToDevice, s));
 
@@ -3978,10 +4061,11 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
                        overflow.p);
     if (read_one(overflow.p, s)) fail(DTE_ERR_INTERNAL, "bootstrap multiplicity above 65535");
   } else {
-    E = (int64_t)T * N;
+    E = 0;
     for (int t = 0; t < T; ++t) {
-      root_start[t] = (int64_t)t * N;
-      root_m[t] = N;
+      root_start[t] = E;
+      root_m[t] = tN[t];
+      E += tN[t];
     }
     rowA.alloc(E);
     auxA.alloc(E);
@@ -3991,6 +4075,45 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       eyA.alloc(E);
       eyB.alloc(E);
     }
+    if (J) {
+      // joint build: each forest's trees hold its own rows, entries of the
+      // forests one after another
+      std::vector<uint32_t> rc((size_t)T * std::max(C, 1), 0u);
+      std::vector<int64_t> rs(T, 0);
+      std::vector<uint64_t> rh(T, 0);
+      DevBuf<uint32_t> c1(std::max(C, 1));
+      DevBuf<int64_t> s1(1);
+      DevBuf<uint64_t> h1(1);
+      for (const JointForest& jf : J->f) {
+        const int64_t off = root_start[jf.t0];
+        const uint32_t* lab = jf.fd->label.p;
+        const int64_t* yf = jf.fd->yfix.p;
+        hipLaunchKernelGGL(k_identity_entries, dim3(grid_for(jf.N, kBlock, 4096), jf.T), dim3(kBlock), 0, s, jf.N,
+                           jf.T, lab, yf, rowA.p + off, auxA.p + off, task == 1 ? eyA.p + off : nullptr);
+        DTE_HIP(hipMemsetAsync(c1.p, 0, c1.bytes(), s));
+        DTE_HIP(hipMemsetAsync(s1.p, 0, 8, s));
+        DTE_HIP(hipMemsetAsync(h1.p, 0, 8, s));
+        hipLaunchKernelGGL(k_root_stats_identity, dim3(grid_for(jf.N, kBlock, 4096)), dim3(kBlock), 0, s, jf.N, lab,
+                           yf, C, c1.p, s1.p, h1.p);
+        std::vector<uint32_t> c1h(std::max(C, 1));
+        int64_t s1h;
+        uint64_t h1h;
+        DTE_HIP(hipMemcpyAsync(c1h.data(), c1.p, c1.bytes(), hipMemcpyDeviceToHost, s));
+        DTE_HIP(hipMemcpyAsync(&s1h, s1.p, 8, hipMemcpyDeviceToHost, s));
+        DTE_HIP(hipMemcpyAsync(&h1h, h1.p, 8, hipMemcpyDeviceToHost, s));
+        DTE_HIP(hipStreamSynchronize(s));
+        for (int q = 0; q < jf.T; ++q) {
+          const int t = jf.t0 + q;
+          rs[t] = s1h;
+          rh[t] = h1h;
+          for (int c = 0; c < std::max(C, 1); ++c) rc[(size_t)t * std::max(C, 1) + c] = c1h[c];
+        }
+      }
+      DTE_HIP(hipMemcpyAsync(root_cls.p, rc.data(), 4 * rc.size(), hipMemcpyHostToDevice, s));
+      DTE_HIP(hipMemcpyAsync(root_sum.p, rs.data(), 8 * T, hipMemcpyHostToDevice, s));
+      DTE_HIP(hipMemcpyAsync(root_hash.p, rh.data(), 8 * T, hipMemcpyHostToDevice, s));
+      DTE_HIP(hipStreamSynchronize(s));  // the host vectors above go out of scope
+    } else {
     hipLaunchKernelGGL(k_identity_entries, dim3(grid_for(N, kBlock, 4096), T), dim3(kBlock), 0, s, N, T, label,
                        yfix, rowA.p, auxA.p, eyA.p);
     DevBuf<uint32_t> c1(std::max(C, 1));
@@ -4016,6 +4139,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     DTE_HIP(hipMemcpyAsync(root_cls.p, rc.data(), 4 * rc.size(), hipMemcpyHostToDevice, s));
     DTE_HIP(hipMemcpyAsync(root_sum.p, rs.data(), 8 * T, hipMemcpyHostToDevice, s));
     DTE_HIP(hipMemcpyAsync(root_hash.p, rh.data(), 8 * T, hipMemcpyHostToDevice, s));
+    }
   }
 
   // ---- level-major node arrays (capacity: a tree has at most 2m-1 nodes).
@@ -4036,10 +4160,10 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     std::vector<uint8_t> op(T);
     for (int t = 0; t < T; ++t) {
       tr[t] = t;
-      cnt[t] = (int32_t)N;
+      cnt[t] = (int32_t)tN[t];
       mm[t] = (int32_t)root_m[t];
       st[t] = root_start[t];
-      op[t] = (N > p.min_node_size && 0 < p.max_depth) ? 1 : 0;
+      op[t] = (tN[t] > tree_min_node(t) && 0 < tree_max_depth(t)) ? 1 : 0;
     }
     DTE_HIP(hipMemcpyAsync(o_tree.p, tr.data(), 4 * T, hipMemcpyHostToDevice, s));
     DTE_HIP(hipMemcpyAsync(o_depth.p, dep.data(), 4 * T, hipMemcpyHostToDevice, s));
@@ -4057,7 +4181,8 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       DTE_HIP(hipMemcpyAsync(rs.data(), root_sum.p, 8 * T, hipMemcpyDeviceToHost, s));
       DTE_HIP(hipStreamSynchronize(s));
       std::vector<float> rv(T);
-      for (int t = 0; t < T; ++t) rv[t] = (float)std::ldexp((double)rs[t] / (double)N, -shift);
+      for (int t = 0; t < T; ++t)
+        rv[t] = (float)std::ldexp((double)rs[t] / (double)tN[t], -(J ? J->f[tf[t]].fd->shift : shift));
       DTE_HIP(hipMemcpyAsync(o_value.p, rv.data(), 4 * T, hipMemcpyHostToDevice, s));
     } else {
       DTE_HIP(hipMemsetAsync(o_value.p, 0, 4 * T, s));
@@ -4069,7 +4194,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   {
     std::vector<int64_t> st(root_start);
     std::vector<uint8_t> op(T);
-    for (int t = 0; t < T; ++t) op[t] = (N > p.min_node_size && 0 < p.max_depth) ? 1 : 0;
+    for (int t = 0; t < T; ++t) op[t] = (tN[t] > tree_min_node(t) && 0 < tree_max_depth(t)) ? 1 : 0;
     DTE_HIP(hipMemcpyAsync(w_start.p, st.data(), 8 * T, hipMemcpyHostToDevice, s));
     DTE_HIP(hipMemcpyAsync(w_open.p, op.data(), T, hipMemcpyHostToDevice, s));
   }
@@ -4225,8 +4350,12 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   DevBuf<unsigned long long> graph_bytes(kMaxTiers + 1);  // search tiers, partition
   int64_t graph_replays = 0;
   auto device_levels = [&](int depth0) {
-    const int64_t min1 = (int64_t)std::max(p.min_node_size, 0) + 1;
-    const int64_t NO = std::max<int64_t>(1, std::min<int64_t>(E, (int64_t)T * N / min1));
+    int64_t min1 = (int64_t)std::max(p.min_node_size, 0) + 1, W = 0;
+    for (int t = 0; t < T; ++t) {
+      W += tN[t];
+      min1 = std::min<int64_t>(min1, (int64_t)std::max(tree_min_node(t), 0) + 1);
+    }
+    const int64_t NO = std::max<int64_t>(1, std::min<int64_t>(E, W / min1));
     const int64_t NL = std::max<int64_t>(n_level, std::min<int64_t>(E, 2 * NO));
     auto fit_to = [](auto& b, int64_t n) {
       if ((int64_t)b.n < n) b.alloc((size_t)n);
@@ -4373,7 +4502,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         phase(retry_list.p, &L->n_retry, L->rtier, R, D0);
         da.retry_list = retry_list.p;
         da.n_retry_dev = &L->n_retry;
-        hipLaunchKernelGGL(k_decide2, dim3(kElemGrid), dim3(kBlock), 0, s, da);
+        hipLaunchKernelGGL(k_decide2, dim3(kElemGrid), dim3(kBlock), 0, s, d, lvd, da);
       }
       // partition
       PartArgs pa{};
@@ -4477,7 +4606,9 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     };
     {
       Timer tm("level_graph", 0.0, s);
-      const int64_t steps = (int64_t)p.max_depth - depth0 + 1;  // levels depth0 .. max_depth
+      int max_depth = p.max_depth;
+      for (int t = 0; t < T; ++t) max_depth = std::max(max_depth, tree_max_depth(t));
+      const int64_t steps = (int64_t)max_depth - depth0 + 1;  // levels depth0 .. max_depth
       if (steps <= 64) {
         replay(std::max<int64_t>(steps, 1));
       } else {
@@ -4633,7 +4764,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
           if (plain) plain_phase(it); else run_phase(it);
           da.retry_list = retry_list.p;
           da.n_retry = n_retry;
-          hipLaunchKernelGGL(k_decide2, dim3(grid_for(n_retry)), dim3(kBlock), 0, s, da);
+          hipLaunchKernelGGL(k_decide2, dim3(grid_for(n_retry)), dim3(kBlock), 0, s, d, lv, da);
         }
       }
       // ---- partition
@@ -4812,7 +4943,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   fo->task = task;
   fo->algorithm = algorithm;
   fo->n_classes = C;
-  fo->n_features = F;
+  fo->n_features = J ? J->F : F;
   fo->n_trees = T;
   fo->fixed_shift = shift;
   fo->seed = p.seed;
@@ -4952,6 +5083,137 @@ Forest* fit_forest(const float* X_dev_in, const float* y_host, int64_t N, int32_
         fit_batch(fd, p.tree_offset + t0, std::min(kPlainTrees, p.n_trees - t0), N, F, p, s, true));
   if (parts.size() == 1) return parts[0].release();
   return merge_forests(parts, s);
+}
+
+// Node range [off, off + n) of a joint build's forest, as a forest of its
+// own: child links relative to the range, class counts narrowed from the
+// joint width Cs to the forest's C.
+struct SliceArgs {
+  int64_t off, n;
+  int Cs, C;
+  const int32_t *feature, *left, *depth, *count, *m;
+  const float *thr, *score, *value;
+  const uint64_t* hash;
+  const uint32_t* cls;
+  const int64_t* sum;
+  int32_t *o_feature, *o_left, *o_depth, *o_count, *o_m;
+  float *o_thr, *o_score, *o_value;
+  uint64_t* o_hash;
+  uint32_t* o_cls;
+  int64_t* o_sum;
+};
+
+__global__ void k_slice_nodes(SliceArgs a) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < a.n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = a.off + i;
+    a.o_feature[i] = a.feature[g];
+    const int32_t l = a.left[g];
+    a.o_left[i] = l < 0 ? -1 : (int32_t)(l - a.off);
+    a.o_depth[i] = a.depth[g];
+    a.o_count[i] = a.count[g];
+    a.o_m[i] = a.m[g];
+    a.o_thr[i] = a.thr[g];
+    a.o_score[i] = a.score[g];
+    a.o_value[i] = a.value[g];
+    a.o_hash[i] = a.hash[g];
+    a.o_sum[i] = a.sum[g];
+    if (a.cls)
+      for (int c = 0; c < a.C; ++c) a.o_cls[i * a.C + c] = a.cls[g * a.Cs + c];
+  }
+}
+
+std::vector<Forest*> fit_joint(const float* const* X, const float* const* y, const int64_t* N, const int32_t* F,
+                               const dte_params* ps, int n, hipStream_t s) {
+  PoolScope pool(s);
+  const int task = ps[0].task;
+  std::vector<std::unique_ptr<FitData>> fds(n);
+  std::vector<DataArgs> da(n);
+  JointSpec J;
+  J.C = 1;
+  J.D0 = J.R = J.F = 0;
+  int T = 0;
+  for (int f = 0; f < n; ++f) {
+    fds[f] = std::make_unique<FitData>();
+    prepare_data(*fds[f], X[f], y[f], N[f], F[f], ps[f], s);
+    const int k = ps[f].n_features_per_node > 0 ? ps[f].n_features_per_node
+                                                : (int)std::round(std::log((double)F[f])) + 1;
+    da[f] = fds[f]->d;
+    da[f].D0 = std::min(k, F[f]);
+    da[f].R = k < F[f] ? std::min(10, F[f] - k) : 0;
+    J.f.push_back(JointForest{fds[f].get(), N[f], F[f], ps[f].n_trees, T, ps[f]});
+    T += ps[f].n_trees;
+    if (task == 0) J.C = std::max(J.C, ps[f].n_classes);
+    J.D0 = std::max(J.D0, da[f].D0);
+    J.R = std::max(J.R, da[f].R);
+    J.F = std::max(J.F, F[f]);
+  }
+  for (auto& a : da) a.C = task == 0 ? J.C : 0;  // joint class width in every forest's data
+  DevBuf<DataArgs> dforests(n);
+  std::vector<int32_t> fot(T);
+  for (const JointForest& jf : J.f)
+    for (int q = 0; q < jf.T; ++q) fot[jf.t0 + q] = (int32_t)(&jf - J.f.data());
+  DevBuf<int32_t> dfot(T);
+  DTE_HIP(hipMemcpyAsync(dforests.p, da.data(), sizeof(DataArgs) * n, hipMemcpyHostToDevice, s));
+  DTE_HIP(hipMemcpyAsync(dfot.p, fot.data(), 4 * (size_t)T, hipMemcpyHostToDevice, s));
+  J.d = da[0];
+  J.d.forests = dforests.p;
+  J.d.forest_of_tree = dfot.p;
+  dte_params pj = ps[0];
+  pj.n_trees = T;
+  pj.n_classes = task == 0 ? J.C : 0;
+  std::unique_ptr<Forest> all(fit_batch(*fds[0], 0, T, N[0], F[0], pj, s, false, &J));
+  // one forest per input, from its trees' node range
+  std::vector<int64_t> ts(T + 1);
+  DTE_HIP(hipMemcpyAsync(ts.data(), all->tree_start.p, 8 * (size_t)(T + 1), hipMemcpyDeviceToHost, s));
+  DTE_HIP(hipStreamSynchronize(s));
+  std::vector<std::unique_ptr<Forest>> out;
+  PoolSuspend own;  // the forests outlive the fit
+  for (const JointForest& jf : J.f) {
+    const int64_t off = ts[jf.t0], nn = ts[jf.t0 + jf.T] - off;
+    const int Cf = task == 0 ? jf.p.n_classes : 1;
+    auto fo = std::make_unique<Forest>();
+    fo->device = all->device;
+    fo->task = task;
+    fo->algorithm = all->algorithm;
+    fo->n_classes = Cf;
+    fo->n_features = jf.F;
+    fo->n_trees = jf.T;
+    fo->fixed_shift = jf.fd->shift;
+    fo->seed = jf.p.seed;
+    fo->tree_offset = jf.p.tree_offset;
+    fo->fit_rows = 0;  // extra trees: no bootstrap record
+    fo->n_nodes = nn;
+    fo->n_leaves = (nn + jf.T) / 2;
+    fo->has_stats = true;
+    std::vector<int64_t> tsf(jf.T + 1);
+    for (int q = 0; q <= jf.T; ++q) tsf[q] = ts[jf.t0 + q] - off;
+    fo->tree_start.alloc(jf.T + 1);
+    DTE_HIP(hipMemcpyAsync(fo->tree_start.p, tsf.data(), 8 * (size_t)(jf.T + 1), hipMemcpyHostToDevice, s));
+    fo->feature.alloc(nn);
+    fo->left.alloc(nn);
+    fo->depth.alloc(nn);
+    fo->count.alloc(nn);
+    fo->n_entries.alloc(nn);
+    fo->threshold.alloc(nn);
+    fo->score.alloc(nn);
+    fo->value.alloc(nn);
+    fo->rowhash.alloc(nn);
+    fo->class_counts.alloc(task == 0 ? (size_t)nn * Cf : 1);
+    fo->sum_fixed.alloc(nn);
+    SliceArgs sa{off, nn, all->n_classes, task == 0 ? Cf : 0,
+                 all->feature.p, all->left.p, all->depth.p, all->count.p, all->n_entries.p,
+                 all->threshold.p, all->score.p, all->value.p, all->rowhash.p,
+                 task == 0 ? all->class_counts.p : nullptr, all->sum_fixed.p,
+                 fo->feature.p, fo->left.p, fo->depth.p, fo->count.p, fo->n_entries.p,
+                 fo->threshold.p, fo->score.p, fo->value.p, fo->rowhash.p, fo->class_counts.p, fo->sum_fixed.p};
+    hipLaunchKernelGGL(k_slice_nodes, dim3(grid_for(nn)), dim3(kBlock), 0, s, sa);
+    build_predict_nodes(*fo, s);
+    DTE_HIP(hipStreamSynchronize(s));  // tsf goes out of scope
+    out.push_back(std::move(fo));
+  }
+  std::vector<Forest*> res;
+  for (auto& f : out) res.push_back(f.release());
+  return res;
 }
 
 void build_predict_nodes(Forest& f, hipStream_t s) {

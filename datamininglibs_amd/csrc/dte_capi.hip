@@ -497,6 +497,22 @@ int dte_fit_many(int32_t n, const float* const* samples_dev, const float* const*
     }
     const int dev = params[0].device_id;
     set_device(dev);
+    // Extra-trees forests of one task, batched: one joint build whose level
+    // steps serve the open nodes of every forest at once.
+    bool joint = n >= 2;
+    for (int32_t i = 0; i < n; ++i)
+      joint = joint && params[i].algorithm == DTE_EXTRA_TREES && params[i].build_mode == DTE_BUILD_BATCHED &&
+              params[i].task == params[0].task;
+    if (joint) {
+      std::vector<Forest*> fs = fit_joint(samples_dev, targets, n_rows, n_features, params, n, library_stream(dev));
+      for (int32_t i = 0; i < n; ++i) {
+        auto* h = new dte_forest();
+        h->f = std::move(*fs[i]);
+        delete fs[i];
+        out[i] = h;
+      }
+      return;
+    }
     const int W = std::max(1, std::min<int>(max_concurrent > 0 ? max_concurrent : 8, n));
     std::vector<hipStream_t> streams(W);
     for (int k = 0; k < W; ++k) streams[k] = worker_stream(dev, k);

@@ -218,6 +218,10 @@ struct FitInput {
 Forest* fit_forest(const float* X_dev, const float* y_host, int64_t n_rows, int32_t n_features,
                    const dte_params& p, hipStream_t stream);
 void build_predict_nodes(Forest& f, hipStream_t stream);
+// Several extra-trees forests of one task in one build: every level step
+// serves the open nodes of all forests (see dte_fit_many).
+std::vector<Forest*> fit_joint(const float* const* X_dev, const float* const* y_host, const int64_t* n_rows,
+                               const int32_t* n_features, const dte_params* params, int n, hipStream_t stream);
 void predict_forest(const Forest& f, const float* X_dev, int64_t n_rows, int64_t ld,
                     int row_major, int32_t* votes_dev, float* sums_dev, hipStream_t stream);
 // In place: int32 vote counts -> the same values as float.

@@ -140,13 +140,17 @@ int dte_fit(const float* samples, const float* targets, int64_t n_rows, int32_t 
 int dte_fit_device(const float* samples_dev, const float* targets, int64_t n_rows,
                    int32_t n_features, const dte_params* params, dte_forest** out);
 
-/* n independent fits, samples resident in HBM (as dte_fit_device), run
- * concurrently: up to max_concurrent (0: 8) host threads, each with its own
- * HIP stream, take the fits in order; every forest is bit-identical to its
- * fit alone.  The many-forests workload of the reference, which fits one
- * model per device thread (lib_algorithms/source/gpu_algorithm.cc:18-31) and
- * otherwise one after another.  All params[i].device_id must be equal.  On
- * failure no forest is returned. */
+/* n independent fits, samples resident in HBM (as dte_fit_device); every
+ * forest is bit-identical to its fit alone.  The many-forests workload of
+ * the reference, which fits one model per device thread
+ * (lib_algorithms/source/gpu_algorithm.cc:18-31) and otherwise one after
+ * another.  When n >= 2 and every fit is extra trees, DTE_BUILD_BATCHED and
+ * of one task, the forests are built jointly: one level step (a replay of a
+ * captured hipGraph) serves the open nodes of every forest at once; rows,
+ * features, classes, draws per node, depth and node-size limits, seeds and
+ * tree offsets may all differ.  Otherwise up to max_concurrent (0: 8) host
+ * threads, each with its own HIP stream, take the fits in order.  All
+ * params[i].device_id must be equal.  On failure no forest is returned. */
 /* Split-search tuning for one data set (the autotuner of the batched
  * build): trial fits of trial_trees trees (0: 16) of *params on these
  * samples (best of two runs each), one candidate setting of cut_team,

@@ -169,3 +169,63 @@ def test_rf_1m_x_128_trees_equal_oracle():
                      seed=324123, n_threads=2)
     assert_same_forest(g, o)
     g.close()
+
+
+def test_joint_build_mixed_forests_equal_alone():
+    """dte_fit_many builds extra-trees forests of one task jointly: every
+    level step serves the open nodes of all forests.  Forests of different
+    rows, features, classes, draws per node, depth limits and minimum node
+    sizes each equal their fit alone and the oracle
+    (lib_algorithms/source/gpu_algorithm.cc:17-31 fits one model per device
+    thread instead)."""
+    rng = np.random.default_rng(41)
+    specs = []
+    for i in range(7):
+        f = int(rng.choice([6, 16, 40, 90]))
+        specs.append(dict(n=int(rng.integers(300, 6000)), f=f, c=int(rng.integers(2, 9)),
+                          k=int(rng.integers(1, min(f, 12))) if i % 2 else 0,
+                          depth=int(rng.choice([4, 12, 1000])), min_node=int(rng.choice([1, 5, 20])),
+                          trees=int(rng.integers(3, 12)), seed=int(rng.integers(1 << 30))))
+    data = [synth.gaussian_classes(seed=sp["seed"], n=sp["n"], f=sp["f"], n_informative=min(sp["f"], 6),
+                                   n_classes=sp["c"]) for sp in specs]
+    params = [capi.default_params(capi.EXTRA_TREES, n_trees=sp["trees"], n_features_per_node=sp["k"],
+                                  max_depth=sp["depth"], min_node_size=sp["min_node"], n_classes=sp["c"],
+                                  seed=sp["seed"], tree_offset=i) for i, sp in enumerate(specs)]
+    devs = [capi.DeviceArray.from_numpy(X) for X, _ in data]
+    joint = capi.fit_many([d.ptr for d in devs], [X.shape[1] for X, _ in data], [X.shape[0] for X, _ in data],
+                          [y for _, y in data], params)
+    for i, ((X, y), d, p, fj) in enumerate(zip(data, devs, params, joint)):
+        alone = capi.fit_device(d.ptr, X.shape[1], X.shape[0], y, p)
+        a, b = fj.canonical(), alone.canonical()
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (i, k)
+        sp = specs[i]
+        o = pyoracle.fit(X, y, algorithm=1, n_trees=sp["trees"], k=sp["k"], max_depth=sp["depth"],
+                         min_node_size=sp["min_node"], n_classes=sp["c"], seed=sp["seed"], n_threads=8)
+        if i == 0:  # the oracle has no tree offset: compare the forest that starts at tree 0
+            assert_same_forest(fj, o)
+        Xp = X[:, : min(500, X.shape[1])]
+        assert np.array_equal(fj.predict(Xp), alone.predict(Xp)), i
+        alone.close()
+        fj.close()
+        d.free()
+
+
+def test_joint_build_regression_equals_alone():
+    specs = [(2500, 8, 3), (900, 20, 5), (4000, 5, 2)]
+    data = [synth.sine_regression(seed=50 + i, n=n, f=f) for i, (n, f, _) in enumerate(specs)]
+    params = [capi.default_params(capi.EXTRA_TREES, task=capi.REGRESSION, n_trees=t, max_depth=15, n_classes=0,
+                                  seed=60 + i) for i, (_, _, t) in enumerate(specs)]
+    devs = [capi.DeviceArray.from_numpy(X) for X, _ in data]
+    joint = capi.fit_many([d.ptr for d in devs], [X.shape[1] for X, _ in data], [X.shape[0] for X, _ in data],
+                          [y for _, y in data], params)
+    for i, ((X, y), d, p, fj) in enumerate(zip(data, devs, params, joint)):
+        alone = capi.fit_device(d.ptr, X.shape[1], X.shape[0], y, p)
+        a, b = fj.canonical(), alone.canonical()
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (i, k)
+        o = pyoracle.fit(X, y, algorithm=1, task=1, n_trees=specs[i][2], max_depth=15, seed=60 + i, n_threads=8)
+        assert_same_forest(fj, o)
+        alone.close()
+        fj.close()
+        d.free()
