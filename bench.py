@@ -181,7 +181,9 @@ def serve_bench(capi, synth, np, n_requests, seed=0):
     return {"forests": "4 x 500 trees, 256 features, 10 classes, depth <= 20, trained on 200,000 rows",
             "requests_per_point": n_requests, "rows_per_request": "log-uniform 1..65536",
             "capacity_rows_per_s": round(cap_rows, 1), "capacity_qps": round(cap_qps, 1), "points": points,
-            "batches": st["batches_launched"], "resident_nodes": nodes, "hbm_in_use_bytes": st["hbm_in_use"]}
+            "batches": st["batches_launched"], "traversal_launches": st["groups_launched"],
+            "batch_time_ms": {k: round(st[k], 1) for k in ("stage_ms", "h2d_ms", "kernel_ms", "d2h_ms", "scatter_ms")},
+            "resident_nodes": nodes, "hbm_in_use_bytes": st["hbm_in_use"]}
 
 
 def other_workload(name, args):
@@ -280,7 +282,8 @@ def other_workload(name, args):
         forests = capi.fit_many(*args)
         dt = time.perf_counter() - t
         capi.profiling(False)
-        replays = capi.profile_stats().get("level_graph_replays", {"launches": 0})["launches"]
+        jprof = capi.profile_stats()
+        replays = jprof.get("level_graph_replays", {"launches": 0})["launches"]
         assert sum(f.n_nodes for f in forests) == nodes
         for f in forests:
             f.close()
@@ -313,6 +316,7 @@ def other_workload(name, args):
                 "gpu_trees_per_s": round(64 * 50 / dt, 1), "gpu_seconds_all": round(dt, 2), "nodes": nodes,
                 "gpu_path": "one joint build of the 64 forests (dte_fit_many), every level step one hipGraph replay",
                 "level_graph_replays": replays,
+                "joint_ms": {k: round(v["total_ms"], 2) for k, v in jprof.items() if v["total_ms"] > 0},
                 "gpu_trees_per_s_8_streams_host_levels": round(64 * 50 / dt_streams, 1),
                 "tuning": {"cut_ert": tuned.cut_ert, "autotune_s": round(tune_s, 2)},
                 "gpu_trees_per_s_one_after_another": round(64 * 50 / dt_seq, 1),

@@ -3976,6 +3976,7 @@ struct JointSpec {
 // device functions compute every result, so both modes give the same bits.
 Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_params& p, hipStream_t s,
                   bool plain, const JointSpec* J = nullptr) {
+  const auto fit_t0 = std::chrono::steady_clock::now();
   const int task = p.task;
   const int algorithm = p.algorithm;
   const int C = J ? J->C : (task == 0 ? p.n_classes : 1);
@@ -4659,6 +4660,10 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     }
   };
   const bool level_graphs = !plain && p.build_mode == DTE_BUILD_BATCHED;
+  if (profiling_enabled()) {
+    DTE_HIP(hipStreamSynchronize(s));
+    profile_host("fit_setup_host", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - fit_t0).count());
+  }
 
   for (int depth = 0; n_level > 0; ++depth) {
     // Random forests switch to the level graphs once the nodes above
@@ -5125,6 +5130,12 @@ __global__ void k_slice_nodes(SliceArgs a) {
 std::vector<Forest*> fit_joint(const float* const* X, const float* const* y, const int64_t* N, const int32_t* F,
                                const dte_params* ps, int n, hipStream_t s) {
   PoolScope pool(s);
+  const bool prof = profiling_enabled();
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+  };
+  const auto h0 = now();
   const int task = ps[0].task;
   std::vector<std::unique_ptr<FitData>> fds(n);
   std::vector<DataArgs> da(n);
@@ -5161,7 +5172,14 @@ std::vector<Forest*> fit_joint(const float* const* X, const float* const* y, con
   dte_params pj = ps[0];
   pj.n_trees = T;
   pj.n_classes = task == 0 ? J.C : 0;
+  if (prof) {
+    DTE_HIP(hipStreamSynchronize(s));
+    profile_host("joint_prepare_host", ms(h0, now()));
+  }
+  const auto h1 = now();
   std::unique_ptr<Forest> all(fit_batch(*fds[0], 0, T, N[0], F[0], pj, s, false, &J));
+  const auto h2 = now();
+  if (prof) profile_host("joint_build_host", ms(h1, h2));
   // one forest per input, from its trees' node range
   std::vector<int64_t> ts(T + 1);
   DTE_HIP(hipMemcpyAsync(ts.data(), all->tree_start.p, 8 * (size_t)(T + 1), hipMemcpyDeviceToHost, s));
@@ -5211,6 +5229,8 @@ std::vector<Forest*> fit_joint(const float* const* X, const float* const* y, con
     DTE_HIP(hipStreamSynchronize(s));  // tsf goes out of scope
     out.push_back(std::move(fo));
   }
+  all.reset();
+  if (prof) profile_host("joint_split_host", ms(h2, now()));
   std::vector<Forest*> res;
   for (auto& f : out) res.push_back(f.release());
   return res;

@@ -280,6 +280,19 @@ hipStream_t library_stream(int device) {
   return it->second;
 }
 
+// Add a host-measured interval (ms) to a family, as one launch.
+void profile_host(const char* name, double ms) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  drain_pending();
+  auto it = std::find_if(g_stats.begin(), g_stats.end(), [&](const KernelStat& k) { return k.name == name; });
+  if (it == g_stats.end()) {
+    g_stats.push_back(KernelStat{name, 0, 0.0, 0.0});
+    it = g_stats.end() - 1;
+  }
+  it->launches += 1;
+  it->total_ms += ms;
+}
+
 // Add a count to a family's launches without a time (e.g. graph replays).
 void profile_count(const char* name, int64_t n) {
   std::lock_guard<std::mutex> lk(g_prof_mu);

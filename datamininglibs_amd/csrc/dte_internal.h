@@ -142,6 +142,7 @@ struct KernelStat {
 void profile_record(const char* name, hipEvent_t a, hipEvent_t b, double bytes);
 void profile_add_bytes(const char* name, double bytes);
 void profile_count(const char* name, int64_t n);
+void profile_host(const char* name, double ms);
 bool profiling_enabled();
 
 // HIP events around the launches of one kernel family on stream s (only
