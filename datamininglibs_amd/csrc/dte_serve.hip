@@ -26,9 +26,13 @@
 // result depends only on its own features and the forest, so a result is
 // the same whatever batch it rides in.
 //
-// A batch records four events (start, rows copied, traversals done, results
-// copied) and the scheduler times its host staging and scatter, so the
-// server reports where a batch's time goes (dte_server_stats).
+// A request's rows are copied once, at submit, into a pinned block from the
+// server's pool; a batch copies large pieces straight from those blocks to
+// the device and stages only small pieces through the slot's pinned buffer,
+// so most bytes cross the host once.  A batch records four events (start,
+// rows copied, traversals done, results copied) and the scheduler times its
+// host staging and scatter, so the server reports where a batch's time goes
+// (dte_server_stats).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,6 +40,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -48,6 +53,77 @@ namespace dte {
 namespace {
 
 constexpr int kSlots = 4;
+// Pieces of at least this many bytes go to the device straight from their
+// request's pinned block; smaller ones are staged into the slot buffer.
+constexpr size_t kDirectBytes = (size_t)256 << 10;
+
+// Pinned host blocks for request rows, reused by size class (powers of two
+// from 64 KiB); at most kCacheCap bytes are kept free.
+struct PinnedPool {
+  static constexpr size_t kMinBlock = (size_t)64 << 10;
+  static constexpr size_t kCacheCap = (size_t)1 << 30;
+  std::mutex mu;
+  std::multimap<size_t, void*> free_blocks;
+  size_t cached = 0;
+  void* acquire(size_t bytes, size_t* got) {
+    size_t want = kMinBlock;
+    while (want < bytes) want <<= 1;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = free_blocks.find(want);
+      if (it != free_blocks.end()) {
+        void* p = it->second;
+        free_blocks.erase(it);
+        cached -= want;
+        *got = want;
+        return p;
+      }
+    }
+    void* p = nullptr;
+    DTE_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+    *got = want;
+    return p;
+  }
+  void release(void* p, size_t bytes) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (cached + bytes <= kCacheCap) {
+        free_blocks.emplace(bytes, p);
+        cached += bytes;
+        return;
+      }
+    }
+    (void)hipHostFree(p);
+  }
+  ~PinnedPool() {
+    for (auto& kv : free_blocks) (void)hipHostFree(kv.second);
+  }
+};
+
+// A request's rows in a pinned block of the pool (returned on destruction).
+struct PinnedRows {
+  PinnedPool* pool = nullptr;
+  float* p = nullptr;
+  size_t bytes = 0;
+  PinnedRows() = default;
+  PinnedRows(const PinnedRows&) = delete;
+  PinnedRows& operator=(const PinnedRows&) = delete;
+  PinnedRows& operator=(PinnedRows&& o) noexcept {
+    if (this != &o) {
+      reset();
+      pool = o.pool;
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+    }
+    return *this;
+  }
+  ~PinnedRows() { reset(); }
+  void reset() {
+    if (p) pool->release(p, bytes);
+    p = nullptr;
+  }
+};
 
 struct Resident {
   Forest f;  // pnodes + tree_start only
@@ -60,7 +136,7 @@ struct Request {
   int64_t n_rows = 0;
   int n_features = 0;
   int width = 1;  // result floats per row
-  std::vector<float> rows;
+  PinnedRows rows;  // n_rows x n_features, row-major
   std::vector<float> out;
   int64_t next_row = 0;  // rows handed to batches
   int64_t rows_done = 0;
@@ -108,6 +184,7 @@ struct dte_server {
   int c_cap = 0;  // result width capacity
   int64_t slot_bytes = 0;
   int64_t backlog_rows_cap = 0;
+  dte::PinnedPool pinned;  // before `requests`: outlives the requests' blocks
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::vector<std::unique_ptr<dte::Resident>> forests;
@@ -210,18 +287,39 @@ bool take_batch(dte_server& s, Slot& sl) {
 // touches a busy slot, requests with pieces in flight stay alive and
 // resident forests are never freed while the server runs).
 void launch_batch(Slot& sl) {
-  const Clock::time_point t0 = Clock::now();
-  for (const Group& g : sl.groups) {
-    const int F = g.forest->f.n_features;
-    for (const Piece& p : sl.pieces) {
-      if (p.req->forest != g.forest) continue;
-      std::memcpy(sl.h_rows + g.x_off + (p.off - g.row0) * F, p.req->rows.data() + p.row0 * F,
-                  sizeof(float) * p.n * F);
+  DTE_HIP(hipEventRecord(sl.ev0, sl.stream));
+  // Pieces lie back to back in the batch's row space, grouped by forest:
+  // small ones are staged into h_rows at their own offsets and go over in
+  // runs, large ones straight from their request's pinned block.
+  double stage = 0.0;
+  int64_t run_lo = -1, run_hi = -1;  // staged run, in floats of the row space
+  auto flush = [&] {
+    if (run_lo >= 0)
+      DTE_HIP(hipMemcpyAsync(sl.d_rows.p + run_lo, sl.h_rows + run_lo, sizeof(float) * (run_hi - run_lo),
+                             hipMemcpyHostToDevice, sl.stream));
+    run_lo = run_hi = -1;
+  };
+  size_t gi = 0;
+  for (const Piece& p : sl.pieces) {
+    while (p.off >= sl.groups[gi].row0 + sl.groups[gi].n) ++gi;
+    const Group& g = sl.groups[gi];
+    const int64_t F = g.forest->f.n_features;
+    const int64_t x = g.x_off + (p.off - g.row0) * F;
+    const size_t bytes = sizeof(float) * (size_t)(p.n * F);
+    const float* src = p.req->rows.p + p.row0 * F;
+    if (bytes >= kDirectBytes) {
+      flush();
+      DTE_HIP(hipMemcpyAsync(sl.d_rows.p + x, src, bytes, hipMemcpyHostToDevice, sl.stream));
+    } else {
+      const Clock::time_point t0 = Clock::now();
+      std::memcpy(sl.h_rows + x, src, bytes);
+      stage += ms_since(t0);
+      if (run_lo < 0) run_lo = x;
+      run_hi = x + p.n * F;
     }
   }
-  sl.stage_ms = ms_since(t0);
-  DTE_HIP(hipEventRecord(sl.ev0, sl.stream));
-  DTE_HIP(hipMemcpyAsync(sl.d_rows.p, sl.h_rows, sizeof(float) * sl.x_floats, hipMemcpyHostToDevice, sl.stream));
+  flush();
+  sl.stage_ms = stage;
   DTE_HIP(hipEventRecord(sl.ev_in, sl.stream));
   for (const Group& g : sl.groups) {
     const Forest& f = g.forest->f;
@@ -268,7 +366,7 @@ void finish_batch(dte_server& s, Slot& sl) {
     r->rows_done += p.n;
     if (r->rows_done == r->n_rows) {
       r->done = true;
-      std::vector<float>().swap(r->rows);
+      r->rows.reset();  // no piece of it is in flight any more
       s.st.requests_done += 1;
       s.st.rows_done += r->n_rows;
     }
@@ -453,7 +551,11 @@ int dte_server_submit(dte_server* s, int32_t slot, const float* rows, int64_t n_
     r->n_features = fr->f.n_features;
     r->width = fr->f.task == 0 ? fr->f.n_classes : 1;
     lk.unlock();
-    r->rows.assign(rows, rows + n_rows * r->n_features);
+    const size_t bytes = sizeof(float) * (size_t)n_rows * r->n_features;
+    select_device(s->device);  // pinned blocks of this server's device
+    r->rows.pool = &s->pinned;
+    r->rows.p = static_cast<float*>(s->pinned.acquire(bytes, &r->rows.bytes));
+    std::memcpy(r->rows.p, rows, bytes);
     r->out.assign((size_t)n_rows * r->width, 0.0f);
     lk.lock();
     Request* raw = r.get();
