@@ -2329,39 +2329,64 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArg
     float acc = 0.0f;
     for (int s = 0; s < 10; ++s) acc = acc + __shfl(xs, s, kWave);
     const float thr = acc / 10.0f;
+    // Left-side counts.  Extra-trees entries weigh 1 (every row once), so a
+    // chunk's class counts are popcounts of ballots: lane c accumulates
+    // class c in lc, the left total is wave-uniform in nl_u.  A chunk with
+    // another weight takes per-lane accumulators reduced at the end.
     uint32_t loc[MAXC];
 #pragma unroll
     for (int c = 0; c < MAXC; ++c) loc[c] = 0;
-    int64_t nl = 0, sl = 0;
-    for (int e = lane; e < m; e += kWave) {
-      const uint32_t row = lv.row[st + e];
-      if (side_of(Xf[row], thr) == 0) {
-        const uint32_t a = lv.aux[st + e];
-        const uint32_t w = a & 0xFFFFu;
-        nl += w;
-        if (TASK == 0) {
-          const int lab = (int)(a >> 16);
+    uint32_t lc = 0;
+    int64_t nl_u = 0, nl_l = 0, sl = 0;
+    bool any_loc = false;  // wave-uniform
+    for (int e0 = 0; e0 < m; e0 += kWave) {
+      const int e = e0 + lane;
+      bool left = false;
+      uint32_t a = 0;
+      if (e < m) {
+        const uint32_t row = lv.row[st + e];
+        left = side_of(Xf[row], thr) == 0;
+        if (left) a = lv.aux[st + e];
+      }
+      const uint32_t w = a & 0xFFFFu;
+      if (TASK == 0) {
+        const int lab = (int)(a >> 16);
+        if (__ballot(left && w != 1u) == 0) {
+          nl_u += __popcll(__ballot(left));
 #pragma unroll
-          for (int c = 0; c < MAXC; ++c) loc[c] += (c == lab) ? w : 0u;
+          for (int c = 0; c < MAXC; ++c)
+            if (c < d.C) {
+              const uint32_t cnt = (uint32_t)__popcll(__ballot(left && lab == c));
+              if (lane == c) lc += cnt;
+            }
         } else {
-          sl += (int64_t)w * lv.ey[st + e];
+          any_loc = true;
+          if (left) {
+            nl_l += w;
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) loc[c] += (c == lab) ? w : 0u;
+          }
         }
+      } else if (left) {
+        nl_l += w;
+        sl += (int64_t)w * lv.ey[st + e];
       }
     }
-    nl = wave_sum_i64(nl);
+    const int64_t nl = nl_u + ((TASK == 1 || any_loc) ? wave_sum_i64(nl_l) : 0);
     if (lane == 0) my_bytes += (unsigned long long)m * (TASK == 0 ? 12ull : 20ull);
     float score;
     if (nl == 0 || nl == n) {
       score = -FLT_MAX;
     } else if (TASK == 0) {
       // lane c keeps class c's left count
-      uint32_t lc = 0;
+      if (any_loc) {
 #pragma unroll
-      for (int c = 0; c < MAXC; ++c) {
-        uint32_t v = loc[c];
+        for (int c = 0; c < MAXC; ++c) {
+          uint32_t v = loc[c];
 #pragma unroll
-        for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-        lc = (lane == c) ? v : lc;
+          for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+          lc += (lane == c) ? v : 0u;
+        }
       }
       const int F = dd.igF;
       const double inv = 1.0 / ((double)n * kLn2);
@@ -3154,13 +3179,18 @@ __global__ void k_identity_entries(int64_t N, int n_trees, const uint32_t* label
 
 __global__ void k_root_stats_identity(int64_t N, const uint32_t* label, const int64_t* yfix, int C,
                                       uint32_t* cls1, int64_t* sum1, uint64_t* hash1) {
+  __shared__ uint32_t s_cls[32];  // C <= 32: counted in LDS, one global atomic per class and block
+  if (threadIdx.x < 32) s_cls[threadIdx.x] = 0u;
+  __syncthreads();
   uint64_t hsh = 0;
   int64_t sm = 0;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
     hsh += mix64((uint64_t)r);
     if (yfix) sm += yfix[r];
-    else atomicAdd(&cls1[label[r]], 1u);
+    else atomicAdd(&s_cls[label[r]], 1u);
   }
+  __syncthreads();
+  if (!yfix && (int)threadIdx.x < C && s_cls[threadIdx.x]) atomicAdd(&cls1[threadIdx.x], s_cls[threadIdx.x]);
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     hsh += __shfl_xor(hsh, off, kWave);
@@ -3170,7 +3200,6 @@ __global__ void k_root_stats_identity(int64_t N, const uint32_t* label, const in
     atomicAdd((unsigned long long*)hash1, (unsigned long long)hsh);
     if (yfix) atomicAdd((unsigned long long*)sum1, (unsigned long long)sm);
   }
-  (void)C;
 }
 
 __global__ void k_canon(float* X, int64_t n) {
