@@ -2415,6 +2415,151 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArg
   if (lane == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
 }
 
+// ERT, one node per SEG-lane segment (nodes of at most SEG entries and at
+// most SEG classes; kWave / SEG nodes per wave).  A node's entries sit one
+// per lane and stay in registers across its draws: a draw takes its ten
+// threshold samples from those lanes by shuffles, so its two gathers (the
+// samples and the entries' values of the drawn feature) are independent,
+// and the gathers of draw q + 1 are issued before draw q is counted and
+// scored.  The draw-independent entropy term of the node is computed once.
+// Every draw's (score, threshold, feature) equals k_search_ert_wave's bit
+// for bit: the same Philox draws, the same sequential fp32 threshold sum,
+// exact integer counts and the same fixed-point entropy sums.
+template <int MAXC, int TASK, int SEG>
+__global__ void __launch_bounds__(kBlock) k_search_ert_seg(DataArgs d, LevelArgs lv, ItemArgs it) {
+  constexpr int NPW = kWave / SEG;  // nodes per wave
+  resolve_level(lv, d.C);
+  resolve_items(it);
+  if ((int64_t)blockIdx.x * kWaves * NPW >= it.n_groups) return;
+  __shared__ double s_inv[128], s_logc[128];
+  load_ln_table(s_inv, s_logc);
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  const int sl = lane & (SEG - 1);
+  const int base = lane & ~(SEG - 1);
+  const unsigned long long segmask = SEG == kWave ? ~0ull : (((1ull << (SEG & 63)) - 1ull) << base);
+  unsigned long long my_bytes = 0;
+  const int64_t wave_id = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
+  const int64_t n_waves = (int64_t)gridDim.x * kWaves;
+  for (int64_t g0 = wave_id * NPW; g0 < it.n_groups; g0 += n_waves * NPW) {
+    const int64_t g = g0 + lane / SEG;
+    const bool node_ok = g < it.n_groups;
+    uint32_t o = 0, i = 0, tg = 0, nid = 0;
+    int m = 0;
+    int64_t st = 0, n = 0, ssum = 0;
+    int tri = 0;
+    if (node_ok) {
+      o = it.list ? it.list[g] : (uint32_t)g;
+      i = lv.open_idx[o];
+      tri = lv.tree[i];
+      tg = (uint32_t)lv.tree_global[tri];
+      nid = lv.id[i];
+      m = lv.m[i];
+      st = lv.start[i];
+      n = lv.count[i];
+      if (TASK == 1) ssum = lv.sum[i];
+    }
+    const DataArgs dd = forest_data(d, tri);
+    const bool valid = node_ok && sl < m;
+    uint32_t row = 0, a = 0;
+    int64_t ey = 0;
+    if (valid) {
+      row = lv.row[st + sl];
+      a = lv.aux[st + sl];
+      if (TASK == 1) ey = lv.ey[st + sl];
+    }
+    const uint32_t w = a & 0xFFFFu;
+    const int lab = (int)(a >> 16);
+    // Extra-trees entries weigh 1 (every row once): counts are popcounts of
+    // ballots.  A wave holding another weight sums weights by shuffles.
+    const bool unit_w = __ballot(valid && w != 1u) == 0;  // wave-uniform
+    const int F = dd.igF;
+    uint32_t nc = 0;
+    int64_t P = 0;
+    double inv = 0.0;
+    if (TASK == 0) {
+      if (node_ok && sl < d.C) nc = lv.cls[(int64_t)i * d.C + sl];
+      P = xlnx_fixed((uint64_t)n, F, s_inv, s_logc) -
+          seg_sum_i64<SEG>(sl < d.C ? xlnx_fixed(nc, F, s_inv, s_logc) : 0);
+      inv = 1.0 / ((double)n * kLn2);
+    }
+    const uint32_t mm = m > 0 ? (uint32_t)m : 1u;
+    // The gathers of draw j (< 0: none): the drawn feature, this lane's
+    // threshold sample (lanes sl < 10) and its entry's value.
+    auto gather = [&](int j, int& f, float& xs, float& xe) {
+      const uint32_t jj = j >= 0 ? (uint32_t)j : 0u;
+      f = (int)(rand_u32(dd.seed, kStreamFeat, tg, nid, (uint64_t)jj) % (uint32_t)dd.F);
+      const uint32_t pos =
+          rand_u32(dd.seed, kStreamThr, tg, nid, (uint64_t)jj * 10 + (uint32_t)(sl < 10 ? sl : 0)) % mm;
+      const uint32_t rp = (uint32_t)__shfl((int)row, base + (int)pos, kWave);
+      const float* Xf = dd.X + (int64_t)f * dd.N;
+      xs = (j >= 0 && node_ok && sl < 10) ? Xf[rp] : 0.0f;
+      xe = (j >= 0 && valid) ? Xf[row] : 0.0f;
+    };
+    auto draw_of = [&](int q) { return (node_ok && q < it.per_group) ? item_draw(d, dd, it, q) : -1; };
+    int j_c = draw_of(0), f_c = 0;
+    float xs_c = 0.0f, xe_c = 0.0f;
+    gather(j_c, f_c, xs_c, xe_c);
+    for (int q = 0; q < it.per_group; ++q) {
+      const int j_n = draw_of(q + 1);
+      int f_n = 0;
+      float xs_n = 0.0f, xe_n = 0.0f;
+      if (q + 1 < it.per_group) gather(j_n, f_n, xs_n, xe_n);
+      float acc = 0.0f;
+      for (int s = 0; s < 10; ++s) acc = acc + __shfl(xs_c, base + s, kWave);
+      const float thr = acc / 10.0f;
+      const bool left = valid && j_c >= 0 && side_of(xe_c, thr) == 0;
+      int64_t nl, sls = 0;
+      uint32_t lc = 0;
+      if (unit_w) {
+        nl = __popcll(__ballot(left) & segmask);
+      } else {
+        nl = seg_sum_i64<SEG>(left ? (int64_t)w : 0);
+      }
+      if (TASK == 0) {
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c)
+          if (c < d.C) {
+            uint32_t cnt;
+            if (unit_w)
+              cnt = (uint32_t)__popcll(__ballot(left && lab == c) & segmask);
+            else
+              cnt = (uint32_t)seg_sum_i64<SEG>((left && lab == c) ? (int64_t)w : 0);
+            if (sl == c) lc = cnt;
+          }
+      } else {
+        sls = seg_sum_i64<SEG>(left ? (int64_t)w * ey : 0);
+      }
+      float score;
+      if (TASK == 0) {
+        int64_t tl = 0, tr = 0;
+        if (sl < d.C) {
+          tl = xlnx_fixed(lc, F, s_inv, s_logc);
+          tr = xlnx_fixed(nc - lc, F, s_inv, s_logc);
+        }
+        const int64_t Q = (seg_sum_i64<SEG>(tl) - xlnx_fixed((uint64_t)nl, F, s_inv, s_logc)) +
+                          (seg_sum_i64<SEG>(tr) - xlnx_fixed((uint64_t)(n - nl), F, s_inv, s_logc));
+        score = (float)ig_from_fixed(P + Q, F, inv);
+      } else {
+        score = (float)variance_gain(nl, sls, n - nl, ssum - sls, dd.shift);
+      }
+      if (nl == 0 || nl == n) score = -FLT_MAX;
+      if (sl == 0 && j_c >= 0) {
+        const int64_t ridx = (int64_t)o * it.D + j_c;
+        it.res_score[ridx] = score;
+        it.res_thr[ridx] = thr;
+        it.res_feat[ridx] = f_c;
+        my_bytes += (unsigned long long)m * (TASK == 0 ? 12ull : 20ull);
+      }
+      j_c = j_n;
+      f_c = f_n;
+      xs_c = xs_n;
+      xe_c = xe_n;
+    }
+  }
+  if (my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
+}
+
 // Split a list of open positions (or all of them) by node size into the
 // wave tier and the block tier.  Order inside a tier is irrelevant: every
 // item writes its own result slot.
@@ -3497,7 +3642,8 @@ T read_one(const T* p, hipStream_t s) {
 const char* search_family(int algorithm, int tier) {
   static const char* rf[] = {"search_seg16", "search_seg32", "search_seg64", "search_team64", "search_block",
                              "search_chunked", "search_block"};
-  static const char* ert[] = {"search_ert_wave", "search_ert_block"};
+  static const char* ert[] = {"search_ert_seg16", "search_ert_seg32", "search_ert_seg64", "search_ert_wave",
+                              "search_ert_block"};
   return algorithm == DTE_RANDOM_FOREST ? rf[tier] : ert[tier];
 }
 
@@ -3505,7 +3651,8 @@ const char* search_family(int algorithm, int tier) {
 // draws looped in the kernel); 3 = wave teams (four items per workgroup);
 // 4 = whole workgroup per item within the LDS capacity; 5 = the chunk
 // pipeline (big_pipeline); 6 = whole workgroup per item of any size (the
-// plain driver).  ERT tiers: 0 = one wave per item, 1 = workgroup per item.
+// plain driver).  ERT tiers: 0/1/2 = one 16/32/64-lane segment per node
+// (draws looped in the kernel), 3 = one wave per item, 4 = workgroup per item.
 template <int MAXC, int TASK>
 void launch_search(int algorithm, int tier, int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv,
                    const ItemArgs& it) {
@@ -3519,10 +3666,13 @@ void launch_search(int algorithm, int tier, int grid, hipStream_t s, const DataA
       default: hipLaunchKernelGGL((k_search_rf<MAXC, TASK, 256, true>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
     }
   } else {
-    if (tier == 0)
-      hipLaunchKernelGGL((k_search_ert_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
-    else
-      hipLaunchKernelGGL((k_search_ert<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it);
+    switch (tier) {
+      case 0: hipLaunchKernelGGL((k_search_ert_seg<MAXC, TASK, 16>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 1: hipLaunchKernelGGL((k_search_ert_seg<MAXC, TASK, 32>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 2: hipLaunchKernelGGL((k_search_ert_seg<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 3: hipLaunchKernelGGL((k_search_ert_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      default: hipLaunchKernelGGL((k_search_ert<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+    }
   }
 }
 
@@ -4280,7 +4430,10 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     const int c32 = (task == 1 || C <= 32) ? 32 : 0;
     cuts = Cuts{{c16, std::max(c16, c32), 64, cut_team, cut_block, 0x7FFFFFFF}, 6};
   } else {
-    cuts = Cuts{{p.cut_ert ? p.cut_ert : kDefCutErt, 0x7FFFFFFF, 0, 0, 0, 0}, 2};
+    // segment tiers need a lane per class
+    const int c16 = (task == 1 || C <= 16) ? 16 : 0;
+    const int c32 = (task == 1 || C <= 32) ? 32 : 0;
+    cuts = Cuts{{c16, std::max(c16, c32), 64, p.cut_ert ? p.cut_ert : kDefCutErt, 0x7FFFFFFF, 0}, 5};
   }
   DevBuf<uint32_t> tier_list[kMaxTiers];
   DevBuf<unsigned int> tier_cnt(kMaxTiers);
@@ -4308,14 +4461,14 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       a.n_groups = cnt[tier];
       a.bytes = it.bytes ? it.bytes + tier : nullptr;
       const bool rf = algorithm == DTE_RANDOM_FOREST;
-      trace_tier[rf ? (tier <= 2 ? 0 : (tier == 3 ? 1 : 2)) : (tier == 0 ? 0 : 2)] += cnt[tier];
+      trace_tier[rf ? (tier <= 2 ? 0 : (tier == 3 ? 1 : 2)) : (tier <= 2 ? 0 : (tier == 3 ? 1 : 2))] += cnt[tier];
       if (rf && tier == 5) {
         Timer tk(search_family(algorithm, tier), 0.0, s);
         big(bigst, d, lv, a, tier_list[5].p, cnt[5], s, num_cu);
         continue;
       }
       int grid;
-      if (rf && tier <= 3) {
+      if (tier <= (rf ? 3 : 2)) {
         // node-major tiers: 4 / 2 / 1 / 1 nodes per wave
         const int npw = tier == 0 ? 4 : (tier == 1 ? 2 : 1);
         grid = (int)std::min<int64_t>((cnt[tier] + npw * kWaves - 1) / (npw * kWaves), wave_grid_cap);
@@ -4323,7 +4476,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         grid = (int)std::min<int64_t>(cnt[tier], search_grid_cap);  // one node per workgroup
       } else {
         const int64_t items = (int64_t)cnt[tier] * it.per_group;
-        const bool whole = rf ? tier == 4 : tier == 1;
+        const bool whole = tier == 4;
         const int per_block = whole ? 1 : kWaves;
         const int cap = whole ? search_grid_cap : wave_grid_cap;
         grid = (int)std::min<int64_t>((items + per_block - 1) / per_block, cap);
@@ -4342,7 +4495,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   };
   auto plain_phase = [&](ItemArgs it) {
     const uint32_t* lst = it.list ? it.list : identity_list(it.n_groups);
-    const int tier = algorithm == DTE_RANDOM_FOREST ? 6 : 1;  // RF: workgroup per node, any size
+    const int tier = algorithm == DTE_RANDOM_FOREST ? 6 : 4;  // workgroup per node, any size
     Timer tm("split_search", 0.0, s);
     for (int64_t c0 = 0; c0 < it.n_groups; c0 += kPlainChunk) {
       ItemArgs a = it;
@@ -4507,7 +4660,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
           a.per_group = per_group;
           a.j0 = j0;
           a.bytes = graph_bytes.p + tier;
-          const int grid = rf ? (tier <= 3 ? wave_grid_cap : search_grid_cap) : (tier == 0 ? wave_grid_cap : search_grid_cap);
+          const int grid = tier <= 3 ? wave_grid_cap : search_grid_cap;
           search(algorithm, rf && tier == 5 ? 6 : tier, grid, s, d, lvd, a);
         }
       };
