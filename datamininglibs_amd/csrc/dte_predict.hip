@@ -22,12 +22,23 @@ namespace {
 constexpr int kRowsPerBlock = 64;
 constexpr int kWave = 64;
 
+// One visited node = one wide load.  Left alone, the compiler splits the
+// record into dword loads and sinks the threshold's load past the feature's
+// LDS lookup: two memory round trips per level instead of one.  The empty
+// asm statement (no instruction) takes the three fields as operands, so
+// their loads stay together and merge into one.
+__device__ __forceinline__ PNode load_node(const PNode* __restrict__ nodes, int64_t i) {
+  const int4 v = reinterpret_cast<const int4*>(nodes)[i];
+  asm volatile("" : : "v"(v.x), "v"(v.y), "v"(v.z));
+  return PNode{v.x, __int_as_float(v.y), v.z, v.w};
+}
+
 __device__ __forceinline__ int walk(const PNode* __restrict__ nodes, int64_t root, const float* __restrict__ X,
                                     int64_t row, int64_t ld, int row_major, float* leaf_value) {
-  PNode nd = nodes[root];
+  PNode nd = load_node(nodes, root);
   while (nd.feature >= 0) {
     const float x = row_major ? X[row * ld + nd.feature] : X[(int64_t)nd.feature * ld + row];
-    nd = nodes[nd.left + side_of(x, nd.thr)];
+    nd = load_node(nodes, nd.left + side_of(x, nd.thr));
   }
   *leaf_value = nd.thr;
   return nd.left;
@@ -99,8 +110,8 @@ __global__ void __launch_bounds__(kBlock) k_predict_cls_lds(const PNode* __restr
   if (lane < nr) {
     const float* xr = s_x + lane * FP;
     for (int t = t0 + group; t < t1; t += kBlock / R) {
-      PNode nd = nodes[tree_start[t]];
-      while (nd.feature >= 0) nd = nodes[nd.left + side_of(xr[nd.feature], nd.thr)];
+      PNode nd = load_node(nodes, tree_start[t]);
+      while (nd.feature >= 0) nd = load_node(nodes, nd.left + side_of(xr[nd.feature], nd.thr));
       atomicAdd(&s_votes[lane * C + nd.left], 1u);
     }
   }
@@ -164,8 +175,8 @@ __global__ void __launch_bounds__(kBlock) k_predict_reg_lds(const PNode* __restr
     const int c1 = min(T, c0 + kRegChunk);
     if (lane < nr)
       for (int t = c0 + group; t < c1; t += kBlock / R) {
-        PNode nd = nodes[tree_start[t]];
-        while (nd.feature >= 0) nd = nodes[nd.left + side_of(xr[nd.feature], nd.thr)];
+        PNode nd = load_node(nodes, tree_start[t]);
+        while (nd.feature >= 0) nd = load_node(nodes, nd.left + side_of(xr[nd.feature], nd.thr));
         s_val[lane * kRegChunk + (t - c0)] = nd.thr;
       }
     __syncthreads();
@@ -224,12 +235,12 @@ __global__ void __launch_bounds__(kBlock) k_oob(const PNode* __restrict__ nodes,
   double e2 = 0.0;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
     if ((bt[r >> 5] >> (r & 31)) & 1u) continue;  // in the bag
-    PNode nd = nodes[root];
+    PNode nd = load_node(nodes, root);
     uint32_t depth = 0;
     while (nd.feature >= 0) {
       int64_t src = r;
       if (nd.feature == perm) src = rand_u32(seed, kStreamImp, tg, (uint32_t)r, depth) % (uint32_t)N;
-      nd = nodes[nd.left + side_of(X[(int64_t)nd.feature * N + src], nd.thr)];
+      nd = load_node(nodes, nd.left + side_of(X[(int64_t)nd.feature * N + src], nd.thr));
       ++depth;
     }
     ++n_pairs;
@@ -287,7 +298,7 @@ __global__ void __launch_bounds__(kBlock) k_oob_lds(const PNode* __restrict__ no
     for (int t = t0 + group; t < t1; t += kBlock / R) {
       if ((bits[(int64_t)t * words + (r >> 5)] >> (r & 31)) & 1u) continue;  // in the bag
       const uint32_t tg = (uint32_t)(tree0 + t);
-      PNode nd = nodes[tree_start[t]];
+      PNode nd = load_node(nodes, tree_start[t]);
       uint32_t depth = 0;
       while (nd.feature >= 0) {
         float v;
@@ -295,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) k_oob_lds(const PNode* __restrict__ no
           v = X[(int64_t)perm * N + rand_u32(seed, kStreamImp, tg, (uint32_t)r, depth) % (uint32_t)N];
         else
           v = xr[nd.feature];
-        nd = nodes[nd.left + side_of(v, nd.thr)];
+        nd = load_node(nodes, nd.left + side_of(v, nd.thr));
         ++depth;
       }
       ++n_pairs;
