@@ -1073,6 +1073,7 @@ struct V2Item {
   uint32_t width;  // bucket = key / width (v2_bucket)
   uint32_t magic;  // floor(2^32 / width)
   uint32_t node;  // level node index
+  int32_t sgrp;   // buckets per super-bucket of the two-pass scatter (0: one pass)
   int64_t ridx;   // result slot (open position * D + draw)
   // node context (as node_context computes it)
   int64_t n_w;    // sum of weights
@@ -1111,6 +1112,7 @@ struct V2Args {
   int64_t* bsum;    // target sum (regression)
   uint32_t* bbase;  // scratch offset within the item
   uint32_t* bcur;
+  uint32_t* bsup;   // two-pass scatter: write cursor of each super-bucket [boff + s]
   uint32_t* carry_cls;
   int64_t* carry_n;
   int64_t* carry_s;
@@ -1126,6 +1128,7 @@ struct V2Args {
   uint32_t* gkey;   // keys in entry order (histogram -> scatter; aliases skey2)
   uint32_t* saux2;
   int64_t* sy2;
+  uint32_t* tkey;   // two-pass scatter: keys in super-bucket order (aux, y in saux2, sy2)
   // task lists
   V2Task* tasks;
   V2Task* spill;
@@ -1263,6 +1266,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_scan(V2Args a, const uint32_t* li
     const int64_t gb = gb0 + q;
     a.bbase[gb] = off;
     a.bcur[gb] = off;
+    if (it.sgrp > 0 && (b0 + q) % it.sgrp == 0) a.bsup[(int64_t)it.boff + (b0 + q) / it.sgrp] = off;
     a.carry_last[gb] = last;
     off += cnt[q];
     if (!cnt[q]) continue;
@@ -1281,10 +1285,21 @@ __global__ void __launch_bounds__(kBlock) k_v2_scan(V2Args a, const uint32_t* li
 // so that consecutive threads write consecutive slots of a run.
 // Items of at most kV2FewB (kV2MidB) buckets take the scatter variants with
 // small bucket tables (less LDS: three workgroups per CU instead of two).
+//
+// Items with more than kV2MidB buckets hold about one entry per bucket in a
+// tile, or a fraction of one, so a one-pass scatter writes runs of about one
+// entry: every write touches its own cache line.  They go in two passes
+// over the same tiles (MODE 1, then MODE 2): first into kV2Super
+// super-buckets of sgrp consecutive buckets each (runs of tens of entries,
+// into tkey / saux2 / sy2), then each tile of that super-bucket order into
+// its buckets -- a tile now spans one or two super-buckets, so its runs are
+// long again.
+// Entry order inside a bucket is not used by the evaluation (it sorts).
 constexpr int kV2FewB = 512;
 constexpr int kV2MidB = 1024;
+constexpr int kV2Super = 64;
 
-template <int TASK, int MAXB>
+template <int TASK, int MAXB, int MODE = 0>
 __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv, V2Args a) {
   constexpr int TILE = TASK == 0 ? kScatterTile : 2048;
   constexpr int U = TILE / kBlock;
@@ -1298,11 +1313,26 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
   const V2Tile tl = a.tiles[blockIdx.x];
   const V2Item it = a.items[tl.item];
   const int t = threadIdx.x;
-  for (int b = t; b < it.nb; b += kBlock) s_cnt[b] = 0;
-  __syncthreads();
   const int n = min(TILE, it.m - (int)tl.e0);
-  const int64_t e0 = it.start + tl.e0;
-  const uint32_t* gkey = a.gkey + it.soff + tl.e0;
+  // input: keys in entry order (MODE 0 / 1) or in super-bucket order (MODE 2)
+  const uint32_t* gkey = (MODE == 2 ? a.tkey : a.gkey) + it.soff + tl.e0;
+  const uint32_t* gaux = MODE == 2 ? a.saux2 + it.soff + tl.e0 : lv.aux + it.start + tl.e0;
+  const int64_t* gy = MODE == 2 ? a.sy2 + it.soff + tl.e0 : lv.ey + it.start + tl.e0;
+  // table slots: bucket (MODE 0), super-bucket (MODE 1), bucket - b_lo (MODE 2)
+  int b_lo = 0, n_slots = it.nb;
+  if (MODE == 1) n_slots = (it.nb + it.sgrp - 1) / it.sgrp;
+  if (MODE == 2) {
+    const int s_first = (int)v2_bucket(gkey[0], it) / it.sgrp;
+    const int s_last = (int)v2_bucket(gkey[n - 1], it) / it.sgrp;
+    b_lo = s_first * it.sgrp;
+    n_slots = min(it.nb, (s_last + 1) * it.sgrp) - b_lo;
+  }
+  auto slot_of = [&](uint32_t key) -> int {
+    const int b = (int)v2_bucket(key, it);
+    return MODE == 1 ? b / it.sgrp : b - b_lo;
+  };
+  for (int b = t; b < n_slots; b += kBlock) s_cnt[b] = 0;
+  __syncthreads();
   uint32_t keys[U], auxv[U], rank[U];
   int64_t yv[U];
 #pragma unroll
@@ -1310,31 +1340,34 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
     const int e = t + q * kBlock;
     if (e < n) {
       keys[q] = gkey[e];
-      auxv[q] = lv.aux[e0 + e];
-      if (TASK == 1) yv[q] = lv.ey[e0 + e];
+      auxv[q] = gaux[e];
+      if (TASK == 1) yv[q] = gy[e];
     }
   }
 #pragma unroll
   for (int q = 0; q < U; ++q) {
     const int e = t + q * kBlock;
-    if (e < n) rank[q] = atomicAdd(&s_cnt[v2_bucket(keys[q], it)], 1u);
+    if (e < n) rank[q] = atomicAdd(&s_cnt[slot_of(keys[q])], 1u);
   }
   __syncthreads();
   {
-    constexpr int PER = MAXB / kBlock;
+    // slots in chunks of PER per thread (several rounds for a MODE 2 table)
+    constexpr int PER = MAXB / kBlock > 0 ? MAXB / kBlock : 1;
+    uint32_t* cur = MODE == 1 ? a.bsup : a.bcur;
+    const int64_t cbase = (int64_t)it.boff + (MODE == 2 ? b_lo : 0);
     const int b0 = t * PER;
     uint32_t c[PER], loc = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      c[q] = b0 + q < it.nb ? s_cnt[b0 + q] : 0u;
+      c[q] = b0 + q < n_slots ? s_cnt[b0 + q] : 0u;
       loc += c[q];
     }
     uint32_t off = team_excl_scan<256>(loc, s_w, nullptr);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      if (b0 + q >= it.nb) break;
-      s_cnt[b0 + q] = off;  // local offset of the bucket's run
-      s_gpos[b0 + q] = c[q] ? atomicAdd(&a.bcur[(int64_t)it.boff + b0 + q], c[q]) : 0u;
+      if (b0 + q >= n_slots) break;
+      s_cnt[b0 + q] = off;  // local offset of the slot's run
+      s_gpos[b0 + q] = c[q] ? atomicAdd(&cur[cbase + b0 + q], c[q]) : 0u;
       off += c[q];
     }
   }
@@ -1343,7 +1376,7 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
   for (int q = 0; q < U; ++q) {
     const int e = t + q * kBlock;
     if (e < n) {
-      const uint32_t b = v2_bucket(keys[q], it);
+      const int b = slot_of(keys[q]);
       const uint32_t li = s_cnt[b] + rank[q];
       s_key[li] = keys[q];
       s_aux[li] = auxv[q];
@@ -1352,12 +1385,15 @@ __global__ void __launch_bounds__(kBlock) k_v2_scatter(DataArgs d, LevelArgs lv,
     }
   }
   __syncthreads();
+  uint32_t* okey = MODE == 1 ? a.tkey : a.skey;
+  uint32_t* oaux = MODE == 1 ? a.saux2 : a.saux;
+  int64_t* oy = MODE == 1 ? a.sy2 : a.sy;
   for (int i = t; i < n; i += kBlock) {
     const uint32_t b = s_b[i];
     const int64_t pos = it.soff + s_gpos[b] + (uint32_t)i - s_cnt[b];
-    a.skey[pos] = s_key[i];
-    a.saux[pos] = s_aux[i];
-    if (TASK == 1) a.sy[pos] = s_y[i];
+    okey[pos] = s_key[i];
+    oaux[pos] = s_aux[i];
+    if (TASK == 1) oy[pos] = s_y[i];
   }
 }
 
@@ -1798,6 +1834,7 @@ __global__ void k_v2_items(DataArgs d, LevelArgs lv, const uint32_t* groups, int
     b.soff = 0;
     b.boff = 0;
     b.node = i;
+    b.sgrp = 0;
     b.ridx = (int64_t)o * D + j;
     b.n_w = lv.count[i];
     b.P = 0;
@@ -3701,10 +3738,10 @@ void launch_partition_seg(int grid_cap, hipStream_t s, const DataArgs& d, const 
 struct BigState {
   DevBuf<V2Item> items;
   DevBuf<V2Tile> htiles, stiles, ctiles;
-  DevBuf<uint32_t> bcount, bmax, bcls, bwn, bbase, bcur, carry_cls, carry_last, res_key, res_prev;
+  DevBuf<uint32_t> bcount, bmax, bcls, bwn, bbase, bcur, bsup, carry_cls, carry_last, res_key, res_prev;
   DevBuf<int64_t> bsum, carry_n, carry_s;
   DevBuf<double> res_g;
-  DevBuf<uint32_t> skey, saux, skey2, saux2;
+  DevBuf<uint32_t> skey, saux, skey2, saux2, tkey;
   DevBuf<int64_t> sy, sy2;
   DevBuf<V2Task> tasks, spill;
   DevBuf<unsigned int> ctr;
@@ -3751,6 +3788,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
       bi.soff = el;
       const uint32_t li = (uint32_t)(k1 - k0);
       auto& st = bi.nb <= kV2FewB ? B.host_stiles : (bi.nb <= kV2MidB ? B.host_stiles_mid : B.host_stiles_many);
+      bi.sgrp = bi.nb > kV2MidB ? (bi.nb + kV2Super - 1) / kV2Super : 0;  // two-pass scatter
       for (int e0 = 0; e0 < bi.m; e0 += kSTile) st.push_back(V2Tile{li, (uint32_t)e0});
       for (int b0 = 0; b0 < bi.nb; b0 += kV2CTile) B.host_ctiles.push_back(V2Tile{li, (uint32_t)b0});
       el += bi.m;
@@ -3812,6 +3850,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     B.bmax.reserve(nb);
     B.bbase.reserve(nb);
     B.bcur.reserve(nb);
+    B.bsup.reserve(nb);
     B.carry_n.reserve(nb);
     B.carry_last.reserve(nb);
     B.res_g.reserve(nb);
@@ -3832,6 +3871,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     B.saux.reserve(el);
     B.skey2.reserve(el);
     B.saux2.reserve(el);
+    if (n_sall > n_smid) B.tkey.reserve(el);
     if (TASK == 1) {
       B.sy.reserve(el);
       B.sy2.reserve(el);
@@ -3849,6 +3889,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     a.bsum = B.bsum.p;
     a.bbase = B.bbase.p;
     a.bcur = B.bcur.p;
+    a.bsup = B.bsup.p;
     a.carry_cls = B.carry_cls.p;
     a.carry_n = B.carry_n.p;
     a.carry_s = B.carry_s.p;
@@ -3863,6 +3904,7 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     a.gkey = B.skey2.p;
     a.saux2 = B.saux2.p;
     a.sy2 = B.sy2.p;
+    a.tkey = B.tkey.p;
     a.tasks = B.tasks.p;
     a.spill = B.spill.p;
     a.counters = B.ctr.p;
@@ -3890,9 +3932,13 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
         hipLaunchKernelGGL((k_v2_scatter<TASK, kV2MidB>), dim3((unsigned)(n_smid - n_sfew)), dim3(kBlock), 0, s, d,
                            lv, a);
       a.tiles = B.stiles.p + n_smid;
-      if (n_sall > n_smid)
-        hipLaunchKernelGGL((k_v2_scatter<TASK, kV2MaxB>), dim3((unsigned)(n_sall - n_smid)), dim3(kBlock), 0, s, d,
-                           lv, a);
+      if (n_sall > n_smid) {
+        // many buckets: super-buckets, then buckets, over the same tiles
+        hipLaunchKernelGGL((k_v2_scatter<TASK, kV2Super, 1>), dim3((unsigned)(n_sall - n_smid)), dim3(kBlock), 0, s,
+                           d, lv, a);
+        hipLaunchKernelGGL((k_v2_scatter<TASK, kV2MaxB, 2>), dim3((unsigned)(n_sall - n_smid)), dim3(kBlock), 0, s,
+                           d, lv, a);
+      }
     }
     a.tiles = B.ctiles.p;
     {

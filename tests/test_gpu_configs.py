@@ -107,6 +107,20 @@ def test_regression_plain_equals_batched_at_scale():
         assert np.array_equal(a[k], b[k]), k
 
 
+def test_regression_1m_two_trees_vs_oracle():
+    """Root nodes of ~632k entries: more than 1024 buckets per (node, draw)
+    item, so the chunk pipeline scatters them in two passes (super-buckets,
+    then buckets).  Both trees equal the oracle's bit for bit."""
+    X, y = synth.sine_regression(seed=3, n=1_000_000, f=64)
+    p = capi.default_params(capi.RANDOM_FOREST, task=capi.REGRESSION, n_trees=2, n_features_per_node=21,
+                            max_depth=12, min_node_size=5, n_classes=0, seed=11)
+    g = capi.fit(X, y, p)
+    o = pyoracle.fit(X, y, algorithm=0, task=1, n_trees=2, k=21, max_depth=12, min_node_size=5, seed=11,
+                     n_threads=2)
+    assert_same_forest(g, o)
+    g.close()
+
+
 def test_64_uneven_extra_trees_forests():
     specs = synth.uneven_forests(seed=0)
     smallest = set(s["index"] for s in sorted(specs, key=lambda s: s["n"] * s["f"])[:8])
