@@ -2929,6 +2929,205 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
   if (t == 0 && my_bytes && pa.bytes) atomicAdd(pa.bytes, my_bytes);
 }
 
+// Nodes above kPartBig entries (the top levels, where a workgroup per node
+// leaves most CUs idle): the same stable partition over chunks of
+// kPartChunk entries, each chunk a workgroup.
+//   1. k_part_chunk_stats    per chunk: left count, left statistics
+//                            (atomics into the node's accumulators)
+//   2. k_part_chunk_finish   per node: the node's results, left offset of
+//                            every chunk (exclusive scan of the chunks'
+//                            left counts)
+//   3. k_part_chunk_scatter  per chunk: stable two-way compaction at the
+//                            chunk's offsets (lefts before it on the left,
+//                            rights before it after the node's mL lefts)
+// Sums of integers, so the statistics equal k_partition's bit for bit, and
+// entries land where k_partition puts them.
+constexpr int kPartBig = 32768;
+constexpr int kPartChunk = 8192;
+
+struct PartChunks {
+  const uint32_t* list;  // big-node open positions
+  int64_t n_nodes;
+  const uint32_t* coff;  // first chunk of each big node (n_nodes + 1)
+  uint32_t* chunk_ml;    // left entries of each chunk, then their exclusive scan per node
+  uint32_t* acc_ml;      // per big node
+  unsigned long long* acc_cnt;
+  unsigned long long* acc_hash;
+  unsigned long long* acc_sum;
+  uint32_t* acc_cls;     // [node * C + c]
+};
+
+__global__ void k_part_nchunks(const uint32_t* list, int64_t n, const uint32_t* open_idx, const int32_t* node_m,
+                               uint32_t* nch) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    nch[q] = (uint32_t)((node_m[open_idx[list[q]]] + kPartChunk - 1) / kPartChunk);
+}
+
+// Big node of chunk k: the last q with coff[q] <= k.
+__device__ __forceinline__ int64_t part_chunk_node(const PartChunks& pc, int64_t k) {
+  int64_t lo = 0, hi = pc.n_nodes - 1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if ((int64_t)pc.coff[mid] <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_part_chunk_stats(DataArgs d, LevelArgs lv, PartArgs pa, PartChunks pc) {
+  __shared__ uint32_t s_cls[MAXC];
+  __shared__ unsigned long long s_cnt, s_sum, s_hash;
+  __shared__ uint32_t s_ml;
+  const int t = threadIdx.x;
+  const int64_t k = blockIdx.x;
+  const int64_t q = part_chunk_node(pc, k);
+  const uint32_t o = pc.list[q];
+  const uint32_t i = lv.open_idx[o];
+  const int m = lv.m[i];
+  const int e0 = (int)(k - pc.coff[q]) * kPartChunk;
+  const int e1 = min(m, e0 + kPartChunk);
+  const int64_t st = lv.start[i];
+  const DataArgs dd = forest_data(d, lv.tree[i]);
+  const float* Xf = dd.X + (int64_t)pa.ch_feat[o] * dd.N;
+  const float thr = pa.ch_thr[o];
+  if (t < MAXC) s_cls[t] = 0;
+  if (t == 0) {
+    s_cnt = 0;
+    s_sum = 0;
+    s_hash = 0;
+    s_ml = 0;
+  }
+  __syncthreads();
+  uint32_t loc[MAXC];
+#pragma unroll
+  for (int c = 0; c < MAXC; ++c) loc[c] = 0;
+  uint64_t cnt = 0, hsh = 0;
+  int64_t sm = 0;
+  uint32_t ml = 0;
+  for (int e = e0 + t; e < e1; e += kBlock) {
+    const uint32_t row = lv.row[st + e];
+    const uint32_t a = lv.aux[st + e];
+    if (side_of(Xf[row], thr) == 0) {
+      const uint32_t w = a & 0xFFFFu;
+      ++ml;
+      cnt += w;
+      hsh += (uint64_t)w * mix64(row);
+      if (TASK == 0) {
+        const int lab = (int)(a >> 16);
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) loc[c] += (c == lab) ? w : 0u;
+      } else {
+        sm += (int64_t)w * lv.ey[st + e];
+      }
+    }
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    cnt += __shfl_xor(cnt, off, kWave);
+    hsh += __shfl_xor(hsh, off, kWave);
+    sm += __shfl_xor(sm, off, kWave);
+    ml += __shfl_xor(ml, off, kWave);
+  }
+  if (TASK == 0) {
+#pragma unroll
+    for (int c = 0; c < MAXC; ++c) {
+      uint32_t v = loc[c];
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+      if ((t & (kWave - 1)) == 0 && v) atomicAdd(&s_cls[c], v);
+    }
+  }
+  if ((t & (kWave - 1)) == 0) {
+    atomicAdd(&s_cnt, (unsigned long long)cnt);
+    atomicAdd(&s_hash, (unsigned long long)hsh);
+    atomicAdd(&s_sum, (unsigned long long)sm);
+    atomicAdd(&s_ml, ml);
+  }
+  __syncthreads();
+  if (t == 0) {
+    pc.chunk_ml[k] = s_ml;
+    if (s_ml) atomicAdd(&pc.acc_ml[q], s_ml);
+    if (s_cnt) atomicAdd(&pc.acc_cnt[q], s_cnt);
+    if (s_hash) atomicAdd(&pc.acc_hash[q], s_hash);
+    if (TASK == 1 && s_sum) atomicAdd(&pc.acc_sum[q], s_sum);
+  }
+  if (TASK == 0 && t < d.C && s_cls[t]) atomicAdd(&pc.acc_cls[q * d.C + t], s_cls[t]);
+}
+
+// One workgroup per big node.
+template <int TASK>
+__global__ void __launch_bounds__(kBlock) k_part_chunk_finish(DataArgs d, LevelArgs lv, PartArgs pa, PartChunks pc) {
+  __shared__ uint32_t s_w[kWaves];
+  const int64_t q = blockIdx.x;
+  const int t = threadIdx.x;
+  const uint32_t o = pc.list[q];
+  const uint32_t i = lv.open_idx[o];
+  const int m = lv.m[i];
+  const int mL = (int)pc.acc_ml[q];
+  const bool ok = mL > 0 && mL < m;
+  if (t == 0) {
+    pa.split_ok[o] = ok ? 1 : 0;
+    pa.l_m[o] = mL;
+    pa.l_count[o] = (int32_t)pc.acc_cnt[q];
+    pa.l_hash[o] = (uint64_t)pc.acc_hash[q];
+    if (TASK == 1) pa.l_sum[o] = (int64_t)pc.acc_sum[q];
+  }
+  if (TASK == 0 && t < d.C) pa.l_cls[(int64_t)o * d.C + t] = pc.acc_cls[q * d.C + t];
+  // exclusive scan of the chunks' left counts, in place
+  const uint32_t c0 = pc.coff[q], c1 = pc.coff[q + 1];
+  uint32_t base = 0;
+  for (uint32_t k0 = c0; k0 < c1; k0 += kBlock) {
+    const uint32_t k = k0 + t;
+    const uint32_t v = k < c1 ? pc.chunk_ml[k] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(v, s_w, &tot);
+    if (k < c1) pc.chunk_ml[k] = base + ex;
+    base += tot;
+    __syncthreads();
+  }
+  if (t == 0 && pa.bytes) atomicAdd(pa.bytes, (unsigned long long)m * (TASK == 0 ? 12ull : 20ull));
+}
+
+template <int TASK>
+__global__ void __launch_bounds__(kBlock) k_part_chunk_scatter(DataArgs d, LevelArgs lv, PartArgs pa, PartChunks pc) {
+  __shared__ uint32_t s_w[kWaves];
+  const int t = threadIdx.x;
+  const int64_t k = blockIdx.x;
+  const int64_t q = part_chunk_node(pc, k);
+  const uint32_t o = pc.list[q];
+  if (!pa.split_ok[o]) return;  // workgroup-uniform
+  const uint32_t i = lv.open_idx[o];
+  const int m = lv.m[i];
+  const int mL = (int)pc.acc_ml[q];
+  const int e0 = (int)(k - pc.coff[q]) * kPartChunk;
+  const int e1 = min(m, e0 + kPartChunk);
+  const int64_t st = lv.start[i];
+  const DataArgs dd = forest_data(d, lv.tree[i]);
+  const float* Xf = dd.X + (int64_t)pa.ch_feat[o] * dd.N;
+  const float thr = pa.ch_thr[o];
+  uint32_t lbase = pc.chunk_ml[k];                           // lefts before this chunk
+  uint32_t rbase = (uint32_t)mL + (uint32_t)e0 - lbase;      // rights before it follow every left
+  for (int b = e0; b < e1; b += kBlock) {
+    const int e = b + t;
+    uint32_t row = 0, isl = 0;
+    if (e < e1) {
+      row = lv.row[st + e];
+      isl = side_of(Xf[row], thr) == 0 ? 1u : 0u;
+    }
+    uint32_t tot;
+    const uint32_t pl = block_excl_scan(isl, s_w, &tot);
+    if (e < e1) {
+      const uint32_t pos = isl ? lbase + pl : rbase + (uint32_t)(e - b) - pl;
+      pa.nrow[st + pos] = row;
+      pa.naux[st + pos] = lv.aux[st + e];
+      if (TASK == 1) pa.ney[st + pos] = lv.ey[st + e];
+    }
+    lbase += tot;
+    rbase += (uint32_t)min(kBlock, e1 - b) - tot;
+  }
+}
+
 // Small nodes: one wave per node; stable compaction with ballots, tile by
 // tile (64 entries), statistics reduced across the wave.
 template <int MAXC, int TASK>
@@ -3734,6 +3933,14 @@ void launch_partition_seg(int grid_cap, hipStream_t s, const DataArgs& d, const 
 }
 
 
+template <int MAXC, int TASK>
+void launch_partition_chunks(hipStream_t s, const DataArgs& d, const LevelArgs& lv, const PartArgs& pa,
+                             const PartChunks& pc, int64_t n_chunks) {
+  hipLaunchKernelGGL((k_part_chunk_stats<MAXC, TASK>), dim3((unsigned)n_chunks), dim3(kBlock), 0, s, d, lv, pa, pc);
+  hipLaunchKernelGGL((k_part_chunk_finish<TASK>), dim3((unsigned)pc.n_nodes), dim3(kBlock), 0, s, d, lv, pa, pc);
+  hipLaunchKernelGGL((k_part_chunk_scatter<TASK>), dim3((unsigned)n_chunks), dim3(kBlock), 0, s, d, lv, pa, pc);
+}
+
 // Scratch and bookkeeping of the big-node pipeline, reused across levels.
 struct BigState {
   DevBuf<V2Item> items;
@@ -4014,6 +4221,16 @@ PartWaveFn pick_part_seg(int task, int C) {
   if (C <= 8) return launch_partition_seg<8, 0, SEG>;
   if (C <= 16) return launch_partition_seg<16, 0, SEG>;
   return launch_partition_seg<32, 0, SEG>;
+}
+using PartChunkFn = void (*)(hipStream_t, const DataArgs&, const LevelArgs&, const PartArgs&, const PartChunks&,
+                             int64_t);
+PartChunkFn pick_part_chunks(int task, int C) {
+  if (task == 1) return launch_partition_chunks<1, 1>;
+  if (C <= 2) return launch_partition_chunks<2, 0>;
+  if (C <= 4) return launch_partition_chunks<4, 0>;
+  if (C <= 8) return launch_partition_chunks<8, 0>;
+  if (C <= 16) return launch_partition_chunks<16, 0>;
+  return launch_partition_chunks<32, 0>;
 }
 PartWaveFn pick_part_wave(int task, int C) {
   if (task == 1) return launch_partition_wave<1, 1>;
@@ -4461,6 +4678,11 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   PartWaveFn partition_wave = pick_part_wave(task, C);
   PartWaveFn partition_seg16 = pick_part_seg<16>(task, C);
   PartWaveFn partition_seg32 = pick_part_seg<32>(task, C);
+  PartChunkFn partition_chunks = pick_part_chunks(task, C);
+  // host-sized levels: nodes above kPartBig entries partition by chunks
+  constexpr Cuts kPartCutsHost{{16, 32, 512, kPartBig, 0x7FFFFFFF, 0}, 5};
+  DevBuf<uint32_t> pch_n, pch_off, pch_ml, pch_acc_ml, pch_acc_cls;
+  DevBuf<unsigned long long> pch_acc64;
 
   // Size tiers of the split search: wave per item up to wave_cutoff entries,
   // block per item above (the block kernel itself switches to the bucketed
@@ -5034,7 +5256,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         for (auto& tl : tier_list) tl.reserve(n_open);
         DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
         hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(n_open), 1024)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
-                           n_open, lv.open_idx, lv.m, kPartCuts, tier_lists(), tier_cnt.p, nullptr, nullptr);
+                           n_open, lv.open_idx, lv.m, kPartCutsHost, tier_lists(), tier_cnt.p, nullptr, nullptr);
         unsigned int pc[kMaxTiers];
         DTE_HIP(hipMemcpyAsync(pc, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
         DTE_HIP(hipStreamSynchronize(s));
@@ -5049,6 +5271,28 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
           pa.list = tier_list[3].p;
           pa.n_open = pc[3];
           partition((int)std::min<int64_t>(pc[3], search_grid_cap), s, d, lv, pa);
+        }
+        if (pc[4]) {
+          // big nodes: chunks of kPartChunk entries, a workgroup each
+          const int64_t nb = pc[4];
+          pch_n.reserve(nb + 1);
+          pch_off.reserve(nb + 1);
+          hipLaunchKernelGGL(k_part_nchunks, dim3(std::min(grid_for(nb), 1024)), dim3(kBlock), 0, s,
+                             (const uint32_t*)tier_list[4].p, nb, lv.open_idx, lv.m, pch_n.p);
+          scan_u32(pch_n.p, nb, pch_off.p, s, scan_tmp);
+          const int64_t n_chunks = (int64_t)read_one(pch_off.p + nb, s);
+          pch_ml.reserve(n_chunks);
+          pch_acc_ml.reserve(nb);
+          pch_acc64.reserve(3 * nb);
+          pch_acc_cls.reserve(task == 0 ? nb * C : 1);
+          DTE_HIP(hipMemsetAsync(pch_acc_ml.p, 0, 4 * nb, s));
+          DTE_HIP(hipMemsetAsync(pch_acc64.p, 0, 8 * 3 * nb, s));
+          if (task == 0) DTE_HIP(hipMemsetAsync(pch_acc_cls.p, 0, 4 * nb * C, s));
+          PartChunks pcs{tier_list[4].p, nb, pch_off.p, pch_ml.p, pch_acc_ml.p,
+                         pch_acc64.p, pch_acc64.p + nb, pch_acc64.p + 2 * nb, pch_acc_cls.p};
+          pa.list = nullptr;
+          pa.n_open = n_open;
+          partition_chunks(s, d, lv, pa, pcs, n_chunks);
         }
       }
     }
