@@ -186,8 +186,15 @@ def serve_bench(capi, synth, np, n_requests, seed=0):
             "resident_nodes": nodes, "hbm_in_use_bytes": st["hbm_in_use"]}
 
 
+def rate(n, dt, digits):
+    """n / dt rounded, or None when the leg did not run."""
+    return round(n / dt, digits) if dt else None
+
+
 def other_workload(name, args):
-    """Measurement lines of the other workload shapes (not the headline)."""
+    """Measurement lines of the other workload shapes (not the headline).
+    --gpu-path-only keeps the timed GPU path alone (no plain driver, CPU
+    oracle, autotuner or per-forest comparisons), for profiling runs."""
     import ctypes
 
     import numpy as np
@@ -214,14 +221,16 @@ def other_workload(name, args):
         X, y = synth.spam_like(seed=0)
         kw = dict(n_trees=100, n_features_per_node=7, max_depth=25, min_node_size=10, n_classes=2, seed=0)
         dt, nodes = timed_fit(X, y, capi.default_params(capi.EXTRA_TREES, **kw), 5)
-        dtp, _ = timed_fit(X, y, capi.default_params(capi.EXTRA_TREES, build_mode=capi.BUILD_PLAIN, **kw), 2)
-        t = time.perf_counter()
-        pyoracle.fit(X, y, algorithm=1, n_trees=100, k=7, max_depth=25, min_node_size=10, n_classes=2, seed=0,
-                     n_threads=cores)
-        dto = time.perf_counter() - t
+        dtp = dto = None
+        if not args.gpu_path_only:
+            dtp, _ = timed_fit(X, y, capi.default_params(capi.EXTRA_TREES, build_mode=capi.BUILD_PLAIN, **kw), 2)
+            t = time.perf_counter()
+            pyoracle.fit(X, y, algorithm=1, n_trees=100, k=7, max_depth=25, min_node_size=10, n_classes=2, seed=0,
+                         n_threads=cores)
+            dto = time.perf_counter() - t
         line = {"workload": "extra-trees classifier 4,601 x 57 (spam-like), 100 trees, k 7, depth 25, min node 10",
-                "gpu_trees_per_s": round(100 / dt, 1), "gpu_plain_trees_per_s": round(100 / dtp, 1),
-                "cpu_trees_per_s": round(100 / dto, 1), "cpu_cores": cores, "nodes": nodes}
+                "gpu_trees_per_s": round(100 / dt, 1), "gpu_plain_trees_per_s": rate(100, dtp, 1),
+                "cpu_trees_per_s": rate(100, dto, 1), "cpu_cores": cores, "nodes": nodes}
     elif name == "c3":
         X, y = synth.sine_regression(seed=0, n=4_000_000, f=64)
         kw = dict(task=capi.REGRESSION, n_trees=100, n_features_per_node=21, max_depth=18, min_node_size=5,
@@ -231,21 +240,23 @@ def other_workload(name, args):
         dt, nodes = timed_fit(X, y, capi.default_params(capi.RANDOM_FOREST, **kw), 2)
         capi.profiling(False)
         st = capi.profile_stats()
-        kw10 = dict(kw, n_trees=10)  # the plain driver on a 10-tree slice
-        dtp, _ = timed_fit(X, y, capi.default_params(capi.RANDOM_FOREST, build_mode=capi.BUILD_PLAIN, **kw10), 1)
-        dtp *= 10.0
-        log(f"c3: cpu oracle, {cores} trees on {cores} threads")
-        t = time.perf_counter()
-        pyoracle.fit(X, y, algorithm=0, task=1, n_trees=cores, k=21, max_depth=18, min_node_size=5, seed=7,
-                     n_threads=cores)
-        dto = time.perf_counter() - t
+        dtp = dto = None
+        if not args.gpu_path_only:
+            kw10 = dict(kw, n_trees=10)  # the plain driver on a 10-tree slice
+            dtp, _ = timed_fit(X, y, capi.default_params(capi.RANDOM_FOREST, build_mode=capi.BUILD_PLAIN, **kw10), 1)
+            dtp *= 10.0
+            log(f"c3: cpu oracle, {cores} trees on {cores} threads")
+            t = time.perf_counter()
+            pyoracle.fit(X, y, algorithm=0, task=1, n_trees=cores, k=21, max_depth=18, min_node_size=5, seed=7,
+                         n_threads=cores)
+            dto = time.perf_counter() - t
         ss = st.get("split_search", {"total_ms": 0, "bytes": 0})
         lg = st.get("level_graph", {"total_ms": 0, "bytes": 0})
         ss = {"total_ms": ss["total_ms"] + lg["total_ms"], "bytes": ss["bytes"] + lg["bytes"]}
         line = {"workload": "random-forest regression 4,000,000 x 64, 100 trees, variance, k 21, depth 18",
-                "gpu_trees_per_s": round(100 / dt, 2), "gpu_plain_trees_per_s": round(100 / dtp, 2),
+                "gpu_trees_per_s": round(100 / dt, 2), "gpu_plain_trees_per_s": rate(100, dtp, 2),
                 "plain_sample": "10 of the 100 trees",
-                "cpu_trees_per_s": round(cores / dto, 4), "cpu_cores": cores, "nodes": nodes,
+                "cpu_trees_per_s": rate(cores, dto, 4), "cpu_cores": cores, "nodes": nodes,
                 "split_search_GBps": round(ss["bytes"] / (ss["total_ms"] * 1e-3) / 1e9, 1) if ss["total_ms"] else None,
                 "kernel_ms_per_fit": {k: round(v["total_ms"] / 3, 2) for k, v in st.items()}}
     else:  # c4
@@ -256,72 +267,83 @@ def other_workload(name, args):
         devs = [capi.DeviceArray.from_numpy(X) for X, _ in data]
         # one autotuner run on the largest forest; its extra-trees tier cut
         # serves every forest
-        big = max(range(len(specs)), key=lambda i: specs[i]["n"] * specs[i]["f"])
-        tuned, tune_s = capi.autotune(devs[big].ptr, data[big][0].shape[1], data[big][0].shape[0], data[big][1],
-                                      params[big], trial_trees=10)
-        for p in params:
-            p.cut_ert = tuned.cut_ert
-        for (X, y), Xd, p in zip(data, devs, params):
-            capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p).close()
-        t = time.perf_counter()
-        nodes = 0
-        for (X, y), Xd, p in zip(data, devs, params):
-            f = capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p)
-            nodes += f.n_nodes
-            f.close()
-        dt_seq = time.perf_counter() - t
+        tuned, tune_s = None, None
+        if not args.gpu_path_only:
+            big = max(range(len(specs)), key=lambda i: specs[i]["n"] * specs[i]["f"])
+            tuned, tune_s = capi.autotune(devs[big].ptr, data[big][0].shape[1], data[big][0].shape[0],
+                                          data[big][1], params[big], trial_trees=10)
+            for p in params:
+                p.cut_ert = tuned.cut_ert
+        dt_seq = None
+        nodes = None
+        if not args.gpu_path_only:
+            for (X, y), Xd, p in zip(data, devs, params):
+                capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p).close()
+            t = time.perf_counter()
+            nodes = 0
+            for (X, y), Xd, p in zip(data, devs, params):
+                f = capi.fit_device(Xd.ptr, X.shape[1], X.shape[0], y, p)
+                nodes += f.n_nodes
+                f.close()
+            dt_seq = time.perf_counter() - t
         # all 64 forests in one joint build (dte_fit_many: every level step
         # serves the open nodes of every forest)
-        args = ([Xd.ptr for Xd in devs], [X.shape[1] for X, _ in data], [X.shape[0] for X, _ in data],
-                [y for _, y in data], params)
-        for f in capi.fit_many(*args):
+        args_j = ([Xd.ptr for Xd in devs], [X.shape[1] for X, _ in data], [X.shape[0] for X, _ in data],
+                  [y for _, y in data], params)
+        for f in capi.fit_many(*args_j):
             f.close()
         capi.profile_reset()
         capi.profiling(True)
         t = time.perf_counter()
-        forests = capi.fit_many(*args)
+        forests = capi.fit_many(*args_j)
         dt = time.perf_counter() - t
         capi.profiling(False)
         jprof = capi.profile_stats()
         replays = jprof.get("level_graph_replays", {"launches": 0})["launches"]
-        assert sum(f.n_nodes for f in forests) == nodes
+        joint_nodes = sum(f.n_nodes for f in forests)
+        assert nodes is None or joint_nodes == nodes
+        nodes = joint_nodes
         for f in forests:
             f.close()
-        # the same 64 fits as separate fits on 8 concurrent streams, each
-        # sizing its levels on the host (no joint build, no level graphs)
-        hp = []
-        for p in params:
-            q = capi.default_params(capi.EXTRA_TREES)
-            ctypes.memmove(ctypes.addressof(q), ctypes.addressof(p), ctypes.sizeof(q))
-            q.build_mode = capi.BUILD_BATCHED_HOST
-            hp.append(q)
-        hargs = args[:4] + (hp,)
-        for f in capi.fit_many(*hargs, max_concurrent=8):
-            f.close()
-        t = time.perf_counter()
-        hforests = capi.fit_many(*hargs, max_concurrent=8)
-        dt_streams = time.perf_counter() - t
-        assert sum(f.n_nodes for f in hforests) == nodes
-        for f in hforests:
-            f.close()
-        small = sorted(range(64), key=lambda i: specs[i]["n"] * specs[i]["f"])[:8]
-        t = time.perf_counter()
-        for i in small:
-            X, y = data[i]
-            pyoracle.fit(X, y, algorithm=1, n_trees=50, n_classes=specs[i]["classes"], seed=specs[i]["seed"],
-                         n_threads=cores)
-        dto = time.perf_counter() - t
-        ntr = sum(specs[i]["n"] for i in small)
+        dt_streams = dto = None
+        ntr = 0
+        if not args.gpu_path_only:
+            # the same 64 fits as separate fits on 8 concurrent streams, each
+            # sizing its levels on the host (no joint build, no level graphs)
+            hp = []
+            for p in params:
+                q = capi.default_params(capi.EXTRA_TREES)
+                ctypes.memmove(ctypes.addressof(q), ctypes.addressof(p), ctypes.sizeof(q))
+                q.build_mode = capi.BUILD_BATCHED_HOST
+                hp.append(q)
+            hargs = args_j[:4] + (hp,)
+            for f in capi.fit_many(*hargs, max_concurrent=8):
+                f.close()
+            t = time.perf_counter()
+            hforests = capi.fit_many(*hargs, max_concurrent=8)
+            dt_streams = time.perf_counter() - t
+            assert sum(f.n_nodes for f in hforests) == nodes
+            for f in hforests:
+                f.close()
+            small = sorted(range(64), key=lambda i: specs[i]["n"] * specs[i]["f"])[:8]
+            t = time.perf_counter()
+            for i in small:
+                X, y = data[i]
+                pyoracle.fit(X, y, algorithm=1, n_trees=50, n_classes=specs[i]["classes"], seed=specs[i]["seed"],
+                             n_threads=cores)
+            dto = time.perf_counter() - t
+            ntr = sum(specs[i]["n"] for i in small)
         line = {"workload": "64 extra-trees forests, 2k..200k rows, 16..256 features, 2..8 classes, 50 deep trees each",
                 "gpu_trees_per_s": round(64 * 50 / dt, 1), "gpu_seconds_all": round(dt, 2), "nodes": nodes,
                 "gpu_path": "one joint build of the 64 forests (dte_fit_many), every level step one hipGraph replay",
                 "level_graph_replays": replays,
                 "joint_ms": {k: round(v["total_ms"], 2) for k, v in jprof.items() if v["total_ms"] > 0},
-                "gpu_trees_per_s_8_streams_host_levels": round(64 * 50 / dt_streams, 1),
-                "tuning": {"cut_ert": tuned.cut_ert, "autotune_s": round(tune_s, 2)},
-                "gpu_trees_per_s_one_after_another": round(64 * 50 / dt_seq, 1),
-                "cpu_sample": f"8 smallest forests ({ntr} rows total), {dto:.1f} s on {cores} threads",
-                "cpu_trees_per_s_sample": round(8 * 50 / dto, 1)}
+                "gpu_trees_per_s_8_streams_host_levels": rate(64 * 50, dt_streams, 1),
+                "tuning": {"cut_ert": tuned.cut_ert if tuned else "library default",
+                           "autotune_s": round(tune_s, 2) if tune_s is not None else None},
+                "gpu_trees_per_s_one_after_another": rate(64 * 50, dt_seq, 1),
+                "cpu_sample": f"8 smallest forests ({ntr} rows total), {dto:.1f} s on {cores} threads" if dto else None,
+                "cpu_trees_per_s_sample": rate(8 * 50, dto, 1)}
     print(json.dumps({"workload_name": name, **line}), flush=True)
 
 
@@ -340,6 +362,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=1000, help="requests per serving QPS point (0: skip)")
     ap.add_argument("--no-autotune", action="store_true", help="default tier cuts instead of dte_autotune's")
     ap.add_argument("--tuning", default="", help="explicit cut_team,cut_block,bucket_target (no autotuner run)")
+    ap.add_argument("--gpu-path-only", action="store_true",
+                    help="c1/c3/c4: only the timed GPU path (for profiling runs)")
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"],
                     help="c2 (default): the headline random-forest line; c1/c3/c4: the other workload shapes")
     args = ap.parse_args()
