@@ -2597,6 +2597,164 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_seg(DataArgs d, LevelArgs
   if (my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
 }
 
+// ERT, one wave per node of at most kErtTeamIpt * 64 entries, all draws in
+// order.  The node's rows and payloads are read once into registers (slot
+// q of lane l holds entry l + 64 q); a draw gathers its feature's values
+// of all entries at once, and the next draw's feature, threshold samples
+// and gathers are in flight while the current draw is counted and scored.
+// Results equal k_search_ert_wave's bit for bit (the same draws, the same
+// sequential fp32 threshold sum, exact integer counts and fixed-point
+// entropy sums).
+constexpr int kErtTeamIpt = 8;
+
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_search_ert_team(DataArgs d, LevelArgs lv, ItemArgs it) {
+  constexpr int IPT = kErtTeamIpt;
+  resolve_level(lv, d.C);
+  resolve_items(it);
+  if ((int64_t)blockIdx.x * kWaves >= it.n_groups) return;
+  __shared__ double s_inv[128], s_logc[128];
+  load_ln_table(s_inv, s_logc);
+  __syncthreads();
+  const int lane = threadIdx.x & (kWave - 1);
+  unsigned long long my_bytes = 0;
+  const int64_t wave_id = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
+  const int64_t n_waves = (int64_t)gridDim.x * kWaves;
+  for (int64_t g = wave_id; g < it.n_groups; g += n_waves) {
+    const uint32_t o = it.list ? it.list[g] : (uint32_t)g;
+    const uint32_t i = lv.open_idx[o];
+    const int tri = lv.tree[i];
+    const DataArgs dd = forest_data(d, tri);
+    const uint32_t tg = (uint32_t)lv.tree_global[tri];
+    const uint32_t nid = lv.id[i];
+    const int m = lv.m[i];
+    const int64_t st = lv.start[i];
+    const int64_t n = lv.count[i];
+    const int64_t ssum = TASK == 1 ? lv.sum[i] : 0;
+    uint32_t rows[IPT], aux[IPT];
+    int64_t eys[IPT];
+    bool anyw = false;
+#pragma unroll
+    for (int q = 0; q < IPT; ++q) {
+      const int e = lane + q * kWave;
+      rows[q] = 0;
+      aux[q] = 0;
+      eys[q] = 0;
+      if (e < m) {
+        rows[q] = lv.row[st + e];
+        aux[q] = lv.aux[st + e];
+        if (TASK == 1) eys[q] = lv.ey[st + e];
+        anyw |= (aux[q] & 0xFFFFu) != 1u;
+      }
+    }
+    // Extra-trees entries weigh 1 (every row once): counts are popcounts of
+    // ballots.  A node holding another weight sums weights by shuffles.
+    const bool unit_w = __ballot(anyw) == 0;  // wave-uniform
+    const int F = dd.igF;
+    uint32_t nc = 0;
+    int64_t P = 0;
+    double inv = 0.0;
+    if (TASK == 0) {
+      if (lane < d.C) nc = lv.cls[(int64_t)i * d.C + lane];
+      P = xlnx_fixed((uint64_t)n, F, s_inv, s_logc) - wave_sum_i64(lane < d.C ? xlnx_fixed(nc, F, s_inv, s_logc) : 0);
+      inv = 1.0 / ((double)n * kLn2);
+    }
+    // Gathers of draw j: the drawn feature, this lane's threshold sample
+    // (lanes < 10) and every entry's value.
+    auto gather = [&](int j, int& f, float& xs, float* xe) {
+      const uint32_t jj = j >= 0 ? (uint32_t)j : 0u;
+      f = (int)(rand_u32(dd.seed, kStreamFeat, tg, nid, (uint64_t)jj) % (uint32_t)dd.F);
+      const float* Xf = dd.X + (int64_t)f * dd.N;
+      xs = 0.0f;
+      if (j >= 0 && lane < 10) {
+        const uint32_t pos = rand_u32(dd.seed, kStreamThr, tg, nid, (uint64_t)jj * 10 + lane) % (uint32_t)m;
+        xs = Xf[lv.row[st + pos]];
+      }
+#pragma unroll
+      for (int q = 0; q < IPT; ++q) xe[q] = (j >= 0 && lane + q * kWave < m) ? Xf[rows[q]] : 0.0f;
+    };
+    int j_c = item_draw(d, dd, it, 0), f_c = 0;
+    float xs_c = 0.0f, xe_c[IPT];
+    gather(j_c, f_c, xs_c, xe_c);
+    for (int qd = 0; qd < it.per_group; ++qd) {
+      const int j_n = qd + 1 < it.per_group ? item_draw(d, dd, it, qd + 1) : -1;
+      int f_n = 0;
+      float xs_n = 0.0f, xe_n[IPT];
+      if (qd + 1 < it.per_group) gather(j_n, f_n, xs_n, xe_n);
+      if (j_c >= 0) {  // wave-uniform
+        float acc = 0.0f;
+        for (int s2 = 0; s2 < 10; ++s2) acc = acc + __shfl(xs_c, s2, kWave);
+        const float thr = acc / 10.0f;
+        bool left[IPT];
+#pragma unroll
+        for (int q = 0; q < IPT; ++q) left[q] = lane + q * kWave < m && side_of(xe_c[q], thr) == 0;
+        int64_t nl = 0, sls = 0;
+        uint32_t lc = 0;
+        if (unit_w) {
+#pragma unroll
+          for (int q = 0; q < IPT; ++q) nl += __popcll(__ballot(left[q]));
+        } else {
+          int64_t v = 0;
+#pragma unroll
+          for (int q = 0; q < IPT; ++q) v += left[q] ? (int64_t)(aux[q] & 0xFFFFu) : 0;
+          nl = wave_sum_i64(v);
+        }
+        if (TASK == 0) {
+          // one class at a time (lane c keeps class c's left count)
+#pragma unroll 1
+          for (int c = 0; c < d.C; ++c) {
+            uint32_t cnt = 0;
+            if (unit_w) {
+#pragma unroll
+              for (int q = 0; q < IPT; ++q) cnt += (uint32_t)__popcll(__ballot(left[q] && (int)(aux[q] >> 16) == c));
+            } else {
+              int64_t v = 0;
+#pragma unroll
+              for (int q = 0; q < IPT; ++q)
+                v += (left[q] && (int)(aux[q] >> 16) == c) ? (int64_t)(aux[q] & 0xFFFFu) : 0;
+              cnt = (uint32_t)wave_sum_i64(v);
+            }
+            if (lane == c) lc = cnt;
+          }
+        } else {
+          int64_t v = 0;
+#pragma unroll
+          for (int q = 0; q < IPT; ++q) v += left[q] ? (int64_t)(aux[q] & 0xFFFFu) * eys[q] : 0;
+          sls = wave_sum_i64(v);
+        }
+        float score;
+        if (nl == 0 || nl == n) {
+          score = -FLT_MAX;
+        } else if (TASK == 0) {
+          int64_t tl = 0, tr = 0;
+          if (lane < d.C) {
+            tl = xlnx_fixed(lc, F, s_inv, s_logc);
+            tr = xlnx_fixed(nc - lc, F, s_inv, s_logc);
+          }
+          const int64_t Q = (wave_sum_i64(tl) - xlnx_fixed((uint64_t)nl, F, s_inv, s_logc)) +
+                            (wave_sum_i64(tr) - xlnx_fixed((uint64_t)(n - nl), F, s_inv, s_logc));
+          score = (float)ig_from_fixed(P + Q, F, inv);
+        } else {
+          score = (float)variance_gain(nl, sls, n - nl, ssum - sls, dd.shift);
+        }
+        if (lane == 0) {
+          const int64_t ridx = (int64_t)o * it.D + j_c;
+          it.res_score[ridx] = score;
+          it.res_thr[ridx] = thr;
+          it.res_feat[ridx] = f_c;
+          my_bytes += (unsigned long long)m * (TASK == 0 ? 12ull : 20ull);
+        }
+      }
+      j_c = j_n;
+      f_c = f_n;
+      xs_c = xs_n;
+#pragma unroll
+      for (int q = 0; q < IPT; ++q) xe_c[q] = xe_n[q];
+    }
+  }
+  if (lane == 0 && my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
+}
+
 // Split a list of open positions (or all of them) by node size into the
 // wave tier and the block tier.  Order inside a tier is irrelevant: every
 // item writes its own result slot.
@@ -3878,8 +4036,8 @@ T read_one(const T* p, hipStream_t s) {
 const char* search_family(int algorithm, int tier) {
   static const char* rf[] = {"search_seg16", "search_seg32", "search_seg64", "search_team64", "search_block",
                              "search_chunked", "search_block"};
-  static const char* ert[] = {"search_ert_seg16", "search_ert_seg32", "search_ert_seg64", "search_ert_wave",
-                              "search_ert_block"};
+  static const char* ert[] = {"search_ert_seg16", "search_ert_seg32", "search_ert_seg64", "search_ert_team",
+                              "search_ert_wave", "search_ert_block"};
   return algorithm == DTE_RANDOM_FOREST ? rf[tier] : ert[tier];
 }
 
@@ -3888,7 +4046,8 @@ const char* search_family(int algorithm, int tier) {
 // 4 = whole workgroup per item within the LDS capacity; 5 = the chunk
 // pipeline (big_pipeline); 6 = whole workgroup per item of any size (the
 // plain driver).  ERT tiers: 0/1/2 = one 16/32/64-lane segment per node
-// (draws looped in the kernel), 3 = one wave per item, 4 = workgroup per item.
+// (draws looped in the kernel), 3 = one wave per node (entries in
+// registers), 4 = one wave per item, 5 = workgroup per item.
 template <int MAXC, int TASK>
 void launch_search(int algorithm, int tier, int grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv,
                    const ItemArgs& it) {
@@ -3906,7 +4065,8 @@ void launch_search(int algorithm, int tier, int grid, hipStream_t s, const DataA
       case 0: hipLaunchKernelGGL((k_search_ert_seg<MAXC, TASK, 16>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       case 1: hipLaunchKernelGGL((k_search_ert_seg<MAXC, TASK, 32>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       case 2: hipLaunchKernelGGL((k_search_ert_seg<MAXC, TASK, 64>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
-      case 3: hipLaunchKernelGGL((k_search_ert_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 3: hipLaunchKernelGGL((k_search_ert_team<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
+      case 4: hipLaunchKernelGGL((k_search_ert_wave<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
       default: hipLaunchKernelGGL((k_search_ert<MAXC, TASK>), dim3(grid), dim3(kBlock), 0, s, d, lv, it); break;
     }
   }
@@ -4701,7 +4861,8 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     // segment tiers need a lane per class
     const int c16 = (task == 1 || C <= 16) ? 16 : 0;
     const int c32 = (task == 1 || C <= 32) ? 32 : 0;
-    cuts = Cuts{{c16, std::max(c16, c32), 64, p.cut_ert ? p.cut_ert : kDefCutErt, 0x7FFFFFFF, 0}, 5};
+    const int ce = p.cut_ert ? p.cut_ert : kDefCutErt;
+    cuts = Cuts{{c16, std::max(c16, c32), 64, std::min(ce, kErtTeamIpt * kWave), ce, 0x7FFFFFFF}, 6};
   }
   DevBuf<uint32_t> tier_list[kMaxTiers];
   DevBuf<unsigned int> tier_cnt(kMaxTiers);
@@ -4729,14 +4890,14 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       a.n_groups = cnt[tier];
       a.bytes = it.bytes ? it.bytes + tier : nullptr;
       const bool rf = algorithm == DTE_RANDOM_FOREST;
-      trace_tier[rf ? (tier <= 2 ? 0 : (tier == 3 ? 1 : 2)) : (tier <= 2 ? 0 : (tier == 3 ? 1 : 2))] += cnt[tier];
+      trace_tier[rf ? (tier <= 2 ? 0 : (tier == 3 ? 1 : 2)) : (tier <= 2 ? 0 : (tier <= 4 ? 1 : 2))] += cnt[tier];
       if (rf && tier == 5) {
         Timer tk(search_family(algorithm, tier), 0.0, s);
         big(bigst, d, lv, a, tier_list[5].p, cnt[5], s, num_cu);
         continue;
       }
       int grid;
-      if (tier <= (rf ? 3 : 2)) {
+      if (tier <= 3) {
         // node-major tiers: 4 / 2 / 1 / 1 nodes per wave
         const int npw = tier == 0 ? 4 : (tier == 1 ? 2 : 1);
         grid = (int)std::min<int64_t>((cnt[tier] + npw * kWaves - 1) / (npw * kWaves), wave_grid_cap);
@@ -4744,7 +4905,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         grid = (int)std::min<int64_t>(cnt[tier], search_grid_cap);  // one node per workgroup
       } else {
         const int64_t items = (int64_t)cnt[tier] * it.per_group;
-        const bool whole = tier == 4;
+        const bool whole = tier == (rf ? 4 : 5);
         const int per_block = whole ? 1 : kWaves;
         const int cap = whole ? search_grid_cap : wave_grid_cap;
         grid = (int)std::min<int64_t>((items + per_block - 1) / per_block, cap);
@@ -4763,7 +4924,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   };
   auto plain_phase = [&](ItemArgs it) {
     const uint32_t* lst = it.list ? it.list : identity_list(it.n_groups);
-    const int tier = algorithm == DTE_RANDOM_FOREST ? 6 : 4;  // workgroup per node, any size
+    const int tier = algorithm == DTE_RANDOM_FOREST ? 6 : 5;  // workgroup per node, any size
     Timer tm("split_search", 0.0, s);
     for (int64_t c0 = 0; c0 < it.n_groups; c0 += kPlainChunk) {
       ItemArgs a = it;
@@ -4928,7 +5089,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
           a.per_group = per_group;
           a.j0 = j0;
           a.bytes = graph_bytes.p + tier;
-          const int grid = tier <= 3 ? wave_grid_cap : search_grid_cap;
+          const int grid = tier <= (rf ? 3 : 4) ? wave_grid_cap : search_grid_cap;
           search(algorithm, rf && tier == 5 ? 6 : tier, grid, s, d, lvd, a);
         }
       };
