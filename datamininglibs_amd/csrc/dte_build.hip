@@ -4874,6 +4874,28 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   const int wave_grid_cap = num_cu * 16;
   LevelArgs lv{};
   int64_t trace_tier[3] = {0, 0, 0};
+  // Side stream of the split search's tiers beside the chunk pipeline
+  // (created on first use; default priority).
+  struct SideStream {
+    hipStream_t st = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t get() {
+      if (!st) {
+        DTE_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        DTE_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        DTE_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+      }
+      return st;
+    }
+    ~SideStream() {
+      if (st) {
+        (void)hipStreamSynchronize(st);
+        (void)hipEventDestroy(fork);
+        (void)hipEventDestroy(join);
+        (void)hipStreamDestroy(st);
+      }
+    }
+  } side;
   auto run_phase = [&](ItemArgs it) {
     for (auto& tl : tier_list) tl.reserve(it.n_groups);
     DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
@@ -4883,19 +4905,27 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     DTE_HIP(hipMemcpyAsync(cnt, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
     DTE_HIP(hipStreamSynchronize(s));
     Timer tm("split_search", 0.0, s);
+    const bool rf = algorithm == DTE_RANDOM_FOREST;
+    // The chunk pipeline (random-forest nodes above cut_block) runs on the
+    // library stream with host round trips between its batches; the other
+    // tiers touch other nodes and run beside it on a side stream.
+    bool others = false;
+    for (int tier = 0; tier < 5; ++tier) others = others || cnt[tier] > 0;
+    const bool overlap = rf && cnt[5] > 0 && others;
+    hipStream_t ts = s;
+    if (overlap) {
+      ts = side.get();
+      DTE_HIP(hipEventRecord(side.fork, s));
+      DTE_HIP(hipStreamWaitEvent(ts, side.fork, 0));
+    }
     for (int tier = 0; tier < cuts.n; ++tier) {
       if (!cnt[tier]) continue;
       ItemArgs a = it;
       a.list = tier_list[tier].p;
       a.n_groups = cnt[tier];
       a.bytes = it.bytes ? it.bytes + tier : nullptr;
-      const bool rf = algorithm == DTE_RANDOM_FOREST;
       trace_tier[rf ? (tier <= 2 ? 0 : (tier == 3 ? 1 : 2)) : (tier <= 2 ? 0 : (tier <= 4 ? 1 : 2))] += cnt[tier];
-      if (rf && tier == 5) {
-        Timer tk(search_family(algorithm, tier), 0.0, s);
-        big(bigst, d, lv, a, tier_list[5].p, cnt[5], s, num_cu);
-        continue;
-      }
+      if (rf && tier == 5) continue;  // below, after the side launches
       int grid;
       if (tier <= 3) {
         // node-major tiers: 4 / 2 / 1 / 1 nodes per wave
@@ -4910,8 +4940,20 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
         const int cap = whole ? search_grid_cap : wave_grid_cap;
         grid = (int)std::min<int64_t>((items + per_block - 1) / per_block, cap);
       }
-      Timer tk(search_family(algorithm, tier), 0.0, s);
-      search(algorithm, tier, grid, s, d, lv, a);
+      Timer tk(search_family(algorithm, tier), 0.0, ts);
+      search(algorithm, tier, grid, ts, d, lv, a);
+    }
+    if (rf && cnt[5]) {
+      ItemArgs a = it;
+      a.list = tier_list[5].p;
+      a.n_groups = cnt[5];
+      a.bytes = it.bytes ? it.bytes + 5 : nullptr;
+      Timer tk(search_family(algorithm, 5), 0.0, s);
+      big(bigst, d, lv, a, tier_list[5].p, cnt[5], s, num_cu);
+    }
+    if (overlap) {
+      DTE_HIP(hipEventRecord(side.join, ts));
+      DTE_HIP(hipStreamWaitEvent(s, side.join, 0));
     }
   };
   // Plain launch structure: chunks of at most kPlainChunk open nodes, one
