@@ -3039,10 +3039,10 @@ struct PartArgs {
 constexpr int kSideWords = 1024;
 
 template <int MAXC, int TASK>
-__global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, PartArgs pa) {
+__device__ __forceinline__ void part_block_body(DataArgs d, LevelArgs lv, PartArgs pa, int64_t bid, int64_t nblk) {
   resolve_level(lv, d.C);
   if (pa.n_dev) pa.n_open = *pa.n_dev;
-  if ((int64_t)blockIdx.x >= pa.n_open) return;
+  if ((int64_t)bid >= pa.n_open) return;
   __shared__ unsigned long long s_side[kSideWords];
   __shared__ uint32_t s_w[kWaves];
   __shared__ uint32_t s_cls[MAXC];
@@ -3050,7 +3050,7 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
   __shared__ uint32_t s_ml;
   const int t = threadIdx.x;
   unsigned long long my_bytes = 0;
-  for (int64_t q = blockIdx.x; q < pa.n_open; q += gridDim.x) {
+  for (int64_t q = bid; q < pa.n_open; q += nblk) {
     const int64_t o = pa.list ? pa.list[q] : q;
     const uint32_t i = lv.open_idx[o];
     const int m = lv.m[i];
@@ -3156,6 +3156,10 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
     __syncthreads();
   }
   if (t == 0 && my_bytes && pa.bytes) atomicAdd(pa.bytes, my_bytes);
+}
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, PartArgs pa) {
+  part_block_body<MAXC, TASK>(d, lv, pa, blockIdx.x, gridDim.x);
 }
 
 // Nodes above kPartBig entries (the top levels, where a workgroup per node
@@ -3360,15 +3364,15 @@ __global__ void __launch_bounds__(kBlock) k_part_chunk_scatter(DataArgs d, Level
 // Small nodes: one wave per node; stable compaction with ballots, tile by
 // tile (64 entries), statistics reduced across the wave.
 template <int MAXC, int TASK>
-__global__ void __launch_bounds__(kBlock) k_partition_wave(DataArgs d, LevelArgs lv, PartArgs pa,
-                                                           const uint32_t* list, int64_t n_list) {
+__device__ __forceinline__ void part_wave_body(DataArgs d, LevelArgs lv, PartArgs pa, const uint32_t* list,
+                                               int64_t n_list, int64_t bid, int64_t nblk) {
   resolve_level(lv, d.C);
   if (pa.n_dev) n_list = *pa.n_dev;
   const int lane = threadIdx.x & (kWave - 1);
   const uint64_t lt_mask = (1ull << lane) - 1ull;
   unsigned long long my_bytes = 0;
-  for (int64_t q = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave; q < n_list;
-       q += (int64_t)gridDim.x * kWaves) {
+  for (int64_t q = (int64_t)bid * kWaves + threadIdx.x / kWave; q < n_list;
+       q += (int64_t)nblk * kWaves) {
     const uint32_t o = list[q];
     const uint32_t i = lv.open_idx[o];
     const int m = lv.m[i];
@@ -3457,13 +3461,18 @@ __global__ void __launch_bounds__(kBlock) k_partition_wave(DataArgs d, LevelArgs
   }
   if (lane == 0 && my_bytes && pa.bytes) atomicAdd(pa.bytes, my_bytes);
 }
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_partition_wave(DataArgs d, LevelArgs lv, PartArgs pa,
+                                                           const uint32_t* list, int64_t n_list) {
+  part_wave_body<MAXC, TASK>(d, lv, pa, list, n_list, blockIdx.x, gridDim.x);
+}
 
 // Nodes of at most SEG entries: one SEG-lane segment per node, one entry
 // per lane, a single pass (side, left statistics, stable positions from the
 // segment's ballots).  Same outputs as k_partition_wave.
 template <int MAXC, int TASK, int SEG>
-__global__ void __launch_bounds__(kBlock) k_partition_seg(DataArgs d, LevelArgs lv, PartArgs pa,
-                                                          const uint32_t* list, int64_t n_list) {
+__device__ __forceinline__ void part_seg_body(DataArgs d, LevelArgs lv, PartArgs pa, const uint32_t* list,
+                                              int64_t n_list, int64_t bid, int64_t nblk) {
   constexpr int NPW = kWave / SEG;
   resolve_level(lv, d.C);
   if (pa.n_dev) n_list = *pa.n_dev;
@@ -3475,8 +3484,8 @@ __global__ void __launch_bounds__(kBlock) k_partition_seg(DataArgs d, LevelArgs 
   const uint64_t seg_mask = (SEG == kWave ? ~0ull : ((1ull << SEG) - 1ull)) << (seg * SEG);
   const uint64_t below = (1ull << lane) - 1ull;
   unsigned long long my_bytes = 0;
-  const int64_t wave_id = (int64_t)blockIdx.x * kWaves + wv;
-  const int64_t n_waves = (int64_t)gridDim.x * kWaves;
+  const int64_t wave_id = (int64_t)bid * kWaves + wv;
+  const int64_t n_waves = (int64_t)nblk * kWaves;
   for (int64_t q0 = wave_id * NPW; q0 < n_list; q0 += n_waves * NPW) {
     const int64_t q = q0 + seg;
     const bool node_ok = q < n_list;
@@ -3543,6 +3552,54 @@ __global__ void __launch_bounds__(kBlock) k_partition_seg(DataArgs d, LevelArgs 
     }
   }
   if (sl == 0 && my_bytes && pa.bytes) atomicAdd(pa.bytes, my_bytes);
+}
+template <int MAXC, int TASK, int SEG>
+__global__ void __launch_bounds__(kBlock) k_partition_seg(DataArgs d, LevelArgs lv, PartArgs pa,
+                                                          const uint32_t* list, int64_t n_list) {
+  part_seg_body<MAXC, TASK, SEG>(d, lv, pa, list, n_list, blockIdx.x, gridDim.x);
+}
+
+// All partition tiers of a level step in one launch (small data, see
+// k_search_ert_all): workgroups [first[k], first[k + 1]) run tier k
+// (16-lane segments, 32-lane segments, a wave, a workgroup per node).
+struct PartTiers {
+  int64_t first[5];
+  const uint32_t* list[4];
+  const unsigned int* n_dev[4];
+  int64_t n_bound;  // list capacity (the device counts hold the sizes)
+};
+
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock) k_partition_all(DataArgs d, LevelArgs lv, PartArgs pa, PartTiers pt) {
+  const int64_t b = blockIdx.x;
+  int k = 0;
+  while (k < 3 && b >= pt.first[k + 1]) ++k;
+  pa.n_dev = pt.n_dev[k];
+  const int64_t bid = b - pt.first[k], nblk = pt.first[k + 1] - pt.first[k];
+  switch (k) {
+    case 0: part_seg_body<MAXC, TASK, 16>(d, lv, pa, pt.list[0], pt.n_bound, bid, nblk); break;
+    case 1: part_seg_body<MAXC, TASK, 32>(d, lv, pa, pt.list[1], pt.n_bound, bid, nblk); break;
+    case 2: part_wave_body<MAXC, TASK>(d, lv, pa, pt.list[2], pt.n_bound, bid, nblk); break;
+    default:
+      pa.list = pt.list[3];
+      part_block_body<MAXC, TASK>(d, lv, pa, bid, nblk);
+      break;
+  }
+}
+
+template <int MAXC, int TASK>
+void launch_partition_all(int64_t grid, hipStream_t s, const DataArgs& d, const LevelArgs& lv, const PartArgs& pa,
+                          const PartTiers& pt) {
+  hipLaunchKernelGGL((k_partition_all<MAXC, TASK>), dim3((unsigned)grid), dim3(kBlock), 0, s, d, lv, pa, pt);
+}
+using PartAllFn = void (*)(int64_t, hipStream_t, const DataArgs&, const LevelArgs&, const PartArgs&, const PartTiers&);
+PartAllFn pick_part_all(int task, int C) {
+  if (task == 1) return launch_partition_all<1, 1>;
+  if (C <= 2) return launch_partition_all<2, 0>;
+  if (C <= 4) return launch_partition_all<4, 0>;
+  if (C <= 8) return launch_partition_all<8, 0>;
+  if (C <= 16) return launch_partition_all<16, 0>;
+  return launch_partition_all<32, 0>;
 }
 
 // ------------------------------------------------------ children --------
@@ -5147,8 +5204,11 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     // (4,601 x 57, 100 trees: 12.3k -> 15.9k trees/s); on large data the
     // merged kernel's register budget (the largest tier's) costs more
     // occupancy than the overlap gains (64 forests: 8.2k -> 6.9k).
-    const bool ert_merged = !rf && E <= kErtMergeEntries;
+    const bool small_data = E <= kErtMergeEntries;
+    const bool ert_merged = !rf && small_data;
     const ErtAllFn ert_all = pick_ert_all(task, C);
+    // likewise the partition tiers of a step (both algorithms)
+    const PartAllFn partition_all = pick_part_all(task, C);
 
     auto dscan = [&](const uint32_t* in, const int64_t* n64, const unsigned int* n32, uint32_t* out,
                      unsigned int* total) {
@@ -5282,10 +5342,25 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       p2.n_dev = L->ptier + 2;
       p3.n_dev = L->ptier + 3;
       p3.list = tier_list[3].p;
-      partition_seg16(wave_grid_cap, s, d, lvd, p0, tier_list[0].p, NO);  // grid from the bound; count from p0.n_dev
-      partition_seg32(wave_grid_cap, s, d, lvd, p1, tier_list[1].p, NO);
-      partition_wave(wave_grid_cap, s, d, lvd, p2, tier_list[2].p, NO);
-      partition(search_grid_cap, s, d, lvd, p3);
+      if (small_data) {
+        PartTiers pt{};
+        const int64_t g[4] = {wave_grid_cap, wave_grid_cap, wave_grid_cap, search_grid_cap};
+        int64_t acc = 0;
+        for (int k = 0; k < 4; ++k) {
+          pt.first[k] = acc;
+          acc += g[k];
+          pt.list[k] = tier_list[k].p;
+          pt.n_dev[k] = L->ptier + k;
+        }
+        pt.first[4] = acc;
+        pt.n_bound = NO;
+        partition_all(acc, s, d, lvd, pa, pt);
+      } else {
+        partition_seg16(wave_grid_cap, s, d, lvd, p0, tier_list[0].p, NO);  // grid from the bound; count from p0.n_dev
+        partition_seg32(wave_grid_cap, s, d, lvd, p1, tier_list[1].p, NO);
+        partition_wave(wave_grid_cap, s, d, lvd, p2, tier_list[2].p, NO);
+        partition(search_grid_cap, s, d, lvd, p3);
+      }
       // children
       ChildArgs ca{};
       ca.lvl = L;
