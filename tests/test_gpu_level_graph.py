@@ -47,6 +47,9 @@ CASES = [
     # item through a scratch slab)
     (capi.RANDOM_FOREST, capi.CLASSIFICATION, 20000, 10, 4, 3, 20, 2, 4, {"cut_team": 64, "cut_block": 64}),
     (capi.RANDOM_FOREST, capi.REGRESSION, 12000, 8, 0, 3, 20, 2, 3, {"cut_team": 64, "cut_block": 64}),
+    # roots above 32768 entries: host-sized levels partition them by chunks
+    (capi.EXTRA_TREES, capi.CLASSIFICATION, 100000, 16, 4, 0, 14, 5, 3, {}),
+    (capi.EXTRA_TREES, capi.REGRESSION, 60000, 12, 0, 4, 14, 5, 3, {}),
 ]
 
 
