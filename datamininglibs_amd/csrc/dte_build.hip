@@ -2470,8 +2470,9 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_wave(DataArgs d, LevelArg
 // and the gathers of draw q + 1 are issued before draw q is counted and
 // scored.  The draw-independent entropy term of the node is computed once.
 // Every draw's (score, threshold, feature) equals k_search_ert_wave's bit
-// for bit: the same Philox draws, the same sequential fp32 threshold sum,
-// exact integer counts and the same fixed-point entropy sums.
+// for bit: the same Philox draws, the same sequential fp32 threshold sum
+// (gpuert.cu:167-183), exact integer side counts (gpuert.cu:287-301) or
+// target sums (gpuert.cu:348-361) and the same fixed-point entropy sums.
 template <int MAXC, int TASK, int SEG>
 __device__ __forceinline__ void ert_seg_body(DataArgs d, LevelArgs lv, ItemArgs it, int64_t bid, int64_t nblk,
                                              double* s_inv, double* s_logc) {
@@ -2619,7 +2620,7 @@ __global__ void __launch_bounds__(kBlock) k_search_ert_seg(DataArgs d, LevelArgs
 // and gathers are in flight while the current draw is counted and scored.
 // Results equal k_search_ert_wave's bit for bit (the same draws, the same
 // sequential fp32 threshold sum, exact integer counts and fixed-point
-// entropy sums).
+// entropy sums; gpuert.cu:144-230, 287-301, 348-361).
 constexpr int kErtTeamIpt = 8;
 
 template <int MAXC, int TASK>
