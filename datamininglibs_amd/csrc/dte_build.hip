@@ -3175,7 +3175,8 @@ __global__ void __launch_bounds__(kBlock) k_partition(DataArgs d, LevelArgs lv, 
 //                            chunk's offsets (lefts before it on the left,
 //                            rights before it after the node's mL lefts)
 // Sums of integers, so the statistics equal k_partition's bit for bit, and
-// entries land where k_partition puts them.
+// entries land where k_partition puts them (a stable compaction in place of
+// the reference's atomic slot order, gpudte.cu:205-232).
 constexpr int kPartBig = 32768;
 constexpr int kPartChunk = 8192;
 
