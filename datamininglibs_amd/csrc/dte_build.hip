@@ -259,10 +259,21 @@ struct TeamLds {
   } u;
   uint32_t aux[CAP];
   int64_t y[TASK == 1 ? CAP : 1];
-  uint32_t pre[TASK == 0 ? MAXC * TEAM : 1];  // per-lane class columns
+  // Per-lane class columns [c * TEAM + t] of the walk; a wave team of up to
+  // 16 classes keeps them in the sort counters, free once the chunk is sorted.
+  static constexpr bool kColsInCnt = TEAM == kWave && MAXC * TEAM * 4 <= SPAN * 2;
+  uint32_t pre[TASK == 0 && !kColsInCnt ? MAXC * TEAM : 1];
   uint32_t ncls[MAXC];
   uint32_t ctot[MAXC];
-  uint32_t last[TEAM];
+  uint32_t last[TEAM == kWave ? 1 : TEAM];  // histogram walk of whole-workgroup teams
+  __device__ __forceinline__ uint32_t* cols() {
+    if constexpr (kColsInCnt) {
+      return reinterpret_cast<uint32_t*>(u.srt.cnt);
+    } else {
+      return pre;
+    }
+  }
+  __device__ __forceinline__ const uint32_t* cols() const { return const_cast<TeamLds*>(this)->cols(); }
   uint32_t w[8 * kWaves + kWaves];
   int64_t w64[kWaves];
   int64_t w64v[4 * kWaves];
@@ -279,12 +290,12 @@ __device__ __forceinline__ void scan_class_group(TeamLds<MAXC, TASK, TEAM, IPT>&
   const int t = team_tid<TEAM>();
   uint32_t v[G], tot[G];
 #pragma unroll
-  for (int q = 0; q < G; ++q) v[q] = (c0 + q < C) ? L.pre[(c0 + q) * TEAM + t] : 0u;
+  for (int q = 0; q < G; ++q) v[q] = (c0 + q < C) ? L.cols()[(c0 + q) * TEAM + t] : 0u;
   team_excl_scan_vec<TEAM, G>(v, L.w, tot);
 #pragma unroll
   for (int q = 0; q < G; ++q)
     if (c0 + q < C) {
-      L.pre[(c0 + q) * TEAM + t] = v[q] + L.cr.run[c0 + q];
+      L.cols()[(c0 + q) * TEAM + t] = v[q] + L.cr.run[c0 + q];
       if (t == 0) L.ctot[c0 + q] = tot[q];
     }
 }
@@ -324,7 +335,7 @@ __device__ __forceinline__ WalkState walk_init(const TeamLds<MAXC, TASK, TEAM, I
   const int t = team_tid<TEAM>();
   WalkState w{0, 0, 0};
   for (int c = 0; c < nc.C; ++c) {
-    const uint32_t l = L.pre[c * TEAM + t];
+    const uint32_t l = L.cols()[c * TEAM + t];
     w.nl += l;
     w.sl += lx.t(l);
     w.sr += lx.t(L.ncls[c] - l);
@@ -342,9 +353,9 @@ template <int MAXC, int TASK, int TEAM, int IPT, class LX>
 __device__ __forceinline__ void walk_add(WalkState& w, TeamLds<MAXC, TASK, TEAM, IPT>& L, int lab, uint32_t add,
                                          const LX& lx) {
   const int t = team_tid<TEAM>();
-  const uint32_t lo = L.pre[lab * TEAM + t];
+  const uint32_t lo = L.cols()[lab * TEAM + t];
   const uint32_t ln = lo + add;
-  L.pre[lab * TEAM + t] = ln;
+  L.cols()[lab * TEAM + t] = ln;
   const uint32_t tot = L.ncls[lab];
   w.sl += lx.t(ln) - lx.t(lo);
   w.sr += lx.t(tot - ln) - lx.t(tot - lo);
@@ -360,6 +371,111 @@ __device__ __forceinline__ void carry_best(Carry<MAXC, TASK>& cr, const BestCand
   }
 }
 
+// Classification walk of a key-sorted chunk on a one-wave team.  The class
+// counts before each position come from the per-lane class columns (lane t
+// owns a run of consecutive positions) and go to LDS; the x ln x reads and
+// the running sums then go row by row, lane t on position 64 * i + t, so
+// that the 64 reads of one instruction are at neighbouring counts (a few
+// cache lines) instead of 64 runs ipt positions apart.  The candidates and
+// their order are those of the run-per-lane walk: positions ascend within a
+// lane, and the best candidate is a maximum under a total order.
+template <int MAXC, int TASK, int IPT, class LX>
+__device__ void eval_sorted_wave(const uint32_t* keys, const uint16_t* vals, int n, uint32_t kbase,
+                                 const NodeCtx& nc, TeamLds<MAXC, TASK, kWave, IPT>& L, const LX& lx) {
+  const int t = threadIdx.x & (kWave - 1);
+  const int ipt = lane_span<kWave>(n);
+  const int lo = t * ipt;
+  const int hi = min(lo + ipt, n);
+  // Counts before each position, in the key buffer the sort left free.
+  uint32_t* lbp = keys == L.u.srt.k[0] ? L.u.srt.k[1] : L.u.srt.k[0];
+  for (int c = 0; c < nc.C; ++c) L.cols()[c * kWave + t] = 0;
+  for (int p = lo; p < hi; ++p) {
+    const uint32_t a = L.aux[vals[p]];
+    L.cols()[(a >> 16) * kWave + t] += a & 0xFFFFu;
+  }
+  scan_class_columns<MAXC, TASK, kWave, IPT>(L, nc.C);
+  for (int p = lo; p < hi; ++p) {
+    const uint32_t a = L.aux[vals[p]];
+    const int c = (int)(a >> 16);
+    const uint32_t l = L.cols()[c * kWave + t];
+    lbp[p] = l;
+    L.cols()[c * kWave + t] = l + (a & 0xFFFFu);
+  }
+  // Left and right sums and the weight before the chunk (one class per lane).
+  int64_t s0 = 0;
+  uint32_t n0 = 0;
+  if (t < nc.C) {
+    n0 = L.cr.run[t];
+    s0 = lx.t(n0) + lx.t(L.ncls[t] - n0);
+  }
+  s0 = wave_last_i64(wave_incl_scan_i64(s0));
+  n0 = wave_last(wave_incl_scan(n0));
+  team_sync<kWave>();
+  // Row pass 1: every x ln x read of the chunk, independent of each other.
+  const int rows = (n + kWave - 1) / kWave;
+  int64_t dd[IPT], tq[IPT];
+  uint32_t nrun = n0;
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    dd[i] = 0;
+    tq[i] = 0;
+    if (i < rows) {
+      const int p = i * kWave + t;
+      uint32_t w = 0, lb = 0, tc = 0;
+      if (p < n) {
+        const uint32_t a = L.aux[vals[p]];
+        w = a & 0xFFFFu;
+        lb = lbp[p];
+        tc = L.ncls[a >> 16];
+      }
+      const uint32_t inc = wave_incl_scan(w);
+      const uint32_t nlb = nrun + inc - w;
+      nrun += wave_last(inc);
+      if (p < n) {
+        dd[i] = (lx.t(lb + w) - lx.t(lb)) + (lx.t(tc - lb - w) - lx.t(tc - lb));
+        tq[i] = nlb > 0 ? lx.t((uint64_t)nlb) + lx.t((uint64_t)(nc.n - nlb)) : -1;
+      }
+    }
+  }
+  // Row pass 2: running sums, the gain at each boundary between distinct keys.
+  BestCand tb{0.0, kNone, kNone};
+  int64_t best_pq = 0;
+  int64_t srun = s0;
+#pragma unroll
+  for (int i = 0; i < IPT; ++i) {
+    if (i < rows) {
+      const int p = i * kWave + t;
+      const int64_t inc = wave_incl_scan_i64(dd[i]);
+      const int64_t before = srun + inc - dd[i];
+      srun += wave_last_i64(inc);
+      if (p < n && tq[i] >= 0) {
+        const uint32_t k = keys[p] + kbase;
+        const uint32_t prevk = p > 0 ? keys[p - 1] + kbase : L.cr.last_key;
+        if (prevk < k) {
+          const int64_t pq = nc.P + before - tq[i];
+          if (pq > best_pq) {
+            const double g = ig_from_fixed(pq, lx.F, nc.inv);
+            if (g > tb.g) {
+              tb = BestCand{g, k, prevk};
+              best_pq = pq;
+            }
+          }
+        }
+      }
+    }
+  }
+  const BestCand b = team_best<kWave>(tb, L.bg, L.bk, L.bp);
+  if (t == 0) {
+    carry_best(L.cr, b);
+    for (int c = 0; c < nc.C; ++c) {
+      L.cr.run[c] += L.ctot[c];
+      L.cr.run_n += L.ctot[c];
+    }
+    if (n > 0) L.cr.last_key = keys[n - 1] + kbase;
+  }
+  team_sync<kWave>();
+}
+
 // Evaluate every boundary of a key-sorted chunk (keys relative to kbase,
 // vals index the chunk's payload in L.aux / L.y).
 template <int MAXC, int TASK, int TEAM, int IPT, class LX>
@@ -370,11 +486,13 @@ __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, in
   const int lo = t * ipt;
   const int hi = min(lo + ipt, n);
   BestCand tb{0.0, kNone, kNone};
-  if constexpr (TASK == 0) {
-    for (int c = 0; c < nc.C; ++c) L.pre[c * TEAM + t] = 0;
+  if constexpr (TASK == 0 && TEAM == kWave) {
+    eval_sorted_wave<MAXC, TASK, IPT>(keys, vals, n, kbase, nc, L, lx);
+  } else if constexpr (TASK == 0) {
+    for (int c = 0; c < nc.C; ++c) L.cols()[c * TEAM + t] = 0;
     for (int p = lo; p < hi; ++p) {
       const uint32_t a = L.aux[vals[p]];
-      L.pre[(a >> 16) * TEAM + t] += a & 0xFFFFu;
+      L.cols()[(a >> 16) * TEAM + t] += a & 0xFFFFu;
     }
     scan_class_columns<MAXC, TASK, TEAM, IPT>(L, nc.C);
     // Three passes so that the x ln x lookups of all positions are
@@ -385,7 +503,7 @@ __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, in
     int64_t nlb[IPT];
     if (lo < hi) {
       int64_t nl = 0;
-      for (int c = 0; c < nc.C; ++c) nl += L.pre[c * TEAM + t];
+      for (int c = 0; c < nc.C; ++c) nl += L.cols()[c * TEAM + t];
 #pragma unroll
       for (int i = 0; i < IPT; ++i) {
         const int p = lo + i;
@@ -393,8 +511,8 @@ __device__ void eval_sorted_chunk(const uint32_t* keys, const uint16_t* vals, in
           const uint32_t a = L.aux[vals[p]];
           const int c = (int)(a >> 16);
           const uint32_t wt = a & 0xFFFFu;
-          lb[i] = L.pre[c * TEAM + t];
-          L.pre[c * TEAM + t] = lb[i] + wt;
+          lb[i] = L.cols()[c * TEAM + t];
+          L.cols()[c * TEAM + t] = lb[i] + wt;
           tc[i] = L.ncls[c];
           wv[i] = wt;
           nlb[i] = nl;
@@ -530,12 +648,12 @@ __device__ void eval_hist_bucket(const uint32_t* skey, const uint32_t* saux, con
   uint32_t my_last = 0;  // last non-empty slot key + 1 in my range, 0 = none
   BestCand tb{0.0, kNone, kNone};
   if constexpr (TASK == 0) {
-    for (int c = 0; c < C; ++c) L.pre[c * 256 + t] = 0;
+    for (int c = 0; c < C; ++c) L.cols()[c * 256 + t] = 0;
     for (int sl = s0; sl < s1; ++sl) {
       uint32_t any = 0;
       for (int c = 0; c < C; ++c) {
         const uint32_t h = s_hist[sl * C + c];
-        L.pre[c * 256 + t] += h;
+        L.cols()[c * 256 + t] += h;
         any |= h;
       }
       if (any) my_last = kbase + sl + 1;
@@ -1541,7 +1659,9 @@ __device__ __forceinline__ NodeCtx v2_node(const DataArgs& d, const LevelArgs& l
 // One wave per bucket of at most kV2WaveCap entries (four independent
 // buckets per workgroup), grid-stride over the task list.  The loads of a
 // wave's next bucket (task, item and bucket fields, carries, entries) are
-// issued before its current bucket is sorted and walked.
+// issued before its current bucket is sorted and walked.  Three workgroups
+// per CU (up to 16 classes: about 53 KB of LDS each, at most 168 VGPRs):
+// the walk is bound by the latency of its x ln x reads.
 template <int TASK, int IPT>
 struct V2Pre {
   int64_t gb, e0;
@@ -1590,7 +1710,7 @@ __device__ __forceinline__ void v2_prefetch(const DataArgs& d, const LevelArgs& 
 }
 
 template <int MAXC, int TASK>
-__global__ void __launch_bounds__(kBlock) k_v2_eval(DataArgs d, LevelArgs lv, V2Args a, int64_t n_tasks) {
+__global__ void __launch_bounds__(kBlock, 3) k_v2_eval(DataArgs d, LevelArgs lv, V2Args a, int64_t n_tasks) {
   constexpr int IPT = kV2Ipt;
   __shared__ double s_inv[128], s_logc[128];
   __shared__ TeamLds<MAXC, TASK, kWave, IPT> s_team[kWaves];
