@@ -23,8 +23,9 @@ def test_spam_like_small(algorithm):
 
 
 @pytest.mark.parametrize("algorithm", [capi.EXTRA_TREES, capi.RANDOM_FOREST])
-@pytest.mark.parametrize("c", [3, 10])
+@pytest.mark.parametrize("c", [3, 10, 24])
 def test_multiclass(algorithm, c):
+    # 24 classes take the kernels built for up to 32 classes.
     X, y = small_classes(3, 3000, 16, c)
     g, o = fit_both(X, y, algorithm=algorithm, n_trees=3, k=5, max_depth=20, min_node_size=5, n_classes=c)
     assert_same_forest(g, o)
@@ -36,6 +37,15 @@ def test_rf_big_nodes_bucket_path():
     X, y = small_classes(5, 20000, 12, 4)
     g, o = fit_both(X, y, algorithm=capi.RANDOM_FOREST, n_trees=2, k=3, max_depth=12, min_node_size=5,
                     n_classes=4)
+    assert_same_forest(g, o)
+
+
+def test_rf_big_nodes_bucket_path_24_classes():
+    # The bucketed path and the wave and workgroup tiers with the class
+    # columns of up to 32 classes.
+    X, y = small_classes(9, 20000, 12, 24, informative=6)
+    g, o = fit_both(X, y, algorithm=capi.RANDOM_FOREST, n_trees=2, k=3, max_depth=12, min_node_size=5,
+                    n_classes=24)
     assert_same_forest(g, o)
 
 
