@@ -1773,6 +1773,154 @@ __global__ void __launch_bounds__(kBlock, 3) k_v2_eval(DataArgs d, LevelArgs lv,
   }
 }
 
+// LDS of one wave of the regression bucket evaluator.  The payloads
+// (weights, fixed-point targets) stay in registers through the sort and then
+// go into the sort counters (targets) and the key buffer the sort left free
+// (weights): about 10 KB per wave, three workgroups per CU.
+template <int IPT>
+struct RegWaveLds {
+  static constexpr int CAP = kWave * IPT;
+  static constexpr int SPAN = TeamLds<1, 1, kWave, IPT>::SPAN;
+  uint32_t k[2][CAP];
+  uint16_t v[2][CAP];
+  union {
+    uint16_t cnt[SPAN];
+    int64_t y[CAP];
+  } u;
+  uint32_t w[8 * kWaves + kWaves];
+  Carry<1, 1> cr;
+  static_assert(sizeof(uint16_t) * SPAN >= sizeof(int64_t) * CAP, "targets fit the sort counters");
+};
+
+// Regression walk of a key-sorted bucket on one wave (the regression branch
+// of eval_sorted_chunk with the payloads at wts / ys).
+template <int IPT>
+__device__ void eval_sorted_reg_wave(const uint32_t* keys, const uint16_t* vals, const uint32_t* wts,
+                                     const int64_t* ys, int n, uint32_t kbase, const NodeCtx& nc, Carry<1, 1>& cr) {
+  const int t = threadIdx.x & (kWave - 1);
+  const int ipt = lane_span<kWave>(n);
+  const int lo = t * ipt;
+  const int hi = min(lo + ipt, n);
+  BestCand tb{0.0, kNone, kNone};
+  int64_t ln = 0, ls = 0;
+  for (int p = lo; p < hi; ++p) {
+    const uint16_t v = vals[p];
+    const int64_t w = (int64_t)wts[v];
+    ln += w;
+    ls += w * ys[v];
+  }
+  int64_t tot_n, tot_s;
+  int64_t pn = team_excl_scan_i64<kWave>(ln, nullptr, &tot_n);
+  int64_t ps = team_excl_scan_i64<kWave>(ls, nullptr, &tot_s);
+  pn += cr.run_n;
+  ps += cr.run_s;
+  if (lo < hi) {
+    uint32_t prevk = (lo > 0) ? keys[lo - 1] + kbase : cr.last_key;
+    for (int p = lo; p < hi; ++p) {
+      const uint32_t k = keys[p] + kbase;
+      if (pn > 0 && prevk < k) {
+        const double g = variance_gain(pn, ps, nc.n - pn, nc.ssum - ps, 0);
+        if (g > tb.g) tb = BestCand{g, k, prevk};
+      }
+      const uint16_t v = vals[p];
+      const int64_t w = (int64_t)wts[v];
+      pn += w;
+      ps += w * ys[v];
+      prevk = k;
+    }
+  }
+  const BestCand b = wave_best(tb);
+  if (t == 0) {
+    carry_best(cr, b);
+    cr.run_n += tot_n;
+    cr.run_s += tot_s;
+    if (n > 0) cr.last_key = keys[n - 1] + kbase;
+  }
+  team_sync<kWave>();
+}
+
+// k_v2_eval for regression: one wave per bucket, the same task order, sorts
+// and walk, with the payloads written to LDS only once the bucket is sorted.
+template <int MAXC, int TASK>
+__global__ void __launch_bounds__(kBlock, 3) k_v2_eval_reg(DataArgs d, LevelArgs lv, V2Args a, int64_t n_tasks) {
+  static_assert(TASK == 1, "regression only");
+  constexpr int IPT = kV2Ipt;
+  constexpr int kSpan = RegWaveLds<IPT>::SPAN;
+  __shared__ RegWaveLds<IPT> s_wave[kWaves];
+  const int t = threadIdx.x & (kWave - 1);
+  RegWaveLds<IPT>& L = s_wave[threadIdx.x / kWave];
+  const int64_t stride = (int64_t)gridDim.x * kWaves;
+  int64_t q = (int64_t)blockIdx.x * kWaves + threadIdx.x / kWave;
+  if (q >= n_tasks) return;  // wave-uniform; no workgroup barrier follows
+  V2Pre<TASK, IPT> cur;
+  v2_prefetch<TASK, IPT>(d, lv, a, q, cur);
+  while (q < n_tasks) {
+    NodeCtx nc;
+    nc.C = d.C;
+    nc.n = cur.n_w;
+    nc.P = cur.P;
+    nc.ssum = cur.ssum;
+    nc.inv = cur.inv;
+    if (t == 0) {
+      L.cr.run[0] = 0u;
+      L.cr.run_n = cur.carry_n;
+      L.cr.run_s = cur.carry_s;
+      L.cr.last_key = cur.carry_last;
+      L.cr.best_g = 0.0;
+      L.cr.best_key = kNone;
+      L.cr.best_prev = kNone;
+    }
+    const int n = cur.n;
+    uint32_t pw[IPT];
+    int64_t py[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      const int e = t + k * kWave;
+      if (e < n) {
+        L.k[0][e] = cur.key[k] - cur.kbase;
+        L.v[0][e] = (uint16_t)e;
+        pw[k] = cur.aux[k] & 0xFFFFu;
+        py[k] = cur.y[k];
+      }
+    }
+    const int64_t gb = cur.gb;
+    const uint32_t kbase = cur.kbase;
+    const int bits = bit_length(cur.bmax - kbase);
+    team_sync<kWave>();
+    const int64_t qn = q + stride;
+    if (qn < n_tasks) v2_prefetch<TASK, IPT>(d, lv, a, qn, cur);
+    int sorted = 0;  // buffer holding the sorted bucket
+    if (bits > 4 && (1 << bits) <= kSpan) {
+      lds_count_sort<IPT, kWave>(L.k[0], L.v[0], L.k[1], L.v[1], n, 1 << bits, reinterpret_cast<uint32_t*>(L.u.cnt),
+                                 L.w);
+      sorted = 1;
+    } else {
+      for (int sh = 0; sh < bits; sh += 4) {
+        lds_sort_pass<IPT, kWave>(L.k[sorted], L.v[sorted], L.k[sorted ^ 1], L.v[sorted ^ 1], n, sh, L.u.cnt, L.w);
+        sorted ^= 1;
+      }
+    }
+    uint32_t* wts = L.k[sorted ^ 1];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+      const int e = t + k * kWave;
+      if (e < n) {
+        wts[e] = pw[k];
+        L.u.y[e] = py[k];
+      }
+    }
+    team_sync<kWave>();
+    eval_sorted_reg_wave<IPT>(L.k[sorted], L.v[sorted], wts, L.u.y, n, kbase, nc, L.cr);
+    if (t == 0) {
+      a.res_g[gb] = L.cr.best_g;
+      a.res_key[gb] = L.cr.best_key;
+      a.res_prev[gb] = L.cr.best_prev;
+    }
+    team_sync<kWave>();
+    q = qn;
+  }
+}
+
 // One workgroup per overfull bucket: fine sub-buckets of the bucket's key
 // range, packed into LDS-sized chunks or read through an LDS histogram
 // (a single sub-bucket above the capacity: heavy ties), walked in order.
@@ -4584,9 +4732,11 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     // and scored; the other pipeline kernels are the price of bucketing.
     if (nt[0]) {
       Timer tm("chunk_eval", profiling_enabled() ? (double)el * (TASK == 0 ? 12.0 : 20.0) : 0.0, s);
-      hipLaunchKernelGGL((k_v2_eval<MAXC, TASK>),
-                         dim3((unsigned)std::min<int64_t>((nt[0] + kWaves - 1) / kWaves, (int64_t)num_cu * 8)),
-                         dim3(kBlock), 0, s, d, lv, a, (int64_t)nt[0]);
+      const dim3 grid((unsigned)std::min<int64_t>((nt[0] + kWaves - 1) / kWaves, (int64_t)num_cu * 8));
+      if constexpr (TASK == 1)
+        hipLaunchKernelGGL((k_v2_eval_reg<MAXC, TASK>), grid, dim3(kBlock), 0, s, d, lv, a, (int64_t)nt[0]);
+      else
+        hipLaunchKernelGGL((k_v2_eval<MAXC, TASK>), grid, dim3(kBlock), 0, s, d, lv, a, (int64_t)nt[0]);
     }
     if (nt[1]) {
       Timer tm("chunk_spill", 0.0, s);
