@@ -1659,9 +1659,10 @@ __device__ __forceinline__ NodeCtx v2_node(const DataArgs& d, const LevelArgs& l
 // One wave per bucket of at most kV2WaveCap entries (four independent
 // buckets per workgroup), grid-stride over the task list.  The loads of a
 // wave's next bucket (task, item and bucket fields, carries, entries) are
-// issued before its current bucket is sorted and walked.  Three workgroups
-// per CU (up to 16 classes: about 53 KB of LDS each, at most 168 VGPRs):
-// the walk is bound by the latency of its x ln x reads.
+// issued before its current bucket is sorted and walked.  Classification:
+// k_v2_eval, three workgroups per CU (up to 16 classes: about 53 KB of LDS
+// each, at most 168 VGPRs), bound by the latency of its x ln x reads;
+// regression: k_v2_eval_reg.
 template <int TASK, int IPT>
 struct V2Pre {
   int64_t gb, e0;
@@ -1710,7 +1711,8 @@ __device__ __forceinline__ void v2_prefetch(const DataArgs& d, const LevelArgs& 
 }
 
 template <int MAXC, int TASK>
-__global__ void __launch_bounds__(kBlock, 3) k_v2_eval(DataArgs d, LevelArgs lv, V2Args a, int64_t n_tasks) {
+__global__ void __launch_bounds__(kBlock, MAXC <= 16 ? 3 : 1) k_v2_eval(DataArgs d, LevelArgs lv, V2Args a, int64_t n_tasks) {
+  static_assert(TASK == 0, "regression buckets go through k_v2_eval_reg");
   constexpr int IPT = kV2Ipt;
   __shared__ double s_inv[128], s_logc[128];
   __shared__ TeamLds<MAXC, TASK, kWave, IPT> s_team[kWaves];
@@ -1731,12 +1733,12 @@ __global__ void __launch_bounds__(kBlock, 3) k_v2_eval(DataArgs d, LevelArgs lv,
     nc.P = cur.P;
     nc.ssum = cur.ssum;
     nc.inv = cur.inv;
-    if (TASK == 0 && t < d.C) {
+    if (t < d.C) {
       L.ncls[t] = cur.ncls;
       L.cr.run[t] = cur.carry_c;
     }
     if (t == 0) {
-      for (int c = TASK == 0 ? d.C : 0; c < MAXC; ++c) L.cr.run[c] = 0u;
+      for (int c = d.C; c < MAXC; ++c) L.cr.run[c] = 0u;
       L.cr.run_n = cur.carry_n;
       L.cr.run_s = cur.carry_s;
       L.cr.last_key = cur.carry_last;
@@ -1752,7 +1754,6 @@ __global__ void __launch_bounds__(kBlock, 3) k_v2_eval(DataArgs d, LevelArgs lv,
         L.u.srt.k[0][e] = cur.key[k] - cur.kbase;
         L.u.srt.v[0][e] = (uint16_t)e;
         L.aux[e] = cur.aux[k];
-        if (TASK == 1) L.y[e] = cur.y[k];
       }
     }
     const int64_t gb = cur.gb;
