@@ -2876,8 +2876,9 @@ __device__ __forceinline__ void ert_seg_body(DataArgs d, LevelArgs lv, ItemArgs 
   }
   if (my_bytes && it.bytes) atomicAdd(it.bytes, my_bytes);
 }
+// Four waves per SIMD (128 VGPRs, no spills; three at the unbounded 131).
 template <int MAXC, int TASK, int SEG>
-__global__ void __launch_bounds__(kBlock) k_search_ert_seg(DataArgs d, LevelArgs lv, ItemArgs it) {
+__global__ void __launch_bounds__(kBlock, 4) k_search_ert_seg(DataArgs d, LevelArgs lv, ItemArgs it) {
   __shared__ double s_inv[128], s_logc[128];
   ert_seg_body<MAXC, TASK, SEG>(d, lv, it, blockIdx.x, gridDim.x, s_inv, s_logc);
 }
@@ -3059,8 +3060,9 @@ struct ErtTiers {
   unsigned long long* bytes[6];
 };
 
+// Three waves per SIMD (168 VGPRs, no spills; two at the unbounded 171).
 template <int MAXC, int TASK>
-__global__ void __launch_bounds__(kBlock) k_search_ert_all(DataArgs d, LevelArgs lv, ItemArgs it, ErtTiers ts) {
+__global__ void __launch_bounds__(kBlock, 3) k_search_ert_all(DataArgs d, LevelArgs lv, ItemArgs it, ErtTiers ts) {
   __shared__ double s_inv[128], s_logc[128];
   const int64_t b = blockIdx.x;
   int k = 0;
