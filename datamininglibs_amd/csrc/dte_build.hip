@@ -5686,6 +5686,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
           if (x) (void)hipGraphExecDestroy(x);
       }
     } gr;
+    const auto cap_t0 = std::chrono::steady_clock::now();
     for (int par = 0; par < 2; ++par) {
       hipGraph_t g = nullptr;
       DTE_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -5701,6 +5702,9 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       (void)hipGraphDestroy(g);
       DTE_HIP(e);
     }
+    if (profiling_enabled())
+      profile_host("graph_capture_host",
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - cap_t0).count());
     int par = 0;
     auto replay = [&](int64_t k) {
       for (int64_t i = 0; i < k; ++i) {
