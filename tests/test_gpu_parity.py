@@ -23,9 +23,10 @@ def test_spam_like_small(algorithm):
 
 
 @pytest.mark.parametrize("algorithm", [capi.EXTRA_TREES, capi.RANDOM_FOREST])
-@pytest.mark.parametrize("c", [3, 10, 24])
+@pytest.mark.parametrize("c", [3, 10, 16, 24])
 def test_multiclass(algorithm, c):
-    # 24 classes take the kernels built for up to 32 classes.
+    # 16 classes fill the class columns of the kernels built for up to 16;
+    # 24 classes take the kernels built for up to 32.
     X, y = small_classes(3, 3000, 16, c)
     g, o = fit_both(X, y, algorithm=algorithm, n_trees=3, k=5, max_depth=20, min_node_size=5, n_classes=c)
     assert_same_forest(g, o)
@@ -40,12 +41,14 @@ def test_rf_big_nodes_bucket_path():
     assert_same_forest(g, o)
 
 
-def test_rf_big_nodes_bucket_path_24_classes():
-    # The bucketed path and the wave and workgroup tiers with the class
-    # columns of up to 32 classes.
-    X, y = small_classes(9, 20000, 12, 24, informative=6)
+@pytest.mark.parametrize("c", [16, 24])
+def test_rf_big_nodes_bucket_path_many_classes(c):
+    # The bucketed path and the wave and workgroup tiers with every class
+    # column of the kernels built for up to 16 classes, and with those of
+    # the kernels built for up to 32.
+    X, y = small_classes(9, 20000, 12, c, informative=6)
     g, o = fit_both(X, y, algorithm=capi.RANDOM_FOREST, n_trees=2, k=3, max_depth=12, min_node_size=5,
-                    n_classes=24)
+                    n_classes=c)
     assert_same_forest(g, o)
 
 
@@ -57,6 +60,21 @@ def test_rf_heavy_ties_histogram_path():
     y = ((X[0] + X[1] + rng.integers(0, 2, 12000)) % 3).astype(np.float32)
     g, o = fit_both(X, y, algorithm=capi.RANDOM_FOREST, n_trees=2, k=3, max_depth=8, min_node_size=5,
                     n_classes=3)
+    assert_same_forest(g, o)
+
+
+def test_rf_regression_big_nodes_ties_and_wide_keys():
+    # 30k rows: the bucket evaluator of regression with narrow key spans
+    # (features of few values: counting sort, overfull buckets) and wide
+    # ones (continuous features: radix passes).
+    rng = np.random.default_rng(21)
+    n = 30000
+    X = np.empty((6, n), dtype=np.float32)
+    X[:3] = rng.integers(0, 40, size=(3, n)).astype(np.float32)
+    X[3:] = rng.standard_normal((3, n)).astype(np.float32)
+    y = (np.sin(X[0] / 7.0) + 0.3 * X[3] + 0.1 * rng.standard_normal(n)).astype(np.float32)
+    g, o = fit_both(X, y, algorithm=capi.RANDOM_FOREST, task=capi.REGRESSION, n_trees=2, k=3, max_depth=10,
+                    min_node_size=5)
     assert_same_forest(g, o)
 
 
