@@ -4061,17 +4061,23 @@ __global__ void k_nonzero(const uint32_t* mult, int64_t n, uint32_t* flag) {
 }
 
 // Compact in-bag rows into entries (ascending within each tree) and
-// accumulate each root's statistics.
-__global__ void k_boot_compact(const uint32_t* mult, const uint32_t* pos, int64_t N, const uint32_t* label,
-                               const int64_t* yfix, int C, uint32_t* row, uint32_t* aux, int64_t* ey,
-                               uint32_t* root_cls, int64_t* root_sum, uint64_t* root_hash,
-                               uint32_t* overflow) {
-  __shared__ uint32_t s_cls[32];
+// accumulate each root's statistics.  A few workgroups per tree (grid-stride
+// over its rows), class counts in per-thread LDS columns: the root's
+// counters then take a handful of global atomics per workgroup instead of
+// one per row range, and no two lanes add into one LDS word.
+constexpr int kBootCompactBlocks = 32;  // workgroups per tree
+__global__ void __launch_bounds__(kBlock) k_boot_compact(const uint32_t* mult, const uint32_t* pos, int64_t N,
+                                                         const uint32_t* label, const int64_t* yfix, int C,
+                                                         uint32_t* row, uint32_t* aux, int64_t* ey,
+                                                         uint32_t* root_cls, int64_t* root_sum, uint64_t* root_hash,
+                                                         uint32_t* overflow) {
+  __shared__ uint32_t s_cls[32 * kBlock];  // [class * kBlock + thread]
   const int tr = blockIdx.y;
   const uint32_t* mt = mult + (int64_t)tr * N;
   const uint32_t* pt = pos + (int64_t)tr * N;
-  if (threadIdx.x < 32) s_cls[threadIdx.x] = 0;
-  __syncthreads();
+  const int t = threadIdx.x;
+  if (!yfix)
+    for (int c = 0; c < C; ++c) s_cls[c * kBlock + t] = 0;
   uint64_t hsh = 0;
   int64_t sm = 0;
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < N; r += (int64_t)gridDim.x * blockDim.x) {
@@ -4086,13 +4092,16 @@ __global__ void k_boot_compact(const uint32_t* mult, const uint32_t* pos, int64_
       ey[p] = yfix[r];
       sm += (int64_t)w * yfix[r];
     } else {
-      atomicAdd(&s_cls[lab], w);
+      s_cls[lab * kBlock + t] += w;
     }
     hsh += (uint64_t)w * mix64((uint64_t)r);
   }
   __syncthreads();
-  if (!yfix && (int)threadIdx.x < C && s_cls[threadIdx.x])
-    atomicAdd(&root_cls[(int64_t)tr * C + threadIdx.x], s_cls[threadIdx.x]);
+  if (!yfix && t < C) {
+    uint32_t v = 0;
+    for (int i = 0; i < kBlock; ++i) v += s_cls[t * kBlock + i];
+    if (v) atomicAdd(&root_cls[(int64_t)tr * C + t], v);
+  }
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
     hsh += __shfl_xor(hsh, off, kWave);
@@ -5067,7 +5076,8 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     }
     DevBuf<uint32_t> overflow(1);
     DTE_HIP(hipMemsetAsync(overflow.p, 0, 4, s));
-    hipLaunchKernelGGL(k_boot_compact, dim3(grid_for(N, kBlock, 4096), T), dim3(kBlock), 0, s, mult.p, pos.p, N,
+    hipLaunchKernelGGL(k_boot_compact, dim3(grid_for(N, kBlock, kBootCompactBlocks), T), dim3(kBlock), 0, s, mult.p,
+                       pos.p, N,
                        label, yfix, C, rowA.p, auxA.p, eyA.p, root_cls.p, root_sum.p, root_hash.p,
                        overflow.p);
     if (read_one(overflow.p, s)) fail(DTE_ERR_INTERNAL, "bootstrap multiplicity above 65535");
