@@ -3124,6 +3124,9 @@ __device__ __forceinline__ int tier_of(int m, const Cuts& cuts) {
 
 // Each wave owns a contiguous range of the list: it counts its tiers, takes
 // one slot range per tier with a single atomic, then writes its entries.
+// Workgroups of a tier split: every wave adds its tier counts with one
+// global atomic per tier, so the grid bounds the atomics on those few words.
+constexpr int kTierGrid = 256;
 __global__ void k_tier_split(const uint32_t* list, int64_t n, const uint32_t* open_idx, const int32_t* node_m, Cuts cuts,
                              TierLists out, unsigned int* counts, const unsigned int* n_dev, const int64_t* off_dev) {
   if (n_dev) n = *n_dev;            // device-described level
@@ -5317,7 +5320,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
   auto run_phase = [&](ItemArgs it) {
     for (auto& tl : tier_list) tl.reserve(it.n_groups);
     DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
-    hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(it.n_groups), 1024)), dim3(kBlock), 0, s, it.list, it.n_groups,
+    hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(it.n_groups), kTierGrid)), dim3(kBlock), 0, s, it.list, it.n_groups,
                        lv.open_idx, lv.m, cuts, tier_lists(), tier_cnt.p, nullptr, nullptr);
     unsigned int cnt[kMaxTiers];
     DTE_HIP(hipMemcpyAsync(cnt, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
@@ -5548,7 +5551,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       // split search of one phase over the tiers; nodes above cut_block
       // (tier 5) take one workgroup per item through its scratch slab
       auto phase = [&](const uint32_t* list, const unsigned int* n_dev, unsigned int* cnt, int per_group, int j0) {
-        hipLaunchKernelGGL(k_tier_split, dim3(kElemGrid), dim3(kBlock), 0, s, list, (int64_t)0,
+        hipLaunchKernelGGL(k_tier_split, dim3(kTierGrid), dim3(kBlock), 0, s, list, (int64_t)0,
                            (const uint32_t*)open_idx.p, (const int32_t*)o_m.p, cuts, tier_lists(), cnt, n_dev,
                            (const int64_t*)&L->off);
         if (ert_merged) {
@@ -5618,7 +5621,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       pa.l_hash = l_hash.p;
       pa.split_ok = split_ok.p;
       pa.bytes = graph_bytes.p + kMaxTiers;
-      hipLaunchKernelGGL(k_tier_split, dim3(kElemGrid), dim3(kBlock), 0, s, (const uint32_t*)nullptr, (int64_t)0,
+      hipLaunchKernelGGL(k_tier_split, dim3(kTierGrid), dim3(kBlock), 0, s, (const uint32_t*)nullptr, (int64_t)0,
                          (const uint32_t*)open_idx.p, (const int32_t*)o_m.p, kPartCuts, tier_lists(), L->ptier,
                          (const unsigned int*)&L->n_open, (const int64_t*)&L->off);
       PartArgs p0 = pa, p1 = pa, p2 = pa, p3 = pa;
@@ -5923,7 +5926,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       } else {
         for (auto& tl : tier_list) tl.reserve(n_open);
         DTE_HIP(hipMemsetAsync(tier_cnt.p, 0, 4 * kMaxTiers, s));
-        hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(n_open), 1024)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
+        hipLaunchKernelGGL(k_tier_split, dim3(std::min(grid_for(n_open), kTierGrid)), dim3(kBlock), 0, s, (const uint32_t*)nullptr,
                            n_open, lv.open_idx, lv.m, kPartCutsHost, tier_lists(), tier_cnt.p, nullptr, nullptr);
         unsigned int pc[kMaxTiers];
         DTE_HIP(hipMemcpyAsync(pc, tier_cnt.p, 4 * kMaxTiers, hipMemcpyDeviceToHost, s));
