@@ -4747,7 +4747,10 @@ void big_pipeline(BigState& B, const DataArgs& d, const LevelArgs& lv, const Ite
     // and scored; the other pipeline kernels are the price of bucketing.
     if (nt[0]) {
       Timer tm("chunk_eval", profiling_enabled() ? (double)el * (TASK == 0 ? 12.0 : 20.0) : 0.0, s);
-      const dim3 grid((unsigned)std::min<int64_t>((nt[0] + kWaves - 1) / kWaves, (int64_t)num_cu * 8));
+      // Two full rounds of the three workgroups a CU holds: grid-stride
+      // over similar buckets, so no partial last round (8 per CU, 2.67
+      // rounds, was 4.5 % slower).
+      const dim3 grid((unsigned)std::min<int64_t>((nt[0] + kWaves - 1) / kWaves, (int64_t)num_cu * 6));
       if constexpr (TASK == 1)
         hipLaunchKernelGGL((k_v2_eval_reg<MAXC, TASK>), grid, dim3(kBlock), 0, s, d, lv, a, (int64_t)nt[0]);
       else
