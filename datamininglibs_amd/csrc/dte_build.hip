@@ -4263,8 +4263,9 @@ __global__ void k_compact_list_dev(const uint32_t* flag, const uint32_t* pos, co
 
 // Exclusive scan of n u32 (n read from the device) into out[0..n], out[n] =
 // total, also stored to *total: kDScanBlocks workgroups each own a
-// contiguous range (per-range sums, one workgroup scans the sums, then every
-// range is rescanned with its offset in tiles of kScanTile).
+// contiguous range (per-range sums; then every range sums the sums before
+// it for its offset and is rescanned in tiles of kScanTile).  Two kernels:
+// a level step is a chain of short launches, each a fixed latency.
 constexpr int kDScanBlocks = 512;
 
 __device__ __forceinline__ int64_t dscan_n(const int64_t* n64, const unsigned int* n32) {
@@ -4284,29 +4285,22 @@ __global__ void __launch_bounds__(kBlock) k_dscan_sums(const uint32_t* in, const
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(kBlock) k_dscan_top(uint32_t* sums, uint32_t* out, const int64_t* n64,
-                                                      const unsigned int* n32, unsigned int* total) {
-  static_assert(kDScanBlocks == 2 * kBlock, "two sums per thread");
-  __shared__ uint32_t s_w[kWaves];
-  const int t = threadIdx.x;
-  const uint32_t a = sums[2 * t], b = sums[2 * t + 1];
-  uint32_t tot;
-  const uint32_t base = block_excl_scan(a + b, s_w, &tot);
-  sums[2 * t] = base;
-  sums[2 * t + 1] = base + a;
-  if (t == 0) {
-    out[dscan_n(n64, n32)] = tot;
-    if (total) *total = tot;
-  }
-}
-
 __global__ void __launch_bounds__(kBlock) k_dscan_apply(const uint32_t* in, const int64_t* n64,
-                                                        const unsigned int* n32, const uint32_t* offs, uint32_t* out) {
+                                                        const unsigned int* n32, const uint32_t* sums, uint32_t* out,
+                                                        unsigned int* total) {
   __shared__ uint32_t s_w[kWaves];
   const int64_t n = dscan_n(n64, n32);
   const int64_t per = (n + kDScanBlocks - 1) / kDScanBlocks;
   const int64_t lo = blockIdx.x * per, hi = min(lo + per, n);
-  uint32_t carry = offs[blockIdx.x];
+  uint32_t part = 0;
+  for (int i = threadIdx.x; i < (int)blockIdx.x; i += kBlock) part += sums[i];
+  uint32_t carry;
+  (void)block_excl_scan(part, s_w, &carry);  // sum of the ranges before this one
+  if (blockIdx.x == kDScanBlocks - 1 && threadIdx.x == 0) {
+    const uint32_t tot = carry + sums[blockIdx.x];
+    out[n] = tot;
+    if (total) *total = tot;
+  }
   for (int64_t t0 = lo; t0 < hi; t0 += kScanTile) {
     uint32_t v[16];
     uint32_t sum = 0;
@@ -5504,8 +5498,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     auto dscan = [&](const uint32_t* in, const int64_t* n64, const unsigned int* n32, uint32_t* out,
                      unsigned int* total) {
       hipLaunchKernelGGL(k_dscan_sums, dim3(kDScanBlocks), dim3(kBlock), 0, s, in, n64, n32, dsums.p);
-      hipLaunchKernelGGL(k_dscan_top, dim3(1), dim3(kBlock), 0, s, dsums.p, out, n64, n32, total);
-      hipLaunchKernelGGL(k_dscan_apply, dim3(kDScanBlocks), dim3(kBlock), 0, s, in, n64, n32, dsums.p, out);
+      hipLaunchKernelGGL(k_dscan_apply, dim3(kDScanBlocks), dim3(kBlock), 0, s, in, n64, n32, dsums.p, out, total);
     };
     // One level step; par = 1 swaps the ping-pong buffers.
     auto enqueue = [&](int par) {
