@@ -30,6 +30,7 @@
 #include <cstring>
 #include <numeric>
 #include <thread>
+#include <type_traits>
 #include <cstdlib>
 #include <chrono>
 #include <memory>
@@ -4242,12 +4243,6 @@ __global__ void k_predict_nodes(int64_t n, int C, int task, const int32_t* featu
 // Partition tiers: 16- and 32-lane segments, a wave, a workgroup per node.
 constexpr Cuts kPartCuts{{16, 32, 512, 0x7FFFFFFF, 0, 0}, 4};
 
-__global__ void k_open_flags_dev(const uint8_t* open, const DevLevel* L, uint32_t* flag) {
-  const int64_t n = L->n;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    flag[i] = open[i] ? 1u : 0u;
-}
-
 __global__ void k_open_list_dev(const uint8_t* open, const uint32_t* pos, const DevLevel* L, uint32_t* list) {
   const int64_t n = L->n;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -4272,7 +4267,10 @@ __device__ __forceinline__ int64_t dscan_n(const int64_t* n64, const unsigned in
   return n64 ? *n64 : (int64_t)*n32;
 }
 
-__global__ void __launch_bounds__(kBlock) k_dscan_sums(const uint32_t* in, const int64_t* n64,
+// In: uint32_t counts, or uint8_t flags (a level's open flags, scanned as
+// they are).
+template <class In>
+__global__ void __launch_bounds__(kBlock) k_dscan_sums(const In* in, const int64_t* n64,
                                                        const unsigned int* n32, uint32_t* sums) {
   __shared__ uint32_t s_w[kWaves];
   const int64_t n = dscan_n(n64, n32);
@@ -4285,7 +4283,8 @@ __global__ void __launch_bounds__(kBlock) k_dscan_sums(const uint32_t* in, const
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(kBlock) k_dscan_apply(const uint32_t* in, const int64_t* n64,
+template <class In>
+__global__ void __launch_bounds__(kBlock) k_dscan_apply(const In* in, const int64_t* n64,
                                                         const unsigned int* n32, const uint32_t* sums, uint32_t* out,
                                                         unsigned int* total) {
   __shared__ uint32_t s_w[kWaves];
@@ -4307,7 +4306,7 @@ __global__ void __launch_bounds__(kBlock) k_dscan_apply(const uint32_t* in, cons
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int64_t idx = t0 + (int64_t)threadIdx.x * 16 + k;
-      v[k] = idx < hi ? in[idx] : 0u;
+      v[k] = idx < hi ? (uint32_t)in[idx] : 0u;
       sum += v[k];
     }
     uint32_t tile_tot;
@@ -5442,7 +5441,6 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     }
     fit_to(w_start_next, NL + 1);
     fit_to(w_open_next, NL + 1);
-    fit_to(flag32, NL);
     fit_to(open_pos, NL + 1);
     fit_to(open_idx, NO + 1);
     fit_to(res_score, NO * D);
@@ -5495,10 +5493,12 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
     // likewise the partition tiers of a step (both algorithms)
     const PartAllFn partition_all = pick_part_all(task, C);
 
-    auto dscan = [&](const uint32_t* in, const int64_t* n64, const unsigned int* n32, uint32_t* out,
+    auto dscan = [&](const auto* in, const int64_t* n64, const unsigned int* n32, uint32_t* out,
                      unsigned int* total) {
-      hipLaunchKernelGGL(k_dscan_sums, dim3(kDScanBlocks), dim3(kBlock), 0, s, in, n64, n32, dsums.p);
-      hipLaunchKernelGGL(k_dscan_apply, dim3(kDScanBlocks), dim3(kBlock), 0, s, in, n64, n32, dsums.p, out, total);
+      using In = std::remove_cv_t<std::remove_pointer_t<decltype(in)>>;
+      hipLaunchKernelGGL(k_dscan_sums<In>, dim3(kDScanBlocks), dim3(kBlock), 0, s, in, n64, n32, dsums.p);
+      hipLaunchKernelGGL(k_dscan_apply<In>, dim3(kDScanBlocks), dim3(kBlock), 0, s, in, n64, n32, dsums.p, out,
+                         total);
     };
     // One level step; par = 1 swaps the ping-pong buffers.
     auto enqueue = [&](int par) {
@@ -5517,8 +5517,7 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       int64_t* cey = par ? nxt_ey : cur_ey;
       int64_t* ney = par ? cur_ey : nxt_ey;
       // open list
-      hipLaunchKernelGGL(k_open_flags_dev, dim3(kElemGrid), dim3(kBlock), 0, s, co, (const DevLevel*)L, flag32.p);
-      dscan(flag32.p, &L->n, nullptr, open_pos.p, &L->n_open);
+      dscan((const uint8_t*)co, &L->n, nullptr, open_pos.p, &L->n_open);
       hipLaunchKernelGGL(k_open_list_dev, dim3(kElemGrid), dim3(kBlock), 0, s, co, open_pos.p, (const DevLevel*)L,
                          open_idx.p);
       LevelArgs lvd{};
