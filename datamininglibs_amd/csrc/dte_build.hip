@@ -4324,8 +4324,7 @@ __global__ void __launch_bounds__(kBlock) k_dscan_apply(const In* in, const int6
 
 // End of a level step: the next level becomes current; its counters start
 // at zero (a kernel, so that a captured step holds no memset node).
-__global__ void k_level_advance(DevLevel* L) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void level_advance(DevLevel* L) {
   L->off += L->n;
   L->n = L->n_next;
   L->depth += 1;
@@ -4333,6 +4332,18 @@ __global__ void k_level_advance(DevLevel* L) {
   L->n_retry = 0;
   L->n_next = 0;
   for (int k = 0; k < kMaxTiers; ++k) L->tier[k] = L->rtier[k] = L->ptier[k] = 0;
+}
+
+// Last kernel of a device level step: the per-tree cursors of the next level
+// (k_tree_update's work) and, by one thread, the level advance.  Nothing in
+// the kernel reads L, so the advance cannot race with the tree update.
+__global__ void k_tree_update_advance(int n_trees, const uint32_t* cpos, const uint32_t* lf, const uint32_t* base,
+                                      uint32_t* lf_next, uint32_t* base_next, DevLevel* L) {
+  const int tr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tr == 0) level_advance(L);
+  if (tr > n_trees) return;
+  lf_next[tr] = cpos[lf[tr]];
+  if (tr < n_trees) base_next[tr] = base[tr] + (lf[tr + 1] - lf[tr]);
 }
 
 // ------------------------------------------------------ small helpers ---
@@ -5681,9 +5692,8 @@ Forest* fit_batch(FitData& fd, int t0, int T, int64_t N, int32_t F, const dte_pa
       hipLaunchKernelGGL(k_children_count, dim3(kElemGrid), dim3(kBlock), 0, s, ca);
       dscan(nchild.p, &L->n, nullptr, cpos.p, &L->n_next);
       hipLaunchKernelGGL(k_children_write, dim3(kElemGrid), dim3(kBlock), 0, s, d, ca);
-      hipLaunchKernelGGL(k_tree_update, dim3(grid_for(T + 1)), dim3(kBlock), 0, s, T, (const uint32_t*)cpos.p,
-                         (const uint32_t*)clf, (const uint32_t*)cbase, nlf, nbase);
-      hipLaunchKernelGGL(k_level_advance, dim3(1), dim3(kWave), 0, s, L);
+      hipLaunchKernelGGL(k_tree_update_advance, dim3(grid_for(T + 1)), dim3(kBlock), 0, s, T,
+                         (const uint32_t*)cpos.p, (const uint32_t*)clf, (const uint32_t*)cbase, nlf, nbase, L);
       DTE_HIP(hipGetLastError());
     };
 
